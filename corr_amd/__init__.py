@@ -1,0 +1,11 @@
+"""corr_amd: MI355X-native post-HF contraction path of corr.
+
+The compute lives in libcorr_gpu.so (hand-written gfx950 HIP kernels behind
+the C-ABI of include/corr_abi.h); this package is the PyTorch-ROCm front-end.
+"""
+from ._abi import (CorrError, CorrLibraryError, LIB_PATH, PREC_BF16X3, PREC_FP32, PREC_FP64,  # noqa: F401
+                   lib)
+from .posthf import (MOtrans, ao_unfold, build_FMo, calculate_ri, dgemm, ri_form_bia,  # noqa: F401
+                     ri_metric, rimp2_energy, run_canonical_mp2, run_canonical_mp2_ri, transform_ri)
+
+__version__ = "0.1.0"

@@ -1,0 +1,106 @@
+// Shared host/device infrastructure of libcorr_gpu.so: the context, status
+// handling, cached device scratch and small device helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <map>
+#include <vector>
+#include "../../include/corr_abi.h"
+
+struct corr_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    struct Buf { void* p = nullptr; size_t bytes = 0; };
+    std::map<std::string, Buf> scratch;
+    void* blas_handle = nullptr;   // rocBLAS handle used by the eigen metric
+    bool timing = false;
+    struct Timed { std::string name; hipEvent_t start, stop; };
+    std::vector<Timed> pending;                  // recorded, not yet read
+    std::map<std::string, std::pair<double, int64_t>> kernel_ms;
+};
+
+namespace corr {
+
+int set_error(corr_ctx* ctx, int code, const std::string& msg);
+
+// Device scratch owned by the context, reused across calls (grown on demand).
+int scratch(corr_ctx* ctx, const char* name, size_t bytes, void** out);
+
+// Brackets one kernel launch with events when the context's timing is on.
+struct KernelTimer {
+    corr_ctx* ctx;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelTimer(corr_ctx* c, const char* n) : ctx(c), name(n) {
+        if (ctx->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+            (void)hipEventRecord(a, ctx->stream);
+        else
+            a = b = nullptr;
+    }
+    void stop() {
+        if (a && b) {
+            (void)hipEventRecord(b, ctx->stream);
+            ctx->pending.push_back({name, a, b});
+            a = b = nullptr;
+        }
+    }
+    ~KernelTimer() { stop(); }
+};
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t rup(int64_t a, int64_t b) { return cdiv(a, b) * b; }
+
+// GEMM launcher shared by all modules (row-major, see corr_dgemm).
+// a_lower: op(A) is lower triangular (entries with k > m are zero and
+// skipped); only used for the Cholesky-metric application.
+int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
+         double alpha, const double* A, int64_t lda, int64_t sA,
+         const double* B, int64_t ldb, int64_t sB, double beta, double* C,
+         int64_t ldc, int64_t sC, int64_t batch, int prec, bool a_lower = false);
+
+// Deterministic sum of n doubles (device) into *h_out (host), n2 interleaved
+// series: data[i*nser + s].
+int reduce_sum(corr_ctx* ctx, const double* d_data, int64_t n, int nser, double* h_out);
+
+}  // namespace corr
+
+#define CORR_CHECK_CTX(ctx)                                         \
+    do {                                                            \
+        if (!(ctx)) return CORR_EINVAL;                             \
+        hipError_t e_ = hipSetDevice((ctx)->device);                \
+        if (e_ != hipSuccess)                                       \
+            return corr::set_error(ctx, CORR_EHIP,                  \
+                std::string("hipSetDevice: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CORR_HIP(ctx, expr)                                         \
+    do {                                                            \
+        hipError_t e_ = (expr);                                     \
+        if (e_ != hipSuccess)                                       \
+            return corr::set_error(ctx, CORR_EHIP,                  \
+                std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CORR_LAUNCHED(ctx, what)                                    \
+    do {                                                            \
+        hipError_t e_ = hipGetLastError();                          \
+        if (e_ != hipSuccess)                                       \
+            return corr::set_error(ctx, CORR_EHIP,                  \
+                std::string("launch ") + (what) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CORR_TRY(expr)                                              \
+    do { int s_ = (expr); if (s_ != CORR_OK) return s_; } while (0)
+
+#define CORR_REQUIRE(ctx, cond, msg)                                \
+    do { if (!(cond)) return corr::set_error(ctx, CORR_EINVAL, msg); } while (0)
+
+// MFMA operand/accumulator vector types for gfx950.
+typedef double corr_d4 __attribute__((ext_vector_type(4)));
+typedef float corr_f4 __attribute__((ext_vector_type(4)));
+typedef short corr_s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 corr_bf8 __attribute__((ext_vector_type(8)));

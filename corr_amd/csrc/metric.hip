@@ -1,0 +1,258 @@
+// RI metric factorisation (the arithmetic of calculate_ri, src/POSTHF/ri.cc:50-69).
+//
+// CORR_METRIC_EIGEN:    J = V diag(l) V^T (rocSOLVER dsyevd, FP64), then
+//                       J^-1/2 = V diag(s) V^T with s = 0 where l/l_max < 1e-10
+//                       or l < 0, else l^-1/2  (ri.cc:56-69).
+// CORR_METRIC_CHOLESKY: J = L L^T by a blocked right-looking Cholesky on the
+//                       GPU (64x64 diagonal blocks factored and inverted in
+//                       LDS, panel and trailing updates on the MFMA GEMM),
+//                       then L^-1 by blocked forward substitution.  Both give
+//                       M with M^T M = J^-1, hence identical B^T B products.
+#include "common.hpp"
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+#include <vector>
+#include <cmath>
+
+namespace corr {
+
+namespace {
+constexpr int NB = 64;
+
+// Factor the kb x kb diagonal block at A[k0,k0] (lda) in place (lower) and
+// write its inverse (lower, kb x kb, ld NB) to Linv.  Sets *notpd on failure.
+__global__ __launch_bounds__(256) void chol_diag_kernel(double* __restrict__ A, int64_t lda,
+                                                        int64_t k0, int kb,
+                                                        double* __restrict__ Linv,
+                                                        int* __restrict__ notpd) {
+    __shared__ double S[NB][NB + 1];
+    __shared__ double X[NB][NB + 1];
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int r = e / NB, c = e % NB;
+        S[r][c] = (r < kb && c < kb) ? A[(k0 + r) * lda + (k0 + c)] : 0.0;
+        X[r][c] = 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+        if (tid == 0) {
+            const double d = S[j][j];
+            if (!(d > 0.0)) { bad = 1; S[j][j] = 1.0; }
+            else S[j][j] = sqrt(d);
+        }
+        __syncthreads();
+        const double djj = S[j][j];
+        for (int i = j + 1 + tid; i < kb; i += 256) S[i][j] /= djj;
+        __syncthreads();
+        const int m = kb - j - 1;
+        for (int e = tid; e < m * m; e += 256) {
+            const int i = j + 1 + e / m, l = j + 1 + e % m;
+            if (l <= i) S[i][l] -= S[i][j] * S[l][j];
+        }
+        __syncthreads();
+    }
+    // Inverse of the lower factor, one column per thread.
+    if (tid < kb) {
+        const int c = tid;
+        X[c][c] = 1.0 / S[c][c];
+        for (int i = c + 1; i < kb; ++i) {
+            double s = 0.0;
+            for (int k = c; k < i; ++k) s += S[i][k] * X[k][c];
+            X[i][c] = -s / S[i][i];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < kb * kb; e += 256) {
+        const int r = e / kb, c = e % kb;
+        A[(k0 + r) * lda + (k0 + c)] = (c <= r) ? S[r][c] : 0.0;
+        Linv[r * NB + c] = (c <= r) ? X[r][c] : 0.0;
+    }
+    if (tid == 0 && bad) *notpd = 1;
+}
+
+__global__ void copy2d_kernel(const double* __restrict__ src, int64_t lds, double* __restrict__ dst,
+                              int64_t ldd, int64_t rows, int64_t cols) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < rows * cols;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / cols, c = e % cols;
+        dst[r * ldd + c] = src[r * lds + c];
+    }
+}
+
+__global__ void zero_upper_kernel(double* __restrict__ A, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / n, c = e % n;
+        if (c > r) A[e] = 0.0;
+    }
+}
+
+__global__ void fill_kernel(double* __restrict__ A, int64_t n, double v) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        A[e] = v;
+}
+
+// rows of W scaled: W[c][:] *= s(lambda_c) with the reference cutoff.
+__global__ void eig_scale_kernel(double* __restrict__ W, const double* __restrict__ lam, int64_t n,
+                                 int64_t* __restrict__ nkept) {
+    const double lmax = lam[n - 1];
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n;
+        const double l = lam[c];
+        const double s = (l / lmax < 1.0e-10 || l < 0.0) ? 0.0 : 1.0 / sqrt(l);
+        W[e] *= s;
+        if (e % n == 0 && s != 0.0) atomicAdd((unsigned long long*)nkept, 1ull);
+    }
+}
+
+unsigned grid_for(int64_t n) {
+    int64_t g = cdiv(n, 256);
+    return (unsigned)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
+}
+}  // namespace
+
+static int cholesky_inverse(corr_ctx* ctx, int64_t n, const double* J, double* M) {
+    double *A = nullptr, *P = nullptr, *Linv = nullptr, *R = nullptr;
+    int* flag = nullptr;
+    CORR_TRY(scratch(ctx, "chol_A", sizeof(double) * n * n, (void**)&A));
+    CORR_TRY(scratch(ctx, "chol_P", sizeof(double) * n * NB, (void**)&P));
+    CORR_TRY(scratch(ctx, "chol_R", sizeof(double) * n * NB, (void**)&R));
+    const int64_t nblk = cdiv(n, NB);
+    CORR_TRY(scratch(ctx, "chol_Linv", sizeof(double) * NB * NB * nblk, (void**)&Linv));
+    CORR_TRY(scratch(ctx, "chol_flag", sizeof(int) * 4, (void**)&flag));
+    CORR_HIP(ctx, hipMemcpyAsync(A, J, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    CORR_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+    for (int64_t kb0 = 0; kb0 < nblk; ++kb0) {
+        const int64_t k0 = kb0 * NB;
+        const int kb = (int)std::min<int64_t>(NB, n - k0);
+        double* Li = Linv + kb0 * NB * NB;
+        hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, ctx->stream, A, n, k0, kb, Li, flag);
+        CORR_LAUNCHED(ctx, "chol_diag_kernel");
+        const int64_t m = n - k0 - kb;
+        if (m <= 0) break;
+        // Panel: L[k0+kb:, k0:k0+kb] = A[...] * Linv^T
+        double* Ap = A + (k0 + kb) * n + k0;
+        CORR_TRY(gemm(ctx, 0, 1, m, kb, kb, 1.0, Ap, n, 0, Li, NB, 0, 0.0, P, NB, 0, 1, CORR_PREC_FP64));
+        hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for(m * kb)), dim3(256), 0, ctx->stream, P,
+                           (int64_t)NB, Ap, n, m, (int64_t)kb);
+        CORR_LAUNCHED(ctx, "copy2d_kernel");
+        // Trailing update: A22 -= P P^T
+        double* A22 = A + (k0 + kb) * n + (k0 + kb);
+        CORR_TRY(gemm(ctx, 0, 1, m, m, kb, -1.0, P, NB, 0, P, NB, 0, 1.0, A22, n, 0, 1, CORR_PREC_FP64));
+    }
+    hipLaunchKernelGGL(zero_upper_kernel, dim3(grid_for(n * n)), dim3(256), 0, ctx->stream, A, n);
+    CORR_LAUNCHED(ctx, "zero_upper_kernel");
+    // X = L^-1 by block rows: X_k = Linv_kk * (I_k - L[k, :k] X[:k, :])
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n * n)), dim3(256), 0, ctx->stream, M, n * n, 0.0);
+    CORR_LAUNCHED(ctx, "fill_kernel");
+    for (int64_t kb0 = 0; kb0 < nblk; ++kb0) {
+        const int64_t k0 = kb0 * NB;
+        const int kb = (int)std::min<int64_t>(NB, n - k0);
+        const int64_t w = k0 + kb;   // X[k, c] = 0 for c >= w
+        hipLaunchKernelGGL(fill_kernel, dim3(grid_for((int64_t)kb * NB)), dim3(256), 0, ctx->stream,
+                           R, (int64_t)kb * NB, 0.0);
+        CORR_LAUNCHED(ctx, "fill_kernel");
+        // R = -L[k, :k0] X[:k0, :k0]  (kb x k0), then + I on the diagonal block
+        if (k0 > 0) {
+            CORR_TRY(gemm(ctx, 0, 0, kb, k0, k0, -1.0, A + k0 * n, n, 0, M, n, 0, 0.0, R, w, 0, 1,
+                          CORR_PREC_FP64));
+        }
+        // X[k, :w] = Linv_kk * R  with R's identity block folded in:
+        // X[k, :k0] = Linv * R[:, :k0];  X[k, k0:w] = Linv
+        if (k0 > 0)
+            CORR_TRY(gemm(ctx, 0, 0, kb, k0, kb, 1.0, Linv + kb0 * NB * NB, NB, 0, R, w, 0, 0.0,
+                          M + k0 * n, n, 0, 1, CORR_PREC_FP64));
+        hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for((int64_t)kb * kb)), dim3(256), 0, ctx->stream,
+                           Linv + kb0 * NB * NB, (int64_t)NB, M + k0 * n + k0, n, (int64_t)kb,
+                           (int64_t)kb);
+        CORR_LAUNCHED(ctx, "copy2d_kernel");
+    }
+    int h_flag = 0;
+    CORR_HIP(ctx, hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_flag) return set_error(ctx, CORR_ENOTPD, "ri metric: J is not positive definite");
+    return CORR_OK;
+}
+
+static int blas_handle(corr_ctx* ctx, rocblas_handle* h) {
+    if (!ctx->blas_handle) {
+        rocblas_handle hh = nullptr;
+        if (rocblas_create_handle(&hh) != rocblas_status_success)
+            return set_error(ctx, CORR_EHIP, "rocblas_create_handle failed");
+        ctx->blas_handle = hh;
+    }
+    *h = (rocblas_handle)ctx->blas_handle;
+    if (rocblas_set_stream(*h, ctx->stream) != rocblas_status_success)
+        return set_error(ctx, CORR_EHIP, "rocblas_set_stream failed");
+    return CORR_OK;
+}
+
+static int eigen_invsqrt(corr_ctx* ctx, int64_t n, const double* J, double* M, int64_t* nkept) {
+    CORR_REQUIRE(ctx, n < (1 << 30), "ri metric: naux too large for dsyevd");
+    double *W = nullptr, *lam = nullptr, *E = nullptr;
+    int* info = nullptr;
+    int64_t* kept = nullptr;
+    CORR_TRY(scratch(ctx, "eig_W", sizeof(double) * n * n, (void**)&W));
+    CORR_TRY(scratch(ctx, "eig_lam", sizeof(double) * n, (void**)&lam));
+    CORR_TRY(scratch(ctx, "eig_E", sizeof(double) * n, (void**)&E));
+    CORR_TRY(scratch(ctx, "eig_info", 64, (void**)&info));
+    kept = (int64_t*)(info + 2);
+    CORR_HIP(ctx, hipMemcpyAsync(W, J, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    CORR_HIP(ctx, hipMemsetAsync(info, 0, 64, ctx->stream));
+    rocblas_handle h;
+    CORR_TRY(blas_handle(ctx, &h));
+    // J is symmetric, so its row-major buffer is also its column-major form.
+    // On return the buffer holds the eigenvectors as columns (column-major),
+    // i.e. row c of the row-major view is eigenvector c.
+    if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, W,
+                         (rocblas_int)n, lam, E, (rocblas_int*)info) != rocblas_status_success)
+        return set_error(ctx, CORR_EHIP, "rocsolver_dsyevd failed");
+    int h_info = 0;
+    CORR_HIP(ctx, hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_info != 0) return set_error(ctx, CORR_ENOTCONV, "rocsolver_dsyevd did not converge");
+    // Wt = rows scaled copy; M = W^T diag(s) W
+    double* Ws = nullptr;
+    CORR_TRY(scratch(ctx, "eig_Ws", sizeof(double) * n * n, (void**)&Ws));
+    CORR_HIP(ctx, hipMemcpyAsync(Ws, W, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(eig_scale_kernel, dim3(grid_for(n * n)), dim3(256), 0, ctx->stream, Ws, lam, n, kept);
+    CORR_LAUNCHED(ctx, "eig_scale_kernel");
+    CORR_TRY(gemm(ctx, 1, 0, n, n, n, 1.0, W, n, 0, Ws, n, 0, 0.0, M, n, 0, 1, CORR_PREC_FP64));
+    if (nkept) {
+        CORR_HIP(ctx, hipMemcpyAsync(nkept, kept, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return CORR_OK;
+}
+
+int ri_metric(corr_ctx* ctx, int64_t naux, const double* J, int mode, double* M, int64_t* nkept) {
+    CORR_REQUIRE(ctx, naux > 0 && J && M, "ri metric: bad arguments");
+    CORR_REQUIRE(ctx, J != M, "ri metric: J and M must not alias");
+    if (mode == CORR_METRIC_CHOLESKY) {
+        CORR_TRY(cholesky_inverse(ctx, naux, J, M));
+        if (nkept) *nkept = naux;
+        return CORR_OK;
+    }
+    CORR_REQUIRE(ctx, mode == CORR_METRIC_EIGEN, "ri metric: unknown mode");
+    return eigen_invsqrt(ctx, naux, J, M, nkept);
+}
+
+}  // namespace corr
+
+extern "C" int corr_blas_release(corr_ctx* ctx) {
+    if (ctx && ctx->blas_handle) {
+        rocblas_destroy_handle((rocblas_handle)ctx->blas_handle);
+        ctx->blas_handle = nullptr;
+    }
+    return CORR_OK;
+}
+
+extern "C" int corr_ri_metric(corr_ctx* ctx, int64_t naux, const double* d_J, int mode,
+                              double* d_M, int64_t* h_nkept) {
+    CORR_CHECK_CTX(ctx);
+    return corr::ri_metric(ctx, naux, d_J, mode, d_M, h_nkept);
+}

@@ -1,0 +1,473 @@
+// RI-MP2 energy: fused pair-block GEMM + FP64 denominator + deterministic
+// FP64 reduction.  Restates run_canonical_mp2_ri (src/POSTHF/can_mp2.cc:53-132):
+//
+//   M^{ij}_ab = sum_Q B^Q_ia B^Q_jb
+//   EMP2_OS  = - sum_{ij ab} M_ab^2 / D,   EMP2_SS = - sum M_ab (M_ab - M_ba) / D
+//   D = e_a + e_b - e_i - e_j
+//
+// The reference loops over all o^2 ordered pairs with one dgemm each.  Here
+// the o*v x o*v matrix G[(i,a),(j,b)] = M^{ij}_ab is tiled with 128x128 tiles
+// whose rows are (4 occupied) x (32 virtual) and whose columns are likewise
+// (4 occupied) x (32 virtual).  A workgroup owns one tile T1 = G[I x A, J x B]
+// and its exchange partner T2[(i,a),(j,b)] = M^{ij}_ba computed as
+// B[:, J x A]^T B[:, I x B], so that M_ab and M_ba of every element sit in the
+// same lane and register: the exchange energy needs no data movement.
+// Only I <= J and A <= B tiles are launched; weights restore the ordered-pair
+// sum of the reference exactly:
+//   I <  J: x2 (the (j,i) pair gives the same sum with a<->b)
+//   A == B: x1/2 (T1 and T2 cover the same elements)
+//
+// B is repacked once into panels: panel (I, A) is a contiguous
+// [Kp][128] slab (FP64/FP32), or a [Kp/32][hi/lo][128][32] slab of BF16
+// halves (BF16x3), so every K-slab a workgroup stages into LDS is one
+// contiguous, fully coalesced read.
+#include "common.hpp"
+
+namespace corr {
+
+namespace {
+
+constexpr int PI = 4;     // occupied per panel
+constexpr int PA = 32;    // virtuals per panel
+constexpr int PW = PI * PA;  // 128 panel width
+
+__device__ inline void tri_decode(int64_t p, int64_t n, int& r, int& c) {
+    // p enumerates (r, c), r <= c < n, row-major.
+    const double b = 2.0 * n + 1.0;
+    int64_t rr = (int64_t)((b - sqrt(b * b - 8.0 * (double)p)) * 0.5);
+    if (rr < 0) rr = 0;
+    if (rr > n - 1) rr = n - 1;
+    auto start = [n](int64_t x) { return x * n - x * (x - 1) / 2; };
+    while (rr > 0 && start(rr) > p) --rr;
+    while (rr + 1 < n && start(rr + 1) <= p) ++rr;
+    r = (int)rr;
+    c = (int)(rr + (p - start(rr)));
+}
+
+// Map the dispatch order (round-robin over the 8 XCDs) to a logical order in
+// which consecutive ids run on the same XCD and share panels through its L2.
+__device__ inline int64_t xcd_logical(int64_t per) {
+    const int64_t w = blockIdx.x;
+    return (w % 8) * per + (w / 8);
+}
+
+__device__ inline uint16_t f32_to_bf16_rn(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__device__ inline double block_sum(double v, double* sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) t = sh[0] + sh[1] + sh[2] + sh[3];
+    return t;
+}
+
+}  // namespace
+
+// ---- repack ----------------------------------------------------------------
+template <typename T>
+__global__ void repack_real(const double* __restrict__ Bia, T* __restrict__ Bp, int64_t naux,
+                            int o, int v, int nAC, int64_t Kp, int64_t total) {
+    for (int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; d < total;
+         d += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(d % PW);
+        const int64_t q = (d / PW) % Kp;
+        const int64_t blk = d / (PW * Kp);
+        const int ib = (int)(blk / nAC), ac = (int)(blk % nAC);
+        const int i = ib * PI + c / PA, a = ac * PA + c % PA;
+        double x = 0.0;
+        if (q < naux && i < o && a < v) x = Bia[(q * o + i) * (int64_t)v + a];
+        Bp[d] = (T)x;
+    }
+}
+
+// One block per (panel, 32-row K slab): read [32 q][128 c] coalesced along a,
+// write [hi/lo][128 c][32 q] halves.
+__global__ __launch_bounds__(256) void repack_bf16(const double* __restrict__ Bia,
+                                                   uint16_t* __restrict__ Bp, int64_t naux, int o,
+                                                   int v, int nAC, int64_t Kc) {
+    __shared__ double S[32][PW + 1];
+    const int64_t kc = blockIdx.x % Kc;
+    const int64_t blk = blockIdx.x / Kc;
+    const int ib = (int)(blk / nAC), ac = (int)(blk % nAC);
+    for (int e = threadIdx.x; e < 32 * PW; e += 256) {
+        const int q = e / PW, c = e % PW;
+        const int64_t Q = kc * 32 + q;
+        const int i = ib * PI + c / PA, a = ac * PA + c % PA;
+        S[q][c] = (Q < naux && i < o && a < v) ? Bia[(Q * o + i) * (int64_t)v + a] : 0.0;
+    }
+    __syncthreads();
+    uint16_t* out = Bp + (blk * Kc + kc) * (2 * PW * 32);
+    for (int e = threadIdx.x; e < 2 * PW * 4; e += 256) {
+        const int hl = e / (PW * 4), c = (e % (PW * 4)) / 4, kq = e % 4;
+        uint16_t h[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const double x = S[kq * 8 + t][c];
+            const uint16_t hi = f32_to_bf16_rn((float)x);
+            if (hl == 0) h[t] = hi;
+            else h[t] = f32_to_bf16_rn((float)(x - (double)bf16_to_f32(hi)));
+        }
+        uint4 pk;
+        pk.x = h[0] | ((uint32_t)h[1] << 16);
+        pk.y = h[2] | ((uint32_t)h[3] << 16);
+        pk.z = h[4] | ((uint32_t)h[5] << 16);
+        pk.w = h[6] | ((uint32_t)h[7] << 16);
+        *reinterpret_cast<uint4*>(out + (int64_t)e * 8) = pk;
+    }
+}
+
+// ---- shared epilogue ---------------------------------------------------------
+struct PairTile {
+    int IB, JB, AC, BC;
+};
+
+template <typename ACC>
+__device__ inline void energy_epilogue(const ACC (&acc1)[4][4], const ACC (&acc2)[4][4],
+                                       const PairTile& t, int o, int v,
+                                       const double* __restrict__ eo,
+                                       const double* __restrict__ ev, double& sos, double& sx) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        const int i = t.IB * PI + 2 * wr + rb / 2;
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const int j = t.JB * PI + 2 * wc + cb / 2;
+            const int b = t.BC * PA + (cb % 2) * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = t.AC * PA + (rb % 2) * 16 + (lane >> 4) * 4 + r;
+                if (i < o && j < o && a < v && b < v) {
+                    const double m1 = (double)acc1[rb][cb][r], m2 = (double)acc2[rb][cb][r];
+                    const double dinv = 1.0 / (ev[a] + ev[b] - eo[i] - eo[j]);
+                    sos += (m1 * m1 + m2 * m2) * dinv;
+                    sx += m1 * m2 * dinv;
+                }
+            }
+        }
+    }
+}
+
+__device__ inline bool decode_tile(int nIB, int nAC, int64_t nWG, int64_t per, PairTile& t,
+                                   int64_t& id) {
+    id = xcd_logical(per);
+    if (id >= nWG) return false;
+    const int64_t nAB = (int64_t)nAC * (nAC + 1) / 2;
+    tri_decode(id / nAB, nIB, t.IB, t.JB);
+    tri_decode(id % nAB, nAC, t.AC, t.BC);
+    return true;
+}
+
+__device__ inline void write_partial(double sos, double sx, const PairTile& t, int64_t id,
+                                     double* sh, double* __restrict__ partial) {
+    const double w = (t.IB < t.JB ? 2.0 : 1.0) * (t.AC < t.BC ? 1.0 : 0.5);
+    const double os = block_sum(sos, sh);
+    const double x = block_sum(sx, sh);
+    if (threadIdx.x == 0) {
+        partial[id * 2 + 0] = -w * os;
+        partial[id * 2 + 1] = 2.0 * w * x;
+    }
+}
+
+// ---- FP64 / FP32 kernel --------------------------------------------------------
+template <typename T> struct RealMfma;
+template <> struct RealMfma<double> {
+    typedef corr_d4 acc_t;
+    __device__ static inline acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct RealMfma<float> {
+    typedef corr_f4 acc_t;
+    __device__ static inline acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void rimp2_energy_real(
+    const T* __restrict__ Bp, const double* __restrict__ eo, const double* __restrict__ ev,
+    int o, int v, int nIB, int nAC, int64_t Kp, int64_t nWG, int64_t per,
+    double* __restrict__ partial) {
+    // FP64: 8-deep K slabs keep the prefetch registers within budget next to
+    // the 2 x 64 FP64 accumulators; FP32: 16-deep.
+    constexpr int KC = sizeof(T) == 8 ? 8 : 16, LD = PW + 16, EL = KC * PW / 256;
+    __shared__ T L[4][KC][LD];
+    __shared__ double sh[4];
+    typedef typename RealMfma<T>::acc_t acc_t;
+
+    PairTile t;
+    int64_t id;
+    if (!decode_tile(nIB, nAC, nWG, per, t, id)) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1, cl = lane & 15;
+
+    const T* P[4];
+    P[0] = Bp + ((int64_t)t.IB * nAC + t.AC) * Kp * PW;
+    P[1] = Bp + ((int64_t)t.JB * nAC + t.BC) * Kp * PW;
+    P[2] = Bp + ((int64_t)t.JB * nAC + t.AC) * Kp * PW;
+    P[3] = Bp + ((int64_t)t.IB * nAC + t.BC) * Kp * PW;
+
+    acc_t acc1[4][4], acc2[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) { acc1[x][y] = acc_t{0, 0, 0, 0}; acc2[x][y] = acc_t{0, 0, 0, 0}; }
+
+    T reg[4][EL];
+    const int64_t nK = Kp / KC;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int u = 0; u < EL; ++u) reg[p][u] = P[p][u * 256 + threadIdx.x];
+
+    for (int64_t kc = 0; kc < nK; ++kc) {
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int u = 0; u < EL; ++u) {
+                const int e = u * 256 + threadIdx.x;
+                L[p][e / PW][e % PW] = reg[p][u];
+            }
+        __syncthreads();
+        if (kc + 1 < nK) {
+            const int64_t off = (kc + 1) * KC * PW;
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int u = 0; u < EL; ++u) reg[p][u] = P[p][off + u * 256 + threadIdx.x];
+        }
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            T f1[4], f2[4], f3[2][2], f4[2][2];
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) f1[rb] = L[0][kr][(2 * wr + rb / 2) * PA + (rb % 2) * 16 + cl];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) f2[cb] = L[1][kr][(2 * wc + cb / 2) * PA + (cb % 2) * 16 + cl];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    f3[x][y] = L[2][kr][(2 * wc + x) * PA + y * 16 + cl];
+                    f4[x][y] = L[3][kr][(2 * wr + x) * PA + y * 16 + cl];
+                }
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) {
+                    acc1[rb][cb] = RealMfma<T>::mma(f1[rb], f2[cb], acc1[rb][cb]);
+                    acc2[rb][cb] = RealMfma<T>::mma(f3[cb / 2][rb % 2], f4[rb / 2][cb % 2], acc2[rb][cb]);
+                }
+        }
+    }
+    double sos = 0.0, sx = 0.0;
+    energy_epilogue(acc1, acc2, t, o, v, eo, ev, sos, sx);
+    write_partial(sos, sx, t, id, sh, partial);
+}
+
+// ---- BF16x3 kernel -------------------------------------------------------------
+// M ~= hi.hi + hi.lo + lo.hi with hi = bf16(x), lo = bf16(x - hi), FP32 accumulate.
+__device__ inline corr_f4 mma_bf16(const uint4& a, const uint4& b, corr_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(corr_bf8, a),
+                                                   __builtin_bit_cast(corr_bf8, b), c, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void rimp2_energy_bf16x3(
+    const uint16_t* __restrict__ Bp, const double* __restrict__ eo, const double* __restrict__ ev,
+    int o, int v, int nIB, int nAC, int64_t Kc, int64_t nWG, int64_t per,
+    double* __restrict__ partial) {
+    constexpr int LDK = 40;              // 32 bf16 + 8 pad: conflict-free 16 B reads
+    constexpr int SLAB = 2 * PW * 32;    // bf16 per panel per K slab (hi + lo)
+    __shared__ __attribute__((aligned(16))) uint16_t L[4][2][PW][LDK];
+    __shared__ double sh[4];
+
+    PairTile t;
+    int64_t id;
+    if (!decode_tile(nIB, nAC, nWG, per, t, id)) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1, cl = lane & 15, kq = lane >> 4;
+
+    const uint16_t* P[4];
+    P[0] = Bp + ((int64_t)t.IB * nAC + t.AC) * Kc * SLAB;
+    P[1] = Bp + ((int64_t)t.JB * nAC + t.BC) * Kc * SLAB;
+    P[2] = Bp + ((int64_t)t.JB * nAC + t.AC) * Kc * SLAB;
+    P[3] = Bp + ((int64_t)t.IB * nAC + t.BC) * Kc * SLAB;
+
+    corr_f4 acc1[4][4], acc2[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) { acc1[x][y] = corr_f4{0, 0, 0, 0}; acc2[x][y] = corr_f4{0, 0, 0, 0}; }
+
+    uint4 reg[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            reg[p][u] = *reinterpret_cast<const uint4*>(P[p] + (int64_t)(u * 256 + threadIdx.x) * 8);
+
+    for (int64_t kc = 0; kc < Kc; ++kc) {
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = u * 256 + threadIdx.x;
+                const int hl = e / (PW * 4), c = (e % (PW * 4)) / 4, q = e % 4;
+                *reinterpret_cast<uint4*>(&L[p][hl][c][q * 8]) = reg[p][u];
+            }
+        __syncthreads();
+        if (kc + 1 < Kc) {
+            const int64_t off = (kc + 1) * SLAB;
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    reg[p][u] = *reinterpret_cast<const uint4*>(P[p] + off + (int64_t)(u * 256 + threadIdx.x) * 8);
+        }
+        uint4 f1h[4], f1l[4], f2h[4], f2l[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const int c = (2 * wr + rb / 2) * PA + (rb % 2) * 16 + cl;
+            f1h[rb] = *reinterpret_cast<const uint4*>(&L[0][0][c][kq * 8]);
+            f1l[rb] = *reinterpret_cast<const uint4*>(&L[0][1][c][kq * 8]);
+        }
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const int c = (2 * wc + cb / 2) * PA + (cb % 2) * 16 + cl;
+            f2h[cb] = *reinterpret_cast<const uint4*>(&L[1][0][c][kq * 8]);
+            f2l[cb] = *reinterpret_cast<const uint4*>(&L[1][1][c][kq * 8]);
+        }
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                acc1[rb][cb] = mma_bf16(f1l[rb], f2h[cb], acc1[rb][cb]);
+                acc1[rb][cb] = mma_bf16(f1h[rb], f2l[cb], acc1[rb][cb]);
+                acc1[rb][cb] = mma_bf16(f1h[rb], f2h[cb], acc1[rb][cb]);
+            }
+        uint4 f3h[2][2], f3l[2][2], f4h[2][2], f4l[2][2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int c3 = (2 * wc + x) * PA + y * 16 + cl;
+                const int c4 = (2 * wr + x) * PA + y * 16 + cl;
+                f3h[x][y] = *reinterpret_cast<const uint4*>(&L[2][0][c3][kq * 8]);
+                f3l[x][y] = *reinterpret_cast<const uint4*>(&L[2][1][c3][kq * 8]);
+                f4h[x][y] = *reinterpret_cast<const uint4*>(&L[3][0][c4][kq * 8]);
+                f4l[x][y] = *reinterpret_cast<const uint4*>(&L[3][1][c4][kq * 8]);
+            }
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                acc2[rb][cb] = mma_bf16(f3l[cb / 2][rb % 2], f4h[rb / 2][cb % 2], acc2[rb][cb]);
+                acc2[rb][cb] = mma_bf16(f3h[cb / 2][rb % 2], f4l[rb / 2][cb % 2], acc2[rb][cb]);
+                acc2[rb][cb] = mma_bf16(f3h[cb / 2][rb % 2], f4h[rb / 2][cb % 2], acc2[rb][cb]);
+            }
+    }
+    double sos = 0.0, sx = 0.0;
+    energy_epilogue(acc1, acc2, t, o, v, eo, ev, sos, sx);
+    write_partial(sos, sx, t, id, sh, partial);
+}
+
+// ---- host side -------------------------------------------------------------------
+struct RiLayout {
+    int nIB, nAC;
+    int64_t Kp, nWG, per;
+};
+
+static RiLayout ri_layout(int64_t naux, int64_t o, int64_t v) {
+    RiLayout l;
+    l.nIB = (int)cdiv(o, PI);
+    l.nAC = (int)cdiv(v, PA);
+    l.Kp = rup(naux, 32);
+    l.nWG = (int64_t)l.nIB * (l.nIB + 1) / 2 * ((int64_t)l.nAC * (l.nAC + 1) / 2);
+    l.per = cdiv(l.nWG, 8);
+    return l;
+}
+
+// Repacks Bia into the panel layout for the given precision; returns the
+// device pointer in *out.
+int ri_repack(corr_ctx* ctx, const RiLayout& l, int64_t naux, int o, int v, const double* Bia,
+              int prec, const void** out) {
+    const int64_t elems = (int64_t)l.nIB * l.nAC * l.Kp * PW;
+    const size_t esz = prec == CORR_PREC_FP64 ? 8 : 4;   // bf16 hi+lo = 4 B
+    void* Bp = nullptr;
+    CORR_TRY(scratch(ctx, "rimp2_panels", elems * esz, &Bp));
+    if (prec == CORR_PREC_BF16X3) {
+        const int64_t Kc = l.Kp / 32;
+        const int64_t blocks = (int64_t)l.nIB * l.nAC * Kc;
+        hipLaunchKernelGGL(repack_bf16, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, Bia,
+                           (uint16_t*)Bp, naux, o, v, l.nAC, Kc);
+    } else {
+        const int64_t grid = std::min<int64_t>(cdiv(elems, 256), 65536 * 4);
+        if (prec == CORR_PREC_FP64)
+            hipLaunchKernelGGL(repack_real<double>, dim3((unsigned)grid), dim3(256), 0, ctx->stream,
+                               Bia, (double*)Bp, naux, o, v, l.nAC, l.Kp, elems);
+        else
+            hipLaunchKernelGGL(repack_real<float>, dim3((unsigned)grid), dim3(256), 0, ctx->stream,
+                               Bia, (float*)Bp, naux, o, v, l.nAC, l.Kp, elems);
+    }
+    CORR_LAUNCHED(ctx, "rimp2 repack");
+    *out = Bp;
+    return CORR_OK;
+}
+
+int rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
+                 const double* eo, const double* ev, int prec, corr_mp2_energy* out) {
+    CORR_REQUIRE(ctx, naux > 0 && nocc > 0 && nvir > 0, "rimp2: sizes must be positive");
+    CORR_REQUIRE(ctx, nocc < (1 << 20) && nvir < (1 << 20), "rimp2: nocc/nvir too large");
+    CORR_REQUIRE(ctx, Bia && eo && ev && out, "rimp2: null pointer");
+    CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
+                 "rimp2: unknown precision");
+    const RiLayout l = ri_layout(naux, nocc, nvir);
+    CORR_REQUIRE(ctx, l.per * 8 < ((int64_t)1 << 31), "rimp2: too many tiles");
+    const void* Bp = nullptr;
+    CORR_TRY(ri_repack(ctx, l, naux, (int)nocc, (int)nvir, Bia, prec, &Bp));
+    double* partial = nullptr;
+    CORR_TRY(scratch(ctx, "rimp2_partial", sizeof(double) * 2 * l.nWG, (void**)&partial));
+    const dim3 grid((unsigned)(l.per * 8)), block(256);
+    KernelTimer timer(ctx, "rimp2_energy");
+    if (prec == CORR_PREC_FP64)
+        hipLaunchKernelGGL(rimp2_energy_real<double>, grid, block, 0, ctx->stream, (const double*)Bp,
+                           eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp, l.nWG, l.per, partial);
+    else if (prec == CORR_PREC_FP32)
+        hipLaunchKernelGGL(rimp2_energy_real<float>, grid, block, 0, ctx->stream, (const float*)Bp,
+                           eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp, l.nWG, l.per, partial);
+    else
+        hipLaunchKernelGGL(rimp2_energy_bf16x3, grid, block, 0, ctx->stream, (const uint16_t*)Bp,
+                           eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp / 32, l.nWG, l.per,
+                           partial);
+    CORR_LAUNCHED(ctx, "rimp2_energy");
+    timer.stop();
+    double s[2];
+    CORR_TRY(reduce_sum(ctx, partial, l.nWG, 2, s));
+    out->e_os = s[0];
+    out->e_ss = s[0] + s[1];
+    out->e_mp2 = out->e_os + out->e_ss;
+    out->e_scs = out->e_ss / 3.0 + out->e_os * 6.0 / 5.0;
+    return CORR_OK;
+}
+
+}  // namespace corr
+
+extern "C" int corr_rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir,
+                                 const double* d_Bia, const double* d_eps_occ,
+                                 const double* d_eps_vir, int prec, corr_mp2_energy* h_out) {
+    CORR_CHECK_CTX(ctx);
+    return corr::rimp2_energy(ctx, naux, nocc, nvir, d_Bia, d_eps_occ, d_eps_vir, prec, h_out);
+}

@@ -1,0 +1,235 @@
+// MO Fock matrix, 4-index AO->MO transform, canonical MP2 and RI B formation.
+//
+//  build_FMo            src/POSTHF/motrans.cc:8-32      FMo = C F C^T (2 GEMMs)
+//  trans_I / MOtrans    src/POSTHF/motrans.cc:341-403   unfold the sorted unique
+//                       AO list into the dense N^4 tensor (scatter kernel)
+//  mo2int_op            src/POSTHF/motrans.cc:407-503   four quarter transforms;
+//                       each is one GEMM out[m,n] = sum_k in[k,m] C[n,k] with
+//                       M = N^3, so the contracted index is always the leading
+//                       one and the result index rotates to the back:
+//                       [a][bcd] -> [bcd][i] -> [cdi][j] -> [dij][k] -> [ijk][l]
+//  run_canonical_mp2    src/POSTHF/can_mp2.cc:11-49     fused gather + energy
+//  calculate_ri (BPQ)   src/POSTHF/ri.cc:102-103        one GEMM
+//  transform_ri         src/POSTHF/ri.cc:147-172        two GEMMs
+#include "common.hpp"
+
+namespace corr {
+
+namespace {
+
+unsigned grid_for(int64_t n) {
+    int64_t g = cdiv(n, 256);
+    return (unsigned)(g > 65536 * 4 ? 65536 * 4 : (g < 1 ? 1 : g));
+}
+
+__global__ void zero_kernel(double* __restrict__ p, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        p[e] = 0.0;
+}
+
+// Images of one sorted unique integral, by permutation class (motrans.cc:358-402).
+// Index order of each image is (p q | r s) of the dense AO tensor.
+__global__ void unfold_kernel(int64_t N, int64_t nrofint, int64_t s0, int64_t s1, int64_t s2,
+                              int64_t s3, const double* __restrict__ val,
+                              const uint16_t* __restrict__ idx, double* __restrict__ dense) {
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < nrofint;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = idx[4 * x + 0], b = idx[4 * x + 1], c = idx[4 * x + 2], d = idx[4 * x + 3];
+        if (a >= N || b >= N || c >= N || d >= N) continue;
+        const double w = val[x];
+        auto put = [&](int64_t p, int64_t q, int64_t r, int64_t s) {
+            dense[((p * N + q) * N + r) * N + s] = w;
+        };
+        put(a, b, c, d);                       // (1) every class
+        if (x < s0) continue;                  // PERM_1
+        if (x < s1) { put(c, d, a, b); continue; }                       // PERM_15: (1)(5)
+        if (x < s2) { put(a, b, d, c); put(b, a, c, d); put(b, a, d, c); continue; }  // PERM_1234
+        if (x < s3) { put(a, b, d, c); put(c, d, a, b); put(d, c, a, b); continue; }  // PERM_1256
+        put(a, b, d, c); put(b, a, c, d); put(b, a, d, c);               // PERM_ALL
+        put(c, d, a, b); put(d, c, a, b); put(c, d, b, a); put(d, c, b, a);
+    }
+}
+
+// (ia|jb) = prec_ints[((i*N + o+a)*N + j)*N + o+b]; one partial pair per block.
+__global__ __launch_bounds__(256) void canonical_mp2_kernel(const double* __restrict__ I4,
+                                                            const double* __restrict__ FMo,
+                                                            int64_t N, int64_t o,
+                                                            double* __restrict__ partial) {
+    const int64_t v = N - o;
+    const int64_t total = o * o * v * v;
+    double sos = 0.0, sss = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = e % v, a = (e / v) % v, j = (e / (v * v)) % o, i = e / (v * v * o);
+        const double iajb = I4[((i * N + o + a) * N + j) * N + o + b];
+        const double ibja = I4[((i * N + o + b) * N + j) * N + o + a];
+        const double D = FMo[(o + a) * N + o + a] + FMo[(o + b) * N + o + b] - FMo[i * N + i] -
+                         FMo[j * N + j];
+        sos -= iajb * iajb / D;
+        sss -= (iajb - ibja) * iajb / D;
+    }
+    __shared__ double sh[2][256];
+    sh[0][threadIdx.x] = sos;
+    sh[1][threadIdx.x] = sss;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x + 0] = sh[0][0];
+        partial[2 * blockIdx.x + 1] = sh[1][0];
+    }
+}
+
+int gemm_prec(int prec) { return prec == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_PREC_FP32; }
+
+}  // namespace
+
+int build_fmo(corr_ctx* ctx, int64_t N, const double* C, const double* F, double* FMo) {
+    CORR_REQUIRE(ctx, N > 0 && C && F && FMo, "build_fmo: bad arguments");
+    double* T = nullptr;
+    CORR_TRY(scratch(ctx, "fmo_T", sizeof(double) * N * N, (void**)&T));
+    CORR_TRY(gemm(ctx, 0, 0, N, N, N, 1.0, C, N, 0, F, N, 0, 0.0, T, N, 0, 1, CORR_PREC_FP64));
+    CORR_TRY(gemm(ctx, 0, 1, N, N, N, 1.0, T, N, 0, C, N, 0, 0.0, FMo, N, 0, 1, CORR_PREC_FP64));
+    return CORR_OK;
+}
+
+int ao_unfold(corr_ctx* ctx, int64_t N, int64_t nrofint, const int64_t* sc, const double* val,
+              const uint16_t* idx, double* dense) {
+    CORR_REQUIRE(ctx, N > 0 && N <= 65536 && nrofint >= 0 && sc && dense, "ao_unfold: bad arguments");
+    CORR_REQUIRE(ctx, nrofint == 0 || (val && idx), "ao_unfold: null list");
+    CORR_REQUIRE(ctx, 0 <= sc[0] && sc[0] <= sc[1] && sc[1] <= sc[2] && sc[2] <= sc[3] && sc[3] <= nrofint,
+                 "ao_unfold: sortcount must be non-decreasing and <= nrofint");
+    const int64_t n4 = N * N * N * N;
+    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(n4)), dim3(256), 0, ctx->stream, dense, n4);
+    CORR_LAUNCHED(ctx, "zero_kernel");
+    if (nrofint > 0) {
+        hipLaunchKernelGGL(unfold_kernel, dim3(grid_for(nrofint)), dim3(256), 0, ctx->stream, N, nrofint,
+                           sc[0], sc[1], sc[2], sc[3], val, idx, dense);
+        CORR_LAUNCHED(ctx, "unfold_kernel");
+    }
+    return CORR_OK;
+}
+
+int motrans(corr_ctx* ctx, int64_t N, const double* C, const double* ao, double* work, double* out,
+            int prec) {
+    CORR_REQUIRE(ctx, N > 0 && C && ao && work && out, "motrans: bad arguments");
+    CORR_REQUIRE(ctx, ao != out && work != out && work != ao, "motrans: buffers must not alias");
+    const int64_t N3 = N * N * N;
+    const int p = gemm_prec(prec);
+    const double* in[4] = {ao, work, out, work};
+    double* dst[4] = {work, out, work, out};
+    for (int s = 0; s < 4; ++s)
+        CORR_TRY(gemm(ctx, 1, 1, N3, N, N, 1.0, in[s], N3, 0, C, N, 0, 0.0, dst[s], N, 0, 1, p));
+    return CORR_OK;
+}
+
+int canonical_mp2(corr_ctx* ctx, int64_t N, int64_t o, const double* I4, const double* FMo,
+                  corr_mp2_energy* out) {
+    CORR_REQUIRE(ctx, N > 0 && o > 0 && o < N && I4 && FMo && out, "canonical_mp2: bad arguments");
+    const int nblk = 1024;
+    double* partial = nullptr;
+    CORR_TRY(scratch(ctx, "cmp2_partial", sizeof(double) * 2 * nblk, (void**)&partial));
+    hipLaunchKernelGGL(canonical_mp2_kernel, dim3(nblk), dim3(256), 0, ctx->stream, I4, FMo, N, o, partial);
+    CORR_LAUNCHED(ctx, "canonical_mp2_kernel");
+    double s[2];
+    CORR_TRY(reduce_sum(ctx, partial, nblk, 2, s));
+    out->e_os = s[0];
+    out->e_ss = s[1];
+    out->e_mp2 = s[0] + s[1];
+    out->e_scs = s[1] / 3.0 + s[0] * 6.0 / 5.0;
+    return CORR_OK;
+}
+
+int ri_bpq(corr_ctx* ctx, int64_t naux, int64_t N, const double* M, const double* Pmn, double* BPQ,
+           int prec) {
+    CORR_REQUIRE(ctx, naux > 0 && N > 0 && M && Pmn && BPQ, "ri_bpq: bad arguments");
+    return gemm(ctx, 0, 0, naux, N * N, naux, 1.0, M, naux, 0, Pmn, N * N, 0, 0.0, BPQ, N * N, 0, 1,
+                gemm_prec(prec));
+}
+
+// X[Q][i][a] = sum_{mu nu} C[i,mu] C[o+a,nu] Y[Q][mu][nu]
+static int ao2ov(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Y,
+                 double* X, int p) {
+    const int64_t v = N - o;
+    double* BiQ = nullptr;
+    CORR_TRY(scratch(ctx, "ri_BiQ", sizeof(double) * naux * o * N, (void**)&BiQ));
+    // BiQ[Q] (o x N) = C_occ (o x N) * Y[Q] (N x N)
+    CORR_TRY(gemm(ctx, 0, 0, o, N, N, 1.0, C, N, 0, Y, N, N * N, 0.0, BiQ, N, o * N, naux, p));
+    // X (naux*o x v) = BiQ (naux*o x N) * C_vir^T
+    CORR_TRY(gemm(ctx, 0, 1, naux * o, v, N, 1.0, BiQ, N, 0, C + o * N, N, 0, 0.0, X, v, 0, 1, p));
+    return CORR_OK;
+}
+
+int transform_ri(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C,
+                 const double* BPQ, double* Bia, int prec) {
+    CORR_REQUIRE(ctx, naux > 0 && N > 0 && o > 0 && o < N && C && BPQ && Bia, "transform_ri: bad arguments");
+    return ao2ov(ctx, naux, N, o, C, BPQ, Bia, gemm_prec(prec));
+}
+
+int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Pmn,
+                const double* M, int mode, double* Bia, int prec) {
+    CORR_REQUIRE(ctx, naux > 0 && N > 0 && o > 0 && o < N && C && Pmn && M && Bia,
+                 "ri_form_bia: bad arguments");
+    CORR_REQUIRE(ctx, mode == CORR_METRIC_EIGEN || mode == CORR_METRIC_CHOLESKY, "ri_form_bia: unknown metric mode");
+    const int64_t v = N - o;
+    const int p = gemm_prec(prec);
+    double* X = nullptr;
+    CORR_TRY(scratch(ctx, "ri_Pia", sizeof(double) * naux * o * v, (void**)&X));
+    CORR_TRY(ao2ov(ctx, naux, N, o, C, Pmn, X, p));
+    return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1, p,
+                mode == CORR_METRIC_CHOLESKY);
+}
+
+}  // namespace corr
+
+extern "C" {
+
+int corr_build_fmo(corr_ctx* ctx, int64_t nroao, const double* d_MOs, const double* d_Fmat,
+                   double* d_FMo) {
+    CORR_CHECK_CTX(ctx);
+    return corr::build_fmo(ctx, nroao, d_MOs, d_Fmat, d_FMo);
+}
+
+int corr_ao_unfold(corr_ctx* ctx, int64_t nroao, int64_t nrofint, const int64_t* h_sortcount,
+                   const double* d_intval, const uint16_t* d_intnums, double* d_dense_ao) {
+    CORR_CHECK_CTX(ctx);
+    return corr::ao_unfold(ctx, nroao, nrofint, h_sortcount, d_intval, d_intnums, d_dense_ao);
+}
+
+int corr_motrans(corr_ctx* ctx, int64_t nroao, const double* d_MOs, const double* d_dense_ao,
+                 double* d_work, double* d_prec_ints, int prec) {
+    CORR_CHECK_CTX(ctx);
+    return corr::motrans(ctx, nroao, d_MOs, d_dense_ao, d_work, d_prec_ints, prec);
+}
+
+int corr_canonical_mp2(corr_ctx* ctx, int64_t nroao, int64_t nocc, const double* d_prec_ints,
+                       const double* d_FMo, corr_mp2_energy* h_out) {
+    CORR_CHECK_CTX(ctx);
+    return corr::canonical_mp2(ctx, nroao, nocc, d_prec_ints, d_FMo, h_out);
+}
+
+int corr_ri_bpq(corr_ctx* ctx, int64_t naux, int64_t nroao, const double* d_M, const double* d_Pmn,
+                double* d_BPQ, int prec) {
+    CORR_CHECK_CTX(ctx);
+    return corr::ri_bpq(ctx, naux, nroao, d_M, d_Pmn, d_BPQ, prec);
+}
+
+int corr_transform_ri(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc, const double* d_MOs,
+                      const double* d_BPQ, double* d_Bia, int prec) {
+    CORR_CHECK_CTX(ctx);
+    return corr::transform_ri(ctx, naux, nroao, nocc, d_MOs, d_BPQ, d_Bia, prec);
+}
+
+int corr_ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc, const double* d_MOs,
+                     const double* d_Pmn, const double* d_M, int metric_mode, double* d_Bia, int prec) {
+    CORR_CHECK_CTX(ctx);
+    return corr::ri_form_bia(ctx, naux, nroao, nocc, d_MOs, d_Pmn, d_M, metric_mode, d_Bia, prec);
+}
+
+}  // extern "C"

@@ -1,0 +1,227 @@
+/*
+ * corr_abi.h — C-ABI of the MI355X (gfx950) post-HF contraction library
+ * (libcorr_gpu.so).
+ *
+ * Every entry point takes plain pointers and sizes, returns an int status and
+ * never throws.  Array arguments named d_* are device (HBM) addresses; h_*
+ * are host addresses.  Work is enqueued on the context's stream; functions
+ * that return host scalars (energies) synchronise that stream before they
+ * return.  The library never allocates host memory for the caller; device
+ * scratch lives in the context and is reused between calls.
+ *
+ * Layouts follow the reference (row-major, 64-bit indices here):
+ *   MOs        C[p*N + mu]                        (row = MO)
+ *   Fock, FMo  F[p*N + q]
+ *   prec_ints  (ij|kl) at ((i*N + j)*N + k)*N + l (chemist notation)
+ *   BPQ        B[Q*N*N + mu*N + nu]
+ *   Bia        B[Q*o*v + i*v + a]
+ *   T1         [i*v + a],  T2 [((i*o + j)*v + a)*v + b]   (spin orbitals)
+ *
+ * Reference interfaces replaced (Masterluke87/corr):
+ *   build_FMo             src/POSTHF/motrans.cc:8-32
+ *   MOtrans / trans_I /
+ *   mo2int_op             src/POSTHF/motrans.cc:35-211, 341-503
+ *   run_canonical_mp2     src/POSTHF/can_mp2.cc:11-49
+ *   calculate_ri (arith.) src/POSTHF/ri.cc:50-69, 102-103
+ *   transform_ri          src/POSTHF/ri.cc:147-172
+ *   run_canonical_mp2_ri  src/POSTHF/can_mp2.cc:53-132
+ *   ccsd_ur and helpers   src/POSTHF/ccsd.cc:6-580,
+ *                         src/POSTHF/ccsd_intermediates.cc:6-287
+ */
+#ifndef CORR_ABI_H
+#define CORR_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CORR_OK            0
+#define CORR_EINVAL        1   /* bad dimension, null pointer, bad enum      */
+#define CORR_ENOMEM        2   /* device allocation failed                   */
+#define CORR_EHIP          3   /* a HIP runtime call failed                  */
+#define CORR_ENOTCONV      4   /* iteration did not converge                 */
+#define CORR_ENOTPD        5   /* metric not positive definite (Cholesky)    */
+#define CORR_ENODEV        6   /* no usable gfx950 device                    */
+
+/* ---- arithmetic of the bulk contractions ------------------------------- */
+#define CORR_PREC_FP64     0   /* FP64 MFMA, FP64 accumulate (plain method)  */
+#define CORR_PREC_FP32     1   /* FP32 MFMA, FP32 accumulate                 */
+#define CORR_PREC_BF16X3   2   /* hi/lo BF16 split, 3 products, FP32 acc.    */
+
+/* ---- RI metric factorisation ------------------------------------------- */
+#define CORR_METRIC_EIGEN     0 /* J^-1/2 = V L^-1/2 V^T, cutoff as ri.cc:56-66 */
+#define CORR_METRIC_CHOLESKY  1 /* L^-1 with J = L L^T (same B B^T products)   */
+
+/* ---- CCSD iteration mode ------------------------------------------------ */
+#define CORR_CCSD_REFERENCE   0 /* fixed iteration count, no DIIS (ccsd.cc:564) */
+#define CORR_CCSD_DIIS        1 /* iterate to |dE| and residual thresholds     */
+
+typedef struct corr_ctx corr_ctx;
+
+typedef struct corr_mp2_energy {
+    double e_os;       /* EMP2_OS  (opposite spin)                  */
+    double e_ss;       /* EMP2_SS  (same spin)                      */
+    double e_mp2;      /* EMP2 = EMP2_OS + EMP2_SS                  */
+    double e_scs;      /* EMP2(SCS) = EMP2_SS/3 + 6/5 EMP2_OS       */
+} corr_mp2_energy;
+
+/* ---- version / context --------------------------------------------------- */
+int         corr_version(void);
+int         corr_ctx_create(int device, corr_ctx** out);
+int         corr_ctx_destroy(corr_ctx* ctx);
+/* Use an existing hipStream_t (passed as void*); NULL selects the context's
+ * own stream. */
+int         corr_ctx_set_stream(corr_ctx* ctx, void* hip_stream);
+int         corr_ctx_synchronize(corr_ctx* ctx);
+const char* corr_ctx_last_error(const corr_ctx* ctx);
+/* Releases the context's cached device scratch. */
+int         corr_ctx_trim(corr_ctx* ctx);
+/* Kernel timing: when enabled, the main kernel of each entry point is
+ * bracketed by HIP events on the context stream.  corr_ctx_kernel_time
+ * returns the summed duration (ms) and launch count of one kernel name
+ * ("rimp2_energy", "rimp2_gradient", "gemm", ...) since the last reset;
+ * name == NULL resets all counters.  No reference counterpart (the
+ * reference only has omp_get_wtime timers, main.cc:42-45, 159-166). */
+int         corr_ctx_enable_timing(corr_ctx* ctx, int on);
+int         corr_ctx_kernel_time(corr_ctx* ctx, const char* name,
+                                 double* h_total_ms, int64_t* h_launches);
+
+/* ---- device memory helpers (for C/C++ hosts without a tensor library) ---- */
+int corr_dmalloc(corr_ctx* ctx, void** d_ptr, int64_t bytes);
+int corr_dfree(corr_ctx* ctx, void* d_ptr);
+int corr_h2d(corr_ctx* ctx, void* d_dst, const void* h_src, int64_t bytes);
+int corr_d2h(corr_ctx* ctx, void* h_dst, const void* d_src, int64_t bytes);
+
+/* ---- generic strided-batched MFMA GEMM (row-major) ----------------------
+ * C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b],  b < batch.
+ * op(A) is M x K: transa = 0 -> A is M x K (lda >= K), 1 -> A is K x M.
+ * op(B) is K x N: transb = 0 -> B is K x N (ldb >= N), 1 -> B is N x K.
+ * All operands FP64 in memory; prec selects the MFMA arithmetic.
+ * Used for every GEMM-shaped contraction of the path. */
+int corr_dgemm(corr_ctx* ctx, int transa, int transb,
+               int64_t M, int64_t N, int64_t K, double alpha,
+               const double* d_A, int64_t lda, int64_t strideA,
+               const double* d_B, int64_t ldb, int64_t strideB,
+               double beta, double* d_C, int64_t ldc, int64_t strideC,
+               int64_t batch, int prec);
+
+/* ---- a1: FMo = C F C^T  (motrans.cc:8-32) -------------------------------- */
+int corr_build_fmo(corr_ctx* ctx, int64_t nroao, const double* d_MOs,
+                   const double* d_Fmat, double* d_FMo);
+
+/* ---- a2-a4: 4-index AO->MO transform (motrans.cc:35-211, 341-503) --------
+ * Unfold the sorted unique AO list (5 permutation classes, bounds in
+ * sortcount[0..3], nrofint entries, indices 4 x uint16 per entry) into the
+ * dense N^4 AO tensor, then apply four quarter transforms.  d_work must hold
+ * N^4 doubles; d_prec_ints receives (ij|kl).  d_work is overwritten. */
+int corr_ao_unfold(corr_ctx* ctx, int64_t nroao, int64_t nrofint,
+                   const int64_t* h_sortcount, const double* d_intval,
+                   const uint16_t* d_intnums, double* d_dense_ao);
+int corr_motrans(corr_ctx* ctx, int64_t nroao, const double* d_MOs,
+                 const double* d_dense_ao, double* d_work,
+                 double* d_prec_ints, int prec);
+
+/* ---- a6: canonical MP2 from (ia|jb) in prec_ints (can_mp2.cc:11-49) ------ */
+int corr_canonical_mp2(corr_ctx* ctx, int64_t nroao, int64_t nocc,
+                       const double* d_prec_ints, const double* d_FMo,
+                       corr_mp2_energy* h_out);
+
+/* ---- a7: RI metric (ri.cc:50-69) -----------------------------------------
+ * Writes the naux x naux matrix M with M^T M = J^-1 (on the kept space):
+ * EIGEN -> symmetric J^-1/2 with the reference cutoff (eigenvalues with
+ * l/l_max < 1e-10 or l < 0 dropped); CHOLESKY -> lower-triangular L^-1.
+ * h_nkept (optional) receives the number of kept eigenvalues / naux. */
+int corr_ri_metric(corr_ctx* ctx, int64_t naux, const double* d_J,
+                   int mode, double* d_M, int64_t* h_nkept);
+
+/* ---- a7/a8: B^Q_ia formation ---------------------------------------------
+ * corr_ri_bpq:      BPQ = M * (P|mu nu)              (ri.cc:102-103)
+ * corr_transform_ri: Bia[Q,i,a] = sum C_i,mu C_a,nu BPQ[Q,mu,nu]  (ri.cc:147-172)
+ * corr_ri_form_bia: Bia = M * ((P|mu nu) -> (P|ia)), transform first, then
+ *                   the metric: 1.97 instead of 7.8 TFLOP at N=768, naux=2432.
+ *                   metric_mode tells how M was made (CHOLESKY: M is lower
+ *                   triangular and its zero half is skipped).
+ * GEMM steps requested in CORR_PREC_BF16X3 run in FP32 MFMA. */
+int corr_ri_bpq(corr_ctx* ctx, int64_t naux, int64_t nroao, const double* d_M,
+                const double* d_Pmn, double* d_BPQ, int prec);
+int corr_transform_ri(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,
+                      const double* d_MOs, const double* d_BPQ, double* d_Bia,
+                      int prec);
+int corr_ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,
+                     const double* d_MOs, const double* d_Pmn, const double* d_M,
+                     int metric_mode, double* d_Bia, int prec);
+
+/* ---- a9: RI-MP2 energy, fused pair-block GEMM + denominator + reduction ---
+ * (can_mp2.cc:53-132).  eps_occ[i] = FMo[i,i], eps_vir[a] = FMo[o+a,o+a]. */
+int corr_rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir,
+                      const double* d_Bia, const double* d_eps_occ,
+                      const double* d_eps_vir, int prec, corr_mp2_energy* h_out);
+
+/* ---- RI-MP2 energy gradient (no reference counterpart: the reference has no
+ * gradient).  Writes dE/dBia (same layout as Bia), dE/d eps_occ, dE/d eps_vir
+ * for E = EMP2 (c_os = c_ss = 1) or any (c_os, c_ss) spin-component scaling.
+ * Pair blocks are recomputed, the o^2 v^2 amplitudes are never stored. */
+int corr_rimp2_gradient(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir,
+                        const double* d_Bia, const double* d_eps_occ,
+                        const double* d_eps_vir, double c_os, double c_ss,
+                        int prec, double* d_dBia, double* d_deps_occ,
+                        double* d_deps_vir, corr_mp2_energy* h_out);
+
+/* ---- a10-a19: spin-orbital CCSD (ccsd.cc:6-580, ccsd_intermediates.cc) ----
+ * Inputs: nroao spatial MOs, nroe electrons, prec_ints (ij|kl) over spatial
+ * MOs (N^4), MOs, Hmat (AO core Hamiltonian), ion_rep.
+ * Outputs: per-iteration energies (h_energies[maxiter]), iterations done,
+ * final T1 (o x v) and T2 (o^2 v^2) in the reference layout (device).
+ * mode CORR_CCSD_REFERENCE reproduces ccsd_ur: maxiter Jacobi updates, the
+ * energy printed at the top of each iteration; CORR_CCSD_DIIS adds DIIS(8)
+ * and stops when |dE| < e_tol and the residual norm < r_tol.
+ * h_emp2 (optional) receives the spin-orbital MP2 energy of the guess. */
+typedef struct corr_ccsd_opts {
+    int    mode;          /* CORR_CCSD_REFERENCE or CORR_CCSD_DIIS          */
+    int    maxiter;       /* 20 in the reference                            */
+    int    diis_size;     /* subspace size for CORR_CCSD_DIIS (8)           */
+    int    prec;          /* arithmetic of the ladder/ring/T1 contractions  */
+    double e_tol;         /* energy change threshold (DIIS mode)            */
+    double r_tol;         /* amplitude-change norm threshold (DIIS mode)    */
+} corr_ccsd_opts;
+
+typedef struct corr_ccsd_result {
+    double e_hf_check;    /* El of calc_e_check (ccsd.cc:50-81), + ion_rep  */
+    double e_mp2;         /* spin-orbital MP2 of the guess (ccsd.cc:131-143) */
+    double e_ccsd;        /* last energy evaluated                           */
+    double d_e;           /* last energy change (E_old - E)                  */
+    double r_norm;        /* last amplitude-change norm                      */
+    int    iterations;    /* energy evaluations done                         */
+    int    converged;     /* DIIS mode: 1 if both thresholds met             */
+} corr_ccsd_result;
+
+int corr_ccsd(corr_ctx* ctx, int64_t nroao, int64_t nroe,
+              const double* d_prec_ints, const double* d_MOs,
+              const double* d_Hmat, double ion_rep,
+              const corr_ccsd_opts* opts, double* h_energies,
+              double* d_T1, double* d_T2, corr_ccsd_result* h_out);
+
+/* Building blocks of corr_ccsd, exported for parity testing. */
+/* ints_so[pqrs] = <pq||rs> over 2N spin orbitals (ccsd.cc:6-9, 509-535) */
+int corr_ccsd_ints_so(corr_ctx* ctx, int64_t nroao, const double* d_prec_ints,
+                      double* d_ints_so);
+/* One CCSD update from (T1, T2) -> (T1n, T2n) with ints_so and the
+ * spin-orbital Fock matrix f (norb x norb), exactly one pass of
+ * ccsd_build_taus / ccsd_build_intermediates / ccsd_update_T1 / _T2. */
+int corr_ccsd_update(corr_ctx* ctx, int64_t nocc, int64_t nvir,
+                     const double* d_ints_so, const double* d_f,
+                     const double* d_T1, const double* d_T2,
+                     double* d_T1n, double* d_T2n, int prec);
+/* E(CCSD) of ccsd_energy (ccsd.cc:147-219) for given amplitudes. */
+int corr_ccsd_energy(corr_ctx* ctx, int64_t nocc, int64_t nvir,
+                     const double* d_ints_so, const double* d_f,
+                     const double* d_T1, const double* d_T2, double* h_e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CORR_ABI_H */

@@ -1,0 +1,168 @@
+"""GPU parity of the post-HF kernels against the CPU oracle (through the C-ABI).
+
+Tolerances: FP64 paths agree with the FP64 oracle to 1e-12 relative (energies
+to 1e-10 Eh absolute); FP32 MFMA paths to 1e-5 relative on matrix entries;
+BF16x3 energies to 1e-6 relative."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import posthf as O
+from tests import _synth as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+
+
+@pytest.fixture(scope="module")
+def corr():
+    import corr_amd
+    return corr_amd
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (37, 53, 29), (130, 257, 70), (300, 64, 16), (64, 192, 192)])
+def test_gemm_fp64(corr, ta, tb, M, N, K):
+    _say(f"gemm fp64 ta={ta} tb={tb} M={M} N={N} K={K}")
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    ref = (A.T if ta else A) @ (B.T if tb else B)
+    out = corr.dgemm(dev(A), dev(B), bool(ta), bool(tb)).cpu().numpy()
+    assert np.allclose(out, ref, rtol=1e-12, atol=1e-12 * np.sqrt(K))
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 1), (1, 0)])
+def test_gemm_fp32(corr, ta, tb):
+    M, N, K = 200, 150, 300
+    _say(f"gemm fp32 ta={ta} tb={tb}")
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    ref = (A.T if ta else A) @ (B.T if tb else B)
+    out = corr.dgemm(dev(A), dev(B), bool(ta), bool(tb), prec="fp32").cpu().numpy()
+    assert np.max(np.abs(out - ref)) < 1e-5 * np.sqrt(K) * 4
+
+
+@pytest.mark.parametrize("naux,o,v", [(76, 5, 19), (50, 1, 1), (33, 9, 40), (140, 13, 70), (304, 20, 76)])
+def test_rimp2_fp64_matches_oracle(corr, naux, o, v):
+    _say(f"rimp2 fp64 naux={naux} o={o} v={v}")
+    B = S.bia(naux, o, v, seed=naux + o)
+    eo, ev = S.orbital_energies(o, v, seed=o + v)
+    ref = O.rimp2(B, eo, ev)
+    got = corr.rimp2_energy(dev(B), dev(eo), dev(ev), prec="fp64")
+    for k in ref:
+        assert abs(got[k] - ref[k]) < 1e-10 and abs(got[k] - ref[k]) <= 1e-12 * abs(ref[k]) + 1e-15, k
+
+
+@pytest.mark.parametrize("prec,rtol", [("fp32", 2e-6), ("bf16x3", 2e-6)])
+def test_rimp2_reduced_precision(corr, prec, rtol):
+    naux, o, v = 304, 20, 76
+    _say(f"rimp2 {prec}")
+    B = S.bia(naux, o, v, seed=5)
+    eo, ev = S.orbital_energies(o, v, seed=6)
+    ref = O.rimp2(B, eo, ev)
+    got = corr.rimp2_energy(dev(B), dev(eo), dev(ev), prec=prec)
+    for k in ref:
+        assert abs(got[k] - ref[k]) <= rtol * abs(ref[k]), (k, got[k], ref[k])
+
+
+def test_rimp2_deterministic(corr):
+    naux, o, v = 140, 13, 70
+    B, (eo, ev) = S.bia(naux, o, v, seed=9), S.orbital_energies(o, v, seed=9)
+    a = corr.rimp2_energy(dev(B), dev(eo), dev(ev))
+    b = corr.rimp2_energy(dev(B), dev(eo), dev(ev))
+    assert a == b
+
+
+@pytest.mark.parametrize("naux", [7, 64, 65, 200])
+def test_ri_metric(corr, naux):
+    _say(f"ri_metric naux={naux}")
+    J = S.metric(naux, seed=naux, cond=1e5)
+    Me, kept = corr.ri_metric(dev(J), "eigen")
+    Mr, _ = O.ri_metric_eigen(J)
+    assert kept == naux
+    assert np.allclose(Me.cpu().numpy(), Mr, rtol=1e-9, atol=1e-9)
+    Mc, _ = corr.ri_metric(dev(J), "cholesky")
+    assert np.allclose(Mc.cpu().numpy(), O.ri_metric_cholesky(J), rtol=1e-9, atol=1e-9)
+
+
+def test_ri_metric_cutoff(corr):
+    naux = 40
+    q, _ = np.linalg.qr(np.random.default_rng(1).standard_normal((naux, naux)))
+    lam = np.geomspace(1.0, 10.0, naux)
+    lam[:3] = [1e-14, -1e-13, 5e-12]
+    J = (q * lam) @ q.T
+    M, kept = corr.ri_metric(dev(J), "eigen")
+    assert kept == naux - 3
+    Mr, _ = O.ri_metric_eigen(J)
+    assert np.allclose(M.cpu().numpy(), Mr, atol=1e-8)
+
+
+def test_ri_metric_not_pd_raises(corr):
+    J = -np.eye(10)
+    with pytest.raises(corr.CorrError):
+        corr.ri_metric(dev(J), "cholesky")
+
+
+@pytest.mark.parametrize("naux,N,o", [(20, 9, 3), (76, 24, 5), (130, 40, 7)])
+def test_ri_b_formation(corr, naux, N, o):
+    _say(f"ri B naux={naux} N={N} o={o}")
+    rng = np.random.default_rng(N)
+    C = S.orthonormal(N, seed=N)
+    X = rng.standard_normal((naux, N, N)) * 0.1
+    Pmn = 0.5 * (X + X.transpose(0, 2, 1))
+    J = S.metric(naux, seed=o, cond=1e4)
+    Me, _ = O.ri_metric_eigen(J)
+    BPQ = O.ri_bpq(Me, Pmn)
+    ref = O.transform_ri(C, BPQ, o)
+    got_bpq = corr.calculate_ri(dev(J), dev(Pmn), "eigen")
+    assert np.allclose(got_bpq.cpu().numpy(), BPQ, rtol=1e-9, atol=1e-10)
+    got = corr.transform_ri(dev(C), got_bpq, o)
+    assert np.allclose(got.cpu().numpy(), ref, rtol=1e-9, atol=1e-10)
+    # reordered formation (transform first, metric last), both metric kinds
+    Mc = O.ri_metric_cholesky(J)
+    for M, mode in [(Me, "eigen"), (Mc, "cholesky")]:
+        Bf = corr.ri_form_bia(dev(C), dev(Pmn), dev(M), o, mode=mode).cpu().numpy()
+        ref_f = np.einsum("PQ,Qia->Pia", M, O.transform_ri(C, Pmn, o))
+        assert np.allclose(Bf, ref_f, rtol=1e-10, atol=1e-11)
+
+
+def test_build_fmo(corr):
+    N = 37
+    rng = np.random.default_rng(2)
+    C, F = rng.standard_normal((N, N)), rng.standard_normal((N, N))
+    got = corr.build_FMo(dev(C), dev(F)).cpu().numpy()
+    assert np.allclose(got, O.build_fmo(C, F), rtol=1e-12, atol=1e-11)
+
+
+@pytest.mark.parametrize("N,o", [(6, 2), (13, 4), (24, 5)])
+def test_unfold_motrans_canonical_mp2(corr, N, o):
+    _say(f"motrans N={N}")
+    dense = S.eri_like(N, seed=N)
+    val, idx, sc = O.sorted_unique_list(dense)
+    d_idx = torch.from_numpy(idx.view(np.int16).copy()).to("cuda")
+    ao = corr.ao_unfold(N, dev(val), d_idx, sc)
+    assert torch.equal(ao.cpu(), torch.from_numpy(dense))
+    C = S.orthonormal(N, seed=N + 1)
+    ref = O.motrans(C, dense)
+    mo = corr.MOtrans(dev(C), ao)
+    assert np.allclose(mo.cpu().numpy(), ref, rtol=1e-12, atol=1e-13)
+    eo, ev = S.orbital_energies(o, N - o, seed=N)
+    FMo = np.diag(np.concatenate([eo, ev]))
+    e_ref = O.canonical_mp2(ref, FMo, o)
+    e = corr.run_canonical_mp2(mo, dev(FMo), o)
+    for k in e_ref:
+        assert abs(e[k] - e_ref[k]) < 1e-12, k
+    mo32 = corr.MOtrans(dev(C), ao, prec="fp32").cpu().numpy()
+    assert np.max(np.abs(mo32 - ref)) < 1e-5 * np.max(np.abs(ref))
