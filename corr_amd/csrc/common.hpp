@@ -104,3 +104,14 @@ typedef double corr_d4 __attribute__((ext_vector_type(4)));
 typedef float corr_f4 __attribute__((ext_vector_type(4)));
 typedef short corr_s8 __attribute__((ext_vector_type(8)));
 typedef __bf16 corr_bf8 __attribute__((ext_vector_type(8)));
+
+// Row (within the 16x16 output tile) of accumulator element r held by a lane;
+// the column is lane % 16.  FP32 accumulators of the 16x16 MFMAs hold four
+// consecutive rows per lane group; v_mfma_f64_16x16x4_f64 interleaves them
+// (lane group g, element r -> row g + 4 r), as measured on gfx950.
+template <typename ACC> struct AccLayout {
+    __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <> struct AccLayout<corr_d4> {
+    __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};

@@ -91,7 +91,7 @@ int corr_ctx_destroy(corr_ctx* ctx) {
 
 int corr_ctx_set_stream(corr_ctx* ctx, void* hip_stream) {
     CORR_CHECK_CTX(ctx);
-    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    ctx->stream = (hipStream_t)hip_stream;
     return CORR_OK;
 }
 

@@ -134,8 +134,6 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         }
     }
 
-    // Accumulator element r of tile (i,j) sits at row 4*(lane/16)+r,
-    // column lane%16 of that 16x16 tile.
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -143,7 +141,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
             const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
+                const int64_t row = m0 + wm * 64 + i * 16 + AccLayout<acc_t>::row(lane, r);
                 if (row < g.M && col < g.N) {
                     double v = g.alpha * (double)acc[i][j][r];
                     if (g.beta != 0.0) v += g.beta * C[row * g.ldc + col];

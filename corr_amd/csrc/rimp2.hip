@@ -146,7 +146,7 @@ __device__ inline void energy_epilogue(const ACC (&acc1)[4][4], const ACC (&acc2
             const int b = t.BC * PA + (cb % 2) * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int a = t.AC * PA + (rb % 2) * 16 + (lane >> 4) * 4 + r;
+                const int a = t.AC * PA + (rb % 2) * 16 + AccLayout<ACC>::row(lane, r);
                 if (i < o && j < o && a < v && b < v) {
                     const double m1 = (double)acc1[rb][cb][r], m2 = (double)acc2[rb][cb][r];
                     const double dinv = 1.0 / (ev[a] + ev[b] - eo[i] - eo[j]);

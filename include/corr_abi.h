@@ -72,8 +72,8 @@ typedef struct corr_mp2_energy {
 int         corr_version(void);
 int         corr_ctx_create(int device, corr_ctx** out);
 int         corr_ctx_destroy(corr_ctx* ctx);
-/* Use an existing hipStream_t (passed as void*); NULL selects the context's
- * own stream. */
+/* Enqueue on an existing hipStream_t (passed as void*); NULL is the HIP
+ * default (null) stream.  A new context uses a stream of its own. */
 int         corr_ctx_set_stream(corr_ctx* ctx, void* hip_stream);
 int         corr_ctx_synchronize(corr_ctx* ctx);
 const char* corr_ctx_last_error(const corr_ctx* ctx);

@@ -36,3 +36,32 @@ def metric(naux, seed=0, cond=1e4):
     q, _ = np.linalg.qr(rng.standard_normal((naux, naux)))
     lam = np.geomspace(1.0, cond, naux)
     return (q * lam) @ q.T
+
+
+def rhf_orthonormal(h, eri, nocc_spatial, maxiter=200, tol=1e-13):
+    """Closed-shell RHF in an orthonormal basis (S = I): returns (C, eps, F)
+    with C[p, mu] (row = MO) as in the reference layout."""
+    n = h.shape[0]
+    e, Cc = np.linalg.eigh(h)
+    for _ in range(maxiter):
+        Co = Cc[:, :nocc_spatial]
+        P = 2.0 * Co @ Co.T
+        F = h + np.einsum("mnls,ls->mn", eri, P) - 0.5 * np.einsum("mlns,ls->mn", eri, P)
+        e_new, C_new = np.linalg.eigh(F)
+        dP = np.abs(2.0 * C_new[:, :nocc_spatial] @ C_new[:, :nocc_spatial].T - P).max()
+        Cc, e = C_new, e_new
+        if dP < tol:
+            break
+    Co = Cc[:, :nocc_spatial]
+    P = 2.0 * Co @ Co.T
+    F = h + np.einsum("mnls,ls->mn", eri, P) - 0.5 * np.einsum("mlns,ls->mn", eri, P)
+    return Cc.T.copy(), e, F
+
+
+def model_hamiltonian(N, seed=0, eri_scale=0.05):
+    """Symmetric one-electron h with a gap and a small 8-fold symmetric ERI."""
+    rng = np.random.default_rng(seed)
+    h = np.diag(np.linspace(-2.0, 1.5, N)) + 0.05 * rng.standard_normal((N, N))
+    h = 0.5 * (h + h.T)
+    eri = eri_scale * eri_like(N, seed=seed + 1) * N
+    return h, eri
