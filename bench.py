@@ -56,6 +56,20 @@ def water_spectrum(nmol: int, rng: np.random.Generator):
     return np.sort(np.concatenate([core, val])), np.sort(vir)
 
 
+def random_orthogonal(n: int, rng: np.random.Generator, device):
+    """Seeded orthogonal n x n matrix D1 Q P Q D2 built from the orthonormal
+    DCT-II matrix Q, a random permutation P and random sign diagonals D1, D2
+    (closed form: no factorisation library call)."""
+    import torch
+    k = torch.arange(n, dtype=torch.float64, device=device)
+    Q = torch.cos(torch.pi * (k[None, :] + 0.5) * k[:, None] / n) * (2.0 / n) ** 0.5
+    Q[0] *= 0.5 ** 0.5
+    perm = torch.from_numpy(rng.permutation(n)).to(device)
+    d1 = torch.from_numpy(rng.choice([-1.0, 1.0], n)).to(device)
+    d2 = torch.from_numpy(rng.choice([-1.0, 1.0], n)).to(device)
+    return (d1[:, None] * (Q @ Q[perm]) * d2[None, :]).contiguous()
+
+
 def make_inputs(nmol: int, seed: int, device):
     import torch
     N, o, v, naux = sizes(nmol)
@@ -63,9 +77,8 @@ def make_inputs(nmol: int, seed: int, device):
     g.manual_seed(seed)
     rng = np.random.default_rng(seed)
     eo, ev = water_spectrum(nmol, rng)
-    q, _ = torch.linalg.qr(torch.randn(N, N, dtype=torch.float64, device=device, generator=g))
-    C = q.contiguous()
-    Qa, _ = torch.linalg.qr(torch.randn(naux, naux, dtype=torch.float64, device=device, generator=g))
+    C = random_orthogonal(N, rng, device)
+    Qa = random_orthogonal(naux, rng, device)
     lam = torch.logspace(-3, 3, naux, dtype=torch.float64, device=device)
     J = ((Qa * lam) @ Qa.T)
     J = (0.5 * (J + J.T)).contiguous()
@@ -83,10 +96,12 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device-to-device (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(the path shards over occupied pairs, see DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU run would shard the occupied pair list)", file=sys.stderr)
         return 2
 
     import torch
@@ -95,6 +110,13 @@ def main() -> int:
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        x = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+        log("calibration copy of 1 GiB done")
     N, o, v, naux = sizes(args.nmol)
     W_E, W_B = work(N, o, v, naux)
     log(f"(H2O){args.nmol}: N={N} o={o} v={v} naux={naux}; W_E={W_E/1e12:.3f} TF W_B={W_B/1e12:.3f} TF")

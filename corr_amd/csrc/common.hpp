@@ -104,6 +104,7 @@ typedef double corr_d4 __attribute__((ext_vector_type(4)));
 typedef float corr_f4 __attribute__((ext_vector_type(4)));
 typedef short corr_s8 __attribute__((ext_vector_type(8)));
 typedef __bf16 corr_bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int corr_u4 __attribute__((ext_vector_type(4)));
 
 // Row (within the 16x16 output tile) of accumulator element r held by a lane;
 // the column is lane % 16.  FP32 accumulators of the 16x16 MFMAs hold four

@@ -58,18 +58,6 @@ __device__ inline uint16_t f32_to_bf16_rn(float f) {
 }
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-__device__ inline double block_sum(double v, double* sh) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) sh[wave] = v;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0) t = sh[0] + sh[1] + sh[2] + sh[3];
-    return t;
-}
-
 }  // namespace
 
 // ---- repack ----------------------------------------------------------------
@@ -125,25 +113,31 @@ __global__ __launch_bounds__(256) void repack_bf16(const double* __restrict__ Bi
     }
 }
 
-// ---- shared epilogue ---------------------------------------------------------
+// ---- shared pieces -------------------------------------------------------------
+// A workgroup is 8 wave64s (two per SIMD).  Wave (wr, wc), wr in 0..1,
+// wc in 0..3, owns rows (i = 2 wr + {0,1}) x (32 virtuals a) and the column
+// block (j = wc) x (32 virtuals b) of both T1 and T2: 4 x 2 MFMA 16x16 tiles
+// each.  Row block rb = (i offset rb/2, a half rb%2); column block cb = b half.
+constexpr int WG_THREADS = 512;
+
 struct PairTile {
     int IB, JB, AC, BC;
 };
 
 template <typename ACC>
-__device__ inline void energy_epilogue(const ACC (&acc1)[4][4], const ACC (&acc2)[4][4],
+__device__ inline void energy_epilogue(const ACC (&acc1)[4][2], const ACC (&acc2)[4][2],
                                        const PairTile& t, int o, int v,
                                        const double* __restrict__ eo,
                                        const double* __restrict__ ev, double& sos, double& sx) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
+    const int wr = wave >> 2, wc = wave & 3;
+    const int j = t.JB * PI + wc;
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
         const int i = t.IB * PI + 2 * wr + rb / 2;
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            const int j = t.JB * PI + 2 * wc + cb / 2;
-            const int b = t.BC * PA + (cb % 2) * 16 + (lane & 15);
+        for (int cb = 0; cb < 2; ++cb) {
+            const int b = t.BC * PA + cb * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int a = t.AC * PA + (rb % 2) * 16 + AccLayout<ACC>::row(lane, r);
@@ -168,11 +162,24 @@ __device__ inline bool decode_tile(int nIB, int nAC, int64_t nWG, int64_t per, P
     return true;
 }
 
+__device__ inline double block_sum8(double v, double* sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < WG_THREADS / 64; ++w) s += sh[w];
+    return s;
+}
+
 __device__ inline void write_partial(double sos, double sx, const PairTile& t, int64_t id,
                                      double* sh, double* __restrict__ partial) {
     const double w = (t.IB < t.JB ? 2.0 : 1.0) * (t.AC < t.BC ? 1.0 : 0.5);
-    const double os = block_sum(sos, sh);
-    const double x = block_sum(sx, sh);
+    const double os = block_sum8(sos, sh);
+    const double x = block_sum8(sx, sh);
     if (threadIdx.x == 0) {
         partial[id * 2 + 0] = -w * os;
         partial[id * 2 + 1] = 2.0 * w * x;
@@ -183,53 +190,63 @@ __device__ inline void write_partial(double sos, double sx, const PairTile& t, i
 template <typename T> struct RealMfma;
 template <> struct RealMfma<double> {
     typedef corr_d4 acc_t;
+    static constexpr int KC = 16;   // 4 MFMA k-steps of 64 cycles per slab
     __device__ static inline acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
 };
 template <> struct RealMfma<float> {
     typedef corr_f4 acc_t;
+    static constexpr int KC = 32;   // 8 MFMA k-steps of 32 cycles per slab
     __device__ static inline acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void rimp2_energy_real(
+__global__ __launch_bounds__(WG_THREADS) void rimp2_energy_real(
     const T* __restrict__ Bp, const double* __restrict__ eo, const double* __restrict__ ev,
     int o, int v, int nIB, int nAC, int64_t Kp, int64_t nWG, int64_t per,
     double* __restrict__ partial) {
-    // FP64: 8-deep K slabs keep the prefetch registers within budget next to
-    // the 2 x 64 FP64 accumulators; FP32: 16-deep.
-    constexpr int KC = sizeof(T) == 8 ? 8 : 16, LD = PW + 16, EL = KC * PW / 256;
+    constexpr int KC = RealMfma<T>::KC, LD = PW + 16, EL = KC * PW / WG_THREADS;
     __shared__ T L[4][KC][LD];
-    __shared__ double sh[4];
+    __shared__ double sh[WG_THREADS / 64];
     typedef typename RealMfma<T>::acc_t acc_t;
 
     PairTile t;
     int64_t id;
     if (!decode_tile(nIB, nAC, nWG, per, t, id)) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1, cl = lane & 15;
+    const int wr = wave >> 2, wc = wave & 3, cl = lane & 15;
 
     const T* P[4];
-    P[0] = Bp + ((int64_t)t.IB * nAC + t.AC) * Kp * PW;
-    P[1] = Bp + ((int64_t)t.JB * nAC + t.BC) * Kp * PW;
-    P[2] = Bp + ((int64_t)t.JB * nAC + t.AC) * Kp * PW;
-    P[3] = Bp + ((int64_t)t.IB * nAC + t.BC) * Kp * PW;
+    P[0] = Bp + ((int64_t)t.IB * nAC + t.AC) * Kp * PW;   // (I, A)
+    P[1] = Bp + ((int64_t)t.JB * nAC + t.BC) * Kp * PW;   // (J, B)
+    P[2] = Bp + ((int64_t)t.JB * nAC + t.AC) * Kp * PW;   // (J, A)
+    P[3] = Bp + ((int64_t)t.IB * nAC + t.BC) * Kp * PW;   // (I, B)
 
-    acc_t acc1[4][4], acc2[4][4];
+    acc_t acc1[4][2], acc2[4][2];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) { acc1[x][y] = acc_t{0, 0, 0, 0}; acc2[x][y] = acc_t{0, 0, 0, 0}; }
+        for (int y = 0; y < 2; ++y) { acc1[x][y] = acc_t{0, 0, 0, 0}; acc2[x][y] = acc_t{0, 0, 0, 0}; }
 
     T reg[4][EL];
     const int64_t nK = Kp / KC;
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
-        for (int u = 0; u < EL; ++u) reg[p][u] = P[p][u * 256 + threadIdx.x];
+        for (int u = 0; u < EL; ++u) reg[p][u] = P[p][u * WG_THREADS + threadIdx.x];
+
+    // fragment columns inside the 128-wide panels
+    int c1[4], c4[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) c1[rb] = (2 * wr + rb / 2) * PA + (rb % 2) * 16 + cl;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) c4[x][y] = (2 * wr + x) * PA + y * 16 + cl;
+    const int c2 = wc * PA + cl;   // + cb * 16 (P2) and + a16 * 16 (P3)
 
     for (int64_t kc = 0; kc < nK; ++kc) {
         __syncthreads();
@@ -237,7 +254,7 @@ __global__ __launch_bounds__(256) void rimp2_energy_real(
         for (int p = 0; p < 4; ++p)
 #pragma unroll
             for (int u = 0; u < EL; ++u) {
-                const int e = u * 256 + threadIdx.x;
+                const int e = u * WG_THREADS + threadIdx.x;
                 L[p][e / PW][e % PW] = reg[p][u];
             }
         __syncthreads();
@@ -246,29 +263,29 @@ __global__ __launch_bounds__(256) void rimp2_energy_real(
 #pragma unroll
             for (int p = 0; p < 4; ++p)
 #pragma unroll
-                for (int u = 0; u < EL; ++u) reg[p][u] = P[p][off + u * 256 + threadIdx.x];
+                for (int u = 0; u < EL; ++u) reg[p][u] = P[p][off + u * WG_THREADS + threadIdx.x];
         }
-#pragma unroll
+#pragma unroll 2
         for (int kk = 0; kk < KC; kk += 4) {
             const int kr = kk + (lane >> 4);
-            T f1[4], f2[4], f3[2][2], f4[2][2];
+            T f1[4], f2[2], f3[2], f4[2][2];
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) f1[rb] = L[0][kr][(2 * wr + rb / 2) * PA + (rb % 2) * 16 + cl];
+            for (int rb = 0; rb < 4; ++rb) f1[rb] = L[0][kr][c1[rb]];
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) f2[cb] = L[1][kr][(2 * wc + cb / 2) * PA + (cb % 2) * 16 + cl];
+            for (int y = 0; y < 2; ++y) {
+                f2[y] = L[1][kr][c2 + y * 16];
+                f3[y] = L[2][kr][c2 + y * 16];
+            }
 #pragma unroll
             for (int x = 0; x < 2; ++x)
 #pragma unroll
-                for (int y = 0; y < 2; ++y) {
-                    f3[x][y] = L[2][kr][(2 * wc + x) * PA + y * 16 + cl];
-                    f4[x][y] = L[3][kr][(2 * wr + x) * PA + y * 16 + cl];
-                }
+                for (int y = 0; y < 2; ++y) f4[x][y] = L[3][kr][c4[x][y]];
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb) {
+                for (int cb = 0; cb < 2; ++cb) {
                     acc1[rb][cb] = RealMfma<T>::mma(f1[rb], f2[cb], acc1[rb][cb]);
-                    acc2[rb][cb] = RealMfma<T>::mma(f3[cb / 2][rb % 2], f4[rb / 2][cb % 2], acc2[rb][cb]);
+                    acc2[rb][cb] = RealMfma<T>::mma(f3[rb % 2], f4[rb / 2][cb], acc2[rb][cb]);
                 }
         }
     }
@@ -279,106 +296,115 @@ __global__ __launch_bounds__(256) void rimp2_energy_real(
 
 // ---- BF16x3 kernel -------------------------------------------------------------
 // M ~= hi.hi + hi.lo + lo.hi with hi = bf16(x), lo = bf16(x - hi), FP32 accumulate.
-__device__ inline corr_f4 mma_bf16(const uint4& a, const uint4& b, corr_f4 c) {
+__device__ inline corr_f4 mma_bf16(const corr_u4& a, const corr_u4& b, corr_f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(corr_bf8, a),
                                                    __builtin_bit_cast(corr_bf8, b), c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256) void rimp2_energy_bf16x3(
+__device__ inline corr_f4 mma3(const corr_u4& ah, const corr_u4& al, const corr_u4& bh, const corr_u4& bl,
+                               corr_f4 c) {
+    c = mma_bf16(al, bh, c);
+    c = mma_bf16(ah, bl, c);
+    return mma_bf16(ah, bh, c);
+}
+
+// One slab = two 32-deep K sub-slabs (each [hi/lo][128][32] contiguous in
+// the panel); LDS rows hold 64 k values + 8 pad (conflict-free 16 B reads).
+__global__ __launch_bounds__(WG_THREADS) void rimp2_energy_bf16x3(
     const uint16_t* __restrict__ Bp, const double* __restrict__ eo, const double* __restrict__ ev,
     int o, int v, int nIB, int nAC, int64_t Kc, int64_t nWG, int64_t per,
     double* __restrict__ partial) {
-    constexpr int LDK = 40;              // 32 bf16 + 8 pad: conflict-free 16 B reads
-    constexpr int SLAB = 2 * PW * 32;    // bf16 per panel per K slab (hi + lo)
+    constexpr int KS = 2;                // 32-deep sub-slabs per slab
+    constexpr int LDK = 32 * KS + 8;
+    constexpr int SUB = 2 * PW * 32;     // bf16 per panel per sub-slab (hi + lo)
+    constexpr int UPT = SUB / 8 / WG_THREADS;   // 16-byte units per thread per sub-slab (=2)
     __shared__ __attribute__((aligned(16))) uint16_t L[4][2][PW][LDK];
-    __shared__ double sh[4];
+    __shared__ double sh[WG_THREADS / 64];
 
     PairTile t;
     int64_t id;
     if (!decode_tile(nIB, nAC, nWG, per, t, id)) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1, cl = lane & 15, kq = lane >> 4;
+    const int wr = wave >> 2, wc = wave & 3, cl = lane & 15, kq = lane >> 4;
 
     const uint16_t* P[4];
-    P[0] = Bp + ((int64_t)t.IB * nAC + t.AC) * Kc * SLAB;
-    P[1] = Bp + ((int64_t)t.JB * nAC + t.BC) * Kc * SLAB;
-    P[2] = Bp + ((int64_t)t.JB * nAC + t.AC) * Kc * SLAB;
-    P[3] = Bp + ((int64_t)t.IB * nAC + t.BC) * Kc * SLAB;
+    P[0] = Bp + ((int64_t)t.IB * nAC + t.AC) * Kc * SUB;
+    P[1] = Bp + ((int64_t)t.JB * nAC + t.BC) * Kc * SUB;
+    P[2] = Bp + ((int64_t)t.JB * nAC + t.AC) * Kc * SUB;
+    P[3] = Bp + ((int64_t)t.IB * nAC + t.BC) * Kc * SUB;
 
-    corr_f4 acc1[4][4], acc2[4][4];
+    corr_f4 acc1[4][2], acc2[4][2];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) { acc1[x][y] = corr_f4{0, 0, 0, 0}; acc2[x][y] = corr_f4{0, 0, 0, 0}; }
+        for (int y = 0; y < 2; ++y) { acc1[x][y] = corr_f4{0, 0, 0, 0}; acc2[x][y] = corr_f4{0, 0, 0, 0}; }
 
-    uint4 reg[4][4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            reg[p][u] = *reinterpret_cast<const uint4*>(P[p] + (int64_t)(u * 256 + threadIdx.x) * 8);
+    const int64_t nK = Kc / KS;   // Kc is a multiple of KS
+    corr_u4 reg[4][KS][UPT];
+#define CORR_BF16_LOAD(kc_)                                                                     \
+    _Pragma("unroll") for (int p = 0; p < 4; ++p)                                               \
+    _Pragma("unroll") for (int s = 0; s < KS; ++s)                                              \
+    _Pragma("unroll") for (int u = 0; u < UPT; ++u)                                             \
+        reg[p][s][u] = *reinterpret_cast<const corr_u4*>(                                          \
+            P[p] + ((kc_) * KS + s) * SUB + (int64_t)(u * WG_THREADS + threadIdx.x) * 8);
+    CORR_BF16_LOAD(0)
 
-    for (int64_t kc = 0; kc < Kc; ++kc) {
+    int c1[4], c4[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) c1[rb] = (2 * wr + rb / 2) * PA + (rb % 2) * 16 + cl;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) c4[x][y] = (2 * wr + x) * PA + y * 16 + cl;
+    const int c2 = wc * PA + cl;
+
+    for (int64_t kc = 0; kc < nK; ++kc) {
         __syncthreads();
 #pragma unroll
         for (int p = 0; p < 4; ++p)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int e = u * 256 + threadIdx.x;
-                const int hl = e / (PW * 4), c = (e % (PW * 4)) / 4, q = e % 4;
-                *reinterpret_cast<uint4*>(&L[p][hl][c][q * 8]) = reg[p][u];
-            }
+            for (int s = 0; s < KS; ++s)
+#pragma unroll
+                for (int u = 0; u < UPT; ++u) {
+                    const int e = u * WG_THREADS + threadIdx.x;
+                    const int hl = e / (PW * 4), c = (e % (PW * 4)) / 4, q = e % 4;
+                    *reinterpret_cast<corr_u4*>(&L[p][hl][c][s * 32 + q * 8]) = reg[p][s][u];
+                }
         __syncthreads();
-        if (kc + 1 < Kc) {
-            const int64_t off = (kc + 1) * SLAB;
+        if (kc + 1 < nK) { CORR_BF16_LOAD(kc + 1) }
 #pragma unroll
-            for (int p = 0; p < 4; ++p)
+        for (int s = 0; s < KS; ++s) {
+            const int ko = s * 32 + kq * 8;
+#define frag(p_, hl_, c_) (*reinterpret_cast<const corr_u4*>(&L[p_][hl_][c_][ko]))
+            {
+                corr_u4 ah[4], al[4], bh[2], bl[2];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    reg[p][u] = *reinterpret_cast<const uint4*>(P[p] + off + (int64_t)(u * 256 + threadIdx.x) * 8);
-        }
-        uint4 f1h[4], f1l[4], f2h[4], f2l[4];
+                for (int rb = 0; rb < 4; ++rb) { ah[rb] = frag(0, 0, c1[rb]); al[rb] = frag(0, 1, c1[rb]); }
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-            const int c = (2 * wr + rb / 2) * PA + (rb % 2) * 16 + cl;
-            f1h[rb] = *reinterpret_cast<const uint4*>(&L[0][0][c][kq * 8]);
-            f1l[rb] = *reinterpret_cast<const uint4*>(&L[0][1][c][kq * 8]);
-        }
+                for (int y = 0; y < 2; ++y) { bh[y] = frag(1, 0, c2 + y * 16); bl[y] = frag(1, 1, c2 + y * 16); }
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            const int c = (2 * wc + cb / 2) * PA + (cb % 2) * 16 + cl;
-            f2h[cb] = *reinterpret_cast<const uint4*>(&L[1][0][c][kq * 8]);
-            f2l[cb] = *reinterpret_cast<const uint4*>(&L[1][1][c][kq * 8]);
-        }
+                for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-                acc1[rb][cb] = mma_bf16(f1l[rb], f2h[cb], acc1[rb][cb]);
-                acc1[rb][cb] = mma_bf16(f1h[rb], f2l[cb], acc1[rb][cb]);
-                acc1[rb][cb] = mma_bf16(f1h[rb], f2h[cb], acc1[rb][cb]);
+                    for (int cb = 0; cb < 2; ++cb) acc1[rb][cb] = mma3(ah[rb], al[rb], bh[cb], bl[cb], acc1[rb][cb]);
             }
-        uint4 f3h[2][2], f3l[2][2], f4h[2][2], f4l[2][2];
+            {
+                corr_u4 ah[2], al[2], bh[2][2], bl[2][2];
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+                for (int y = 0; y < 2; ++y) { ah[y] = frag(2, 0, c2 + y * 16); al[y] = frag(2, 1, c2 + y * 16); }
 #pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                const int c3 = (2 * wc + x) * PA + y * 16 + cl;
-                const int c4 = (2 * wr + x) * PA + y * 16 + cl;
-                f3h[x][y] = *reinterpret_cast<const uint4*>(&L[2][0][c3][kq * 8]);
-                f3l[x][y] = *reinterpret_cast<const uint4*>(&L[2][1][c3][kq * 8]);
-                f4h[x][y] = *reinterpret_cast<const uint4*>(&L[3][0][c4][kq * 8]);
-                f4l[x][y] = *reinterpret_cast<const uint4*>(&L[3][1][c4][kq * 8]);
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) { bh[x][y] = frag(3, 0, c4[x][y]); bl[x][y] = frag(3, 1, c4[x][y]); }
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc2[rb][cb] = mma3(ah[rb % 2], al[rb % 2], bh[rb / 2][cb], bl[rb / 2][cb], acc2[rb][cb]);
             }
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-                acc2[rb][cb] = mma_bf16(f3l[cb / 2][rb % 2], f4h[rb / 2][cb % 2], acc2[rb][cb]);
-                acc2[rb][cb] = mma_bf16(f3h[cb / 2][rb % 2], f4l[rb / 2][cb % 2], acc2[rb][cb]);
-                acc2[rb][cb] = mma_bf16(f3h[cb / 2][rb % 2], f4h[rb / 2][cb % 2], acc2[rb][cb]);
-            }
+        }
     }
+#undef frag
+#undef CORR_BF16_LOAD
     double sos = 0.0, sx = 0.0;
     energy_epilogue(acc1, acc2, t, o, v, eo, ev, sos, sx);
     write_partial(sos, sx, t, id, sh, partial);
@@ -394,7 +420,7 @@ static RiLayout ri_layout(int64_t naux, int64_t o, int64_t v) {
     RiLayout l;
     l.nIB = (int)cdiv(o, PI);
     l.nAC = (int)cdiv(v, PA);
-    l.Kp = rup(naux, 32);
+    l.Kp = rup(naux, 64);
     l.nWG = (int64_t)l.nIB * (l.nIB + 1) / 2 * ((int64_t)l.nAC * (l.nAC + 1) / 2);
     l.per = cdiv(l.nWG, 8);
     return l;
@@ -440,7 +466,7 @@ int rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const 
     CORR_TRY(ri_repack(ctx, l, naux, (int)nocc, (int)nvir, Bia, prec, &Bp));
     double* partial = nullptr;
     CORR_TRY(scratch(ctx, "rimp2_partial", sizeof(double) * 2 * l.nWG, (void**)&partial));
-    const dim3 grid((unsigned)(l.per * 8)), block(256);
+    const dim3 grid((unsigned)(l.per * 8)), block(WG_THREADS);
     KernelTimer timer(ctx, "rimp2_energy");
     if (prec == CORR_PREC_FP64)
         hipLaunchKernelGGL(rimp2_energy_real<double>, grid, block, 0, ctx->stream, (const double*)Bp,
