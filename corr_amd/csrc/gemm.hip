@@ -6,8 +6,8 @@
 // FP32 path: operands rounded to FP32 when staged into LDS,
 //            v_mfma_f32_16x16x4_f32, FP32 accumulate, epilogue in FP64.
 //
-// A workgroup is 4 wave64s; each wave owns a 64x64 block of C built from
-// 4x4 MFMA 16x16 tiles.  Tiles of A and B are staged through LDS as [k][m]
+// A workgroup is 8 wave64s (two per SIMD); each wave owns a 64x32 block of C
+// built from 4x2 MFMA 16x16 tiles.  Tiles of A and B are staged through LDS as [k][m]
 // (m contiguous, padded so that a wave's fragment read touches all 64 banks
 // once) with the next K-slab prefetched into registers while the current one
 // is consumed.  Global loads are lane-contiguous along whichever dimension
@@ -43,14 +43,14 @@ template <> struct Mfma<float> {
 // Loads one BK x BDIM slab of an operand into registers.
 //  CONTIG_ALONG_K == false: element (k, x) at P[k*ld + x]  (x contiguous)
 //  CONTIG_ALONG_K == true : element (k, x) at P[x*ld + k]  (k contiguous)
-template <int BDIM, int BK, bool CONTIG_ALONG_K, typename T>
+template <int BDIM, int BK, bool CONTIG_ALONG_K, int NT, typename T>
 __device__ inline void load_slab(const double* __restrict__ P, int64_t ld, int64_t x0,
                                  int64_t xmax, int64_t k0, int64_t kmax, T* regs) {
-    constexpr int ELEMS = BDIM * BK / 256;
+    constexpr int ELEMS = BDIM * BK / NT;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int u = 0; u < ELEMS; ++u) {
-        const int e = u * 256 + tid;
+        const int e = u * NT + tid;
         int kk, xx;
         if (CONTIG_ALONG_K) { xx = e / BK; kk = e % BK; }
         else                { kk = e / BDIM; xx = e % BDIM; }
@@ -62,13 +62,13 @@ __device__ inline void load_slab(const double* __restrict__ P, int64_t ld, int64
     }
 }
 
-template <int BDIM, int BK, bool CONTIG_ALONG_K, int LDW, typename T>
+template <int BDIM, int BK, bool CONTIG_ALONG_K, int LDW, int NT, typename T>
 __device__ inline void store_slab(T (*S)[LDW], const T* regs) {
-    constexpr int ELEMS = BDIM * BK / 256;
+    constexpr int ELEMS = BDIM * BK / NT;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int u = 0; u < ELEMS; ++u) {
-        const int e = u * 256 + tid;
+        const int e = u * NT + tid;
         int kk, xx;
         if (CONTIG_ALONG_K) { xx = e / BK; kk = e % BK; }
         else                { kk = e / BDIM; xx = e % BDIM; }
@@ -76,10 +76,17 @@ __device__ inline void store_slab(T (*S)[LDW], const T* regs) {
     }
 }
 
+template <typename T> struct GemmSlab;
+template <> struct GemmSlab<double> { static constexpr int BK = 16; };
+template <> struct GemmSlab<float> { static constexpr int BK = 32; };
+
+// 8 waves (two per SIMD) in a WM x WN grid; each wave owns 64 x 32 of C
+// (4 x 2 MFMA tiles).  BM = 64 WM, BN = 32 WN.
 template <int TA, int TB, typename T, int WM, int WN>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-    constexpr int BM = 64 * WM, BN = 64 * WN, BK = 16, PAD = 16;
-    constexpr int EA = BM * BK / 256, EB = BN * BK / 256;
+__global__ __launch_bounds__(512) void gemm_kernel(GemmArgs g) {
+    constexpr int NT = 512;
+    constexpr int BM = 64 * WM, BN = 32 * WN, BK = GemmSlab<T>::BK, PAD = 16;
+    constexpr int EA = BM * BK / NT, EB = BN * BK / NT;
     __shared__ T As[BK][BM + PAD];
     __shared__ T Bs[BK][BN + PAD];
     typedef typename Mfma<T>::acc_t acc_t;
@@ -97,48 +104,48 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     if (g.a_lower) kEnd = (m0 + BM < g.K) ? (m0 + BM) : g.K;
     const int64_t kTiles = (kEnd + BK - 1) / BK;
 
-    acc_t acc[4][4];
+    acc_t acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+        for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     T ra[EA], rb[EB];
     // op(A)(m,k): TA==0 -> A[m*lda+k] (k contiguous); TA==1 -> A[k*lda+m]
     // op(B)(k,n): TB==0 -> B[k*ldb+n] (n contiguous); TB==1 -> B[n*ldb+k]
     if (kTiles > 0) {
-        load_slab<BM, BK, TA == 0>(A, g.lda, m0, g.M, 0, kEnd, ra);
-        load_slab<BN, BK, TB == 1>(B, g.ldb, n0, g.N, 0, kEnd, rb);
+        load_slab<BM, BK, TA == 0, NT>(A, g.lda, m0, g.M, 0, kEnd, ra);
+        load_slab<BN, BK, TB == 1, NT>(B, g.ldb, n0, g.N, 0, kEnd, rb);
     }
     for (int64_t kt = 0; kt < kTiles; ++kt) {
         __syncthreads();
-        store_slab<BM, BK, TA == 0, BM + PAD>(As, ra);
-        store_slab<BN, BK, TB == 1, BN + PAD>(Bs, rb);
+        store_slab<BM, BK, TA == 0, BM + PAD, NT>(As, ra);
+        store_slab<BN, BK, TB == 1, BN + PAD, NT>(Bs, rb);
         __syncthreads();
         if (kt + 1 < kTiles) {
-            load_slab<BM, BK, TA == 0>(A, g.lda, m0, g.M, (kt + 1) * BK, kEnd, ra);
-            load_slab<BN, BK, TB == 1>(B, g.ldb, n0, g.N, (kt + 1) * BK, kEnd, rb);
+            load_slab<BM, BK, TA == 0, NT>(A, g.lda, m0, g.M, (kt + 1) * BK, kEnd, ra);
+            load_slab<BN, BK, TB == 1, NT>(B, g.ldb, n0, g.N, (kt + 1) * BK, kEnd, rb);
         }
-#pragma unroll
+#pragma unroll 2
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + (lane >> 4);
-            T fa[4], fb[4];
+            T fa[4], fb[2];
 #pragma unroll
             for (int i = 0; i < 4; ++i) fa[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) fb[j] = Bs[kr][wn * 64 + j * 16 + (lane & 15)];
+            for (int j = 0; j < 2; ++j) fb[j] = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mma(fa[i], fb[j], acc[i][j]);
+                for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(fa[i], fb[j], acc[i][j]);
         }
     }
 
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
+        for (int j = 0; j < 2; ++j) {
+            const int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t row = m0 + wm * 64 + i * 16 + AccLayout<acc_t>::row(lane, r);
@@ -153,8 +160,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 
 template <int TA, int TB, typename T>
 static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
-    // Narrow-N problems (N <= 64, or a multiple of 64 that 128 does not
-    // divide) use 256x64 tiles; the rest 128x128.
+    // N <= 64, or a multiple of 64 that 128 does not divide: 256 x 64 tiles;
+    // otherwise 128 x 128.
     const bool tall = (a.N <= 64) || (a.N % 128 != 0 && a.N % 64 == 0);
     const int BM = tall ? 256 : 128, BN = tall ? 64 : 128;
     a.tilesM = cdiv(a.M, BM);
@@ -167,9 +174,9 @@ static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
         dim3 grid((unsigned)tiles, (unsigned)nb);
         KernelTimer timer(ctx, "gemm");
         if (tall)
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 1>), grid, dim3(256), 0, ctx->stream, s);
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2>), grid, dim3(512), 0, ctx->stream, s);
         else
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 2>), grid, dim3(256), 0, ctx->stream, s);
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 4>), grid, dim3(512), 0, ctx->stream, s);
         CORR_LAUNCHED(ctx, "gemm_kernel");
     }
     return CORR_OK;

@@ -6,7 +6,8 @@
 // CORR_METRIC_CHOLESKY: J = L L^T by a blocked right-looking Cholesky on the
 //                       GPU (64x64 diagonal blocks factored and inverted in
 //                       LDS, panel and trailing updates on the MFMA GEMM),
-//                       then L^-1 by blocked forward substitution.  Both give
+//                       then L^-1 by recursive halving (two GEMMs per split,
+//                       X21 = -X22 L21 X11).  Both give
 //                       M with M^T M = J^-1, hence identical B^T B products.
 #include "common.hpp"
 #include <rocblas/rocblas.h>
@@ -21,12 +22,17 @@ constexpr int NB = 64;
 
 // Factor the kb x kb diagonal block at A[k0,k0] (lda) in place (lower) and
 // write its inverse (lower, kb x kb, ld NB) to Linv.  Sets *notpd on failure.
+// Right-looking, unscaled: step j subtracts S[i][j] S[l][j] / S[j][j] from the
+// trailing triangle (it only reads column j, which step j does not write, so
+// one barrier per step suffices); the columns are scaled by 1/sqrt(S[j][j])
+// at the end.
 __global__ __launch_bounds__(256) void chol_diag_kernel(double* __restrict__ A, int64_t lda,
                                                         int64_t k0, int kb,
                                                         double* __restrict__ Linv,
                                                         int* __restrict__ notpd) {
     __shared__ double S[NB][NB + 1];
     __shared__ double X[NB][NB + 1];
+    __shared__ double dsq[NB];
     __shared__ int bad;
     const int tid = threadIdx.x;
     if (tid == 0) bad = 0;
@@ -37,30 +43,39 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(double* __restrict__ A, 
     }
     __syncthreads();
     for (int j = 0; j < kb; ++j) {
-        if (tid == 0) {
-            const double d = S[j][j];
-            if (!(d > 0.0)) { bad = 1; S[j][j] = 1.0; }
-            else S[j][j] = sqrt(d);
+        double dj = S[j][j];
+        if (!(dj > 0.0)) {
+            if (tid == 0) bad = 1;
+            dj = 1.0;
         }
-        __syncthreads();
-        const double djj = S[j][j];
-        for (int i = j + 1 + tid; i < kb; i += 256) S[i][j] /= djj;
-        __syncthreads();
+        const double inv = 1.0 / dj;
         const int m = kb - j - 1;
         for (int e = tid; e < m * m; e += 256) {
             const int i = j + 1 + e / m, l = j + 1 + e % m;
-            if (l <= i) S[i][l] -= S[i][j] * S[l][j];
+            if (l <= i) S[i][l] -= S[i][j] * S[l][j] * inv;
         }
         __syncthreads();
     }
+    if (tid < kb) {
+        const double d = S[tid][tid];
+        dsq[tid] = d > 0.0 ? sqrt(d) : 1.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < kb * kb; e += 256) {
+        const int r = e / kb, c = e % kb;
+        if (c < r) S[r][c] /= dsq[c];
+    }
+    __syncthreads();
+    if (tid < kb) S[tid][tid] = dsq[tid];
+    __syncthreads();
     // Inverse of the lower factor, one column per thread.
     if (tid < kb) {
         const int c = tid;
         X[c][c] = 1.0 / S[c][c];
         for (int i = c + 1; i < kb; ++i) {
-            double s = 0.0;
-            for (int k = c; k < i; ++k) s += S[i][k] * X[k][c];
-            X[i][c] = -s / S[i][i];
+            double acc = 0.0;
+            for (int k = c; k < i; ++k) acc += S[i][k] * X[k][c];
+            X[i][c] = -acc / S[i][i];
         }
     }
     __syncthreads();
@@ -115,12 +130,32 @@ unsigned grid_for(int64_t n) {
 }
 }  // namespace
 
+// X[r0:r0+nr, r0:r0+nr] = inverse of the lower factor block of L (ld n);
+// the block spans nb whole NB-blocks starting at block r0 / NB.
+static int trinv_rec(corr_ctx* ctx, const double* L, int64_t n, int64_t r0, int64_t nb, int64_t nr,
+                     double* X, const double* Linv, double* T) {
+    if (nb == 1) {
+        hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for(nr * nr)), dim3(256), 0, ctx->stream,
+                           Linv + (r0 / NB) * NB * NB, (int64_t)NB, X + r0 * n + r0, n, nr, nr);
+        CORR_LAUNCHED(ctx, "copy2d_kernel");
+        return CORR_OK;
+    }
+    const int64_t nb1 = (nb + 1) / 2, n1 = nb1 * NB, n2 = nr - n1;
+    CORR_TRY(trinv_rec(ctx, L, n, r0, nb1, n1, X, Linv, T));
+    CORR_TRY(trinv_rec(ctx, L, n, r0 + n1, nb - nb1, n2, X, Linv, T));
+    // T (n2 x n1) = L21 X11 ;  X21 = -X22 T  (X22 lower triangular)
+    CORR_TRY(gemm(ctx, 0, 0, n2, n1, n1, 1.0, L + (r0 + n1) * n + r0, n, 0, X + r0 * n + r0, n, 0,
+                  0.0, T, n1, 0, 1, CORR_PREC_FP64));
+    CORR_TRY(gemm(ctx, 0, 0, n2, n1, n2, -1.0, X + (r0 + n1) * n + (r0 + n1), n, 0, T, n1, 0, 0.0,
+                  X + (r0 + n1) * n + r0, n, 0, 1, CORR_PREC_FP64, true));
+    return CORR_OK;
+}
+
 static int cholesky_inverse(corr_ctx* ctx, int64_t n, const double* J, double* M) {
-    double *A = nullptr, *P = nullptr, *Linv = nullptr, *R = nullptr;
+    double *A = nullptr, *P = nullptr, *Linv = nullptr;
     int* flag = nullptr;
     CORR_TRY(scratch(ctx, "chol_A", sizeof(double) * n * n, (void**)&A));
     CORR_TRY(scratch(ctx, "chol_P", sizeof(double) * n * NB, (void**)&P));
-    CORR_TRY(scratch(ctx, "chol_R", sizeof(double) * n * NB, (void**)&R));
     const int64_t nblk = cdiv(n, NB);
     CORR_TRY(scratch(ctx, "chol_Linv", sizeof(double) * NB * NB * nblk, (void**)&Linv));
     CORR_TRY(scratch(ctx, "chol_flag", sizeof(int) * 4, (void**)&flag));
@@ -146,31 +181,13 @@ static int cholesky_inverse(corr_ctx* ctx, int64_t n, const double* J, double* M
     }
     hipLaunchKernelGGL(zero_upper_kernel, dim3(grid_for(n * n)), dim3(256), 0, ctx->stream, A, n);
     CORR_LAUNCHED(ctx, "zero_upper_kernel");
-    // X = L^-1 by block rows: X_k = Linv_kk * (I_k - L[k, :k] X[:k, :])
+    // X = L^-1 by recursive halving over whole diagonal blocks:
+    //   inv([[L11, 0], [L21, L22]]) = [[X11, 0], [-X22 L21 X11, X22]]
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n * n)), dim3(256), 0, ctx->stream, M, n * n, 0.0);
     CORR_LAUNCHED(ctx, "fill_kernel");
-    for (int64_t kb0 = 0; kb0 < nblk; ++kb0) {
-        const int64_t k0 = kb0 * NB;
-        const int kb = (int)std::min<int64_t>(NB, n - k0);
-        const int64_t w = k0 + kb;   // X[k, c] = 0 for c >= w
-        hipLaunchKernelGGL(fill_kernel, dim3(grid_for((int64_t)kb * NB)), dim3(256), 0, ctx->stream,
-                           R, (int64_t)kb * NB, 0.0);
-        CORR_LAUNCHED(ctx, "fill_kernel");
-        // R = -L[k, :k0] X[:k0, :k0]  (kb x k0), then + I on the diagonal block
-        if (k0 > 0) {
-            CORR_TRY(gemm(ctx, 0, 0, kb, k0, k0, -1.0, A + k0 * n, n, 0, M, n, 0, 0.0, R, w, 0, 1,
-                          CORR_PREC_FP64));
-        }
-        // X[k, :w] = Linv_kk * R  with R's identity block folded in:
-        // X[k, :k0] = Linv * R[:, :k0];  X[k, k0:w] = Linv
-        if (k0 > 0)
-            CORR_TRY(gemm(ctx, 0, 0, kb, k0, kb, 1.0, Linv + kb0 * NB * NB, NB, 0, R, w, 0, 0.0,
-                          M + k0 * n, n, 0, 1, CORR_PREC_FP64));
-        hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for((int64_t)kb * kb)), dim3(256), 0, ctx->stream,
-                           Linv + kb0 * NB * NB, (int64_t)NB, M + k0 * n + k0, n, (int64_t)kb,
-                           (int64_t)kb);
-        CORR_LAUNCHED(ctx, "copy2d_kernel");
-    }
+    double* T = nullptr;
+    CORR_TRY(scratch(ctx, "chol_T", sizeof(double) * (n / 2 + NB) * (n / 2 + NB), (void**)&T));
+    CORR_TRY(trinv_rec(ctx, A, n, 0, nblk, n, M, Linv, T));
     int h_flag = 0;
     CORR_HIP(ctx, hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
