@@ -8,4 +8,6 @@ from ._abi import (CorrError, CorrLibraryError, LIB_PATH, PREC_BF16X3, PREC_FP32
 from .posthf import (MOtrans, ao_unfold, build_FMo, calculate_ri, dgemm, ri_form_bia,  # noqa: F401
                      ri_metric, rimp2_energy, run_canonical_mp2, run_canonical_mp2_ri, transform_ri)
 
+from .ccsd import ccsd_energy, ccsd_update, ccsd_ur, ints_so  # noqa: F401
+
 __version__ = "0.1.0"

@@ -1,0 +1,620 @@
+// Spin-orbital CCSD (Stanton & Gauss) on CDNA4: ccsd_ur and its helpers
+// (src/POSTHF/ccsd.cc:6-580, src/POSTHF/ccsd_intermediates.cc:6-287).
+//
+// Every contraction of an iteration is one MFMA GEMM (FP64 or FP32) on
+// contiguous blocks of <pq||rs>; index permutations that a GEMM cannot absorb
+// through its leading dimensions and batch strides are done by one generic
+// 4-D gather-accumulate kernel.  Amplitude updates, denominators, energies and
+// DIIS stay FP64.  Spin orbital p is spatial p/2 with spin p%2; the nocc
+// lowest spin orbitals are occupied (ccsd.cc:540-543).
+#include "common.hpp"
+#include <cmath>
+#include <vector>
+
+namespace corr {
+
+namespace {
+
+unsigned grid_for(int64_t n) {
+    int64_t g = cdiv(n, 256);
+    return (unsigned)(g > 65536 * 4 ? 65536 * 4 : (g < 1 ? 1 : g));
+}
+
+#define GRID_STRIDE(e, n) \
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (n); e += (int64_t)gridDim.x * blockDim.x)
+
+// <pq||rs> = (pr|qs) d(sp,sr) d(sq,ss) - (ps|qr) d(sp,ss) d(sq,sr)   (ccsd.cc:6-9)
+__global__ void ints_so_kernel(const double* __restrict__ I, int64_t N, double* __restrict__ out) {
+    const int64_t n = 2 * N, n4 = n * n * n * n;
+    GRID_STRIDE(e, n4) {
+        const int64_t s = e % n, r = (e / n) % n, q = (e / (n * n)) % n, p = e / (n * n * n);
+        const int64_t P = p >> 1, Q = q >> 1, R = r >> 1, S = s >> 1;
+        double x = 0.0;
+        if ((p & 1) == (r & 1) && (q & 1) == (s & 1)) x += I[((P * N + R) * N + Q) * N + S];
+        if ((p & 1) == (s & 1) && (q & 1) == (r & 1)) x -= I[((P * N + S) * N + Q) * N + R];
+        out[e] = x;
+    }
+}
+
+struct Perm4 {
+    int64_t d[4];   // destination dims (contiguous, d[3] fastest)
+    int64_t s[4];   // source strides for the 4 destination indices
+};
+
+// dst[i0][i1][i2][i3] = beta dst + alpha src[i0 s0 + i1 s1 + i2 s2 + i3 s3]
+__global__ void permute4_kernel(double* __restrict__ dst, const double* __restrict__ src, Perm4 p,
+                                double alpha, double beta) {
+    const int64_t total = p.d[0] * p.d[1] * p.d[2] * p.d[3];
+    GRID_STRIDE(e, total) {
+        const int64_t i3 = e % p.d[3], i2 = (e / p.d[3]) % p.d[2];
+        const int64_t i1 = (e / (p.d[3] * p.d[2])) % p.d[1], i0 = e / (p.d[3] * p.d[2] * p.d[1]);
+        const double x = alpha * src[i0 * p.s[0] + i1 * p.s[1] + i2 * p.s[2] + i3 * p.s[3]];
+        dst[e] = (beta == 0.0) ? x : beta * dst[e] + x;
+    }
+}
+
+// Spin-orbital h from hs = C H C^T and f = h + sum_{j<nocc} <pj||qj>  (ccsd.cc:11-48)
+__global__ void form_fock_kernel(const double* __restrict__ hs, const double* __restrict__ V, int64_t N,
+                                 int64_t nocc, double* __restrict__ h, double* __restrict__ f) {
+    const int64_t n = 2 * N;
+    GRID_STRIDE(e, n * n) {
+        const int64_t p = e / n, q = e % n;
+        const double hv = ((p & 1) == (q & 1)) ? hs[(p >> 1) * N + (q >> 1)] : 0.0;
+        double t = 0.0;
+        for (int64_t j = 0; j < nocc; ++j) t += V[((p * n + j) * n + q) * n + j];
+        h[e] = hv;
+        f[e] = hv + t;
+    }
+}
+
+// T1n /= f_ii - f_aa ; T2n /= f_ii + f_jj - f_aa - f_bb
+__global__ void denom_kernel(double* __restrict__ T1, double* __restrict__ T2, const double* __restrict__ f,
+                             int64_t o, int64_t v) {
+    const int64_t n = o + v;
+    GRID_STRIDE(e, o * v + o * o * v * v) {
+        if (e < o * v) {
+            const int64_t i = e / v, a = e % v;
+            if (T1) T1[e] /= (f[i * n + i] - f[(o + a) * n + o + a]);
+        } else {
+            const int64_t x = e - o * v;
+            const int64_t b = x % v, a = (x / v) % v, j = (x / (v * v)) % o, i = x / (v * v * o);
+            T2[x] /= (f[i * n + i] + f[j * n + j] - f[(o + a) * n + o + a] - f[(o + b) * n + o + b]);
+        }
+    }
+}
+
+// tau~ = t2 + 1/2 (t_ia t_jb - t_ib t_ja), tau = t2 + t_ia t_jb - t_ib t_ja  (ccsd.cc:221-251)
+__global__ void taus_kernel(const double* __restrict__ T1, const double* __restrict__ T2, int64_t o,
+                            int64_t v, double* __restrict__ tau_s, double* __restrict__ tau) {
+    GRID_STRIDE(e, o * o * v * v) {
+        const int64_t b = e % v, a = (e / v) % v, j = (e / (v * v)) % o, i = e / (v * v * o);
+        const double x = T1[i * v + a] * T1[j * v + b] - T1[i * v + b] * T1[j * v + a];
+        tau_s[e] = T2[e] + 0.5 * x;
+        tau[e] = T2[e] + x;
+    }
+}
+
+// Fae = (1-d_ae) f_ae - 1/2 sum_m f_{m a} t_m^a (the f index of
+// ccsd_intermediates.cc:33); Fmi = (1-d_mi) f_mi
+__global__ void f_init_kernel(const double* __restrict__ f, const double* __restrict__ T1, int64_t o,
+                              int64_t v, double* __restrict__ Fae, double* __restrict__ Fmi) {
+    const int64_t n = o + v;
+    GRID_STRIDE(e, v * v + o * o) {
+        if (e < v * v) {
+            const int64_t a = e / v, ee = e % v;
+            double x = (a == ee) ? 0.0 : f[(o + a) * n + o + ee];
+            double q = 0.0;
+            for (int64_t m = 0; m < o; ++m) q += f[m * n + o + a] * T1[m * v + a];
+            Fae[e] = x - 0.5 * q;
+        } else {
+            const int64_t x = e - v * v, m = x / o, i = x % o;
+            Fmi[x] = (m == i) ? 0.0 : f[m * n + i];
+        }
+    }
+}
+
+// Zp[n][f][j][b] = 1/2 t2[j][n][f][b] + t_jf t_nb   (Wmbej, ccsd_intermediates.cc:278-283)
+__global__ void zp_kernel(const double* __restrict__ T1, const double* __restrict__ T2, int64_t o,
+                          int64_t v, double* __restrict__ Zp) {
+    GRID_STRIDE(e, o * o * v * v) {
+        const int64_t b = e % v, j = (e / v) % o, f = (e / (v * o)) % v, nn = e / (v * o * v);
+        Zp[e] = 0.5 * T2[((j * o + nn) * v + f) * v + b] + T1[j * v + f] * T1[nn * v + b];
+    }
+}
+
+__global__ void dot_kernel(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                           double* __restrict__ partial) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    GRID_STRIDE(e, n) acc += x[e] * y[e];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// El of calc_e_check (ccsd.cc:50-81): sum_i h_ii + 1/2 sum_ij [(ii|jj) - (ij|ij) d(si,sj)]
+__global__ void ehf_kernel(const double* __restrict__ I, const double* __restrict__ h, int64_t N,
+                           int64_t nocc, double* __restrict__ partial) {
+    __shared__ double sh[256];
+    const int64_t n = 2 * N;
+    double acc = 0.0;
+    GRID_STRIDE(e, nocc * nocc) {
+        const int64_t i = e / nocc, j = e % nocc, si = i >> 1, sj = j >> 1;
+        double t = I[((si * N + si) * N + sj) * N + sj];
+        if ((i & 1) == (j & 1)) t -= I[((si * N + sj) * N + si) * N + sj];
+        acc += 0.5 * t;
+        if (j == 0) acc += h[i * n + i];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void axpby_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n, double a,
+                             double b) {
+    GRID_STRIDE(e, n) y[e] = (b == 0.0 ? 0.0 : b * y[e]) + a * x[e];
+}
+
+__global__ void sub_kernel(double* __restrict__ r, const double* __restrict__ x, const double* __restrict__ y,
+                           int64_t n) {
+    GRID_STRIDE(e, n) r[e] = x[e] - y[e];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------
+struct CCSD {
+    corr_ctx* ctx;
+    int64_t o, v, n;
+    int prec;
+    // constant blocks of <pq||rs>
+    double *Voooo, *Vooov, *Voovo, *Voovv, *Vovoo, *Vovvo, *Vovvv, *Vvovv, *Vvvvv;
+    double *PeOOf, *Pmenf, *Pmine, *Pianf, *Pmbje, *Pabje;
+    double *fo;   // f (n x n)
+    // per-iteration work
+    double *tau_s, *tau, *tauS_a, *Fae, *Faep, *Fmi, *Fmip, *Fme, *Yk, *Wmnij, *Wabef, *Wmbej;
+    double *Zp, *W2, *Wp, *T2p, *T2a, *X1, *X2, *Zr, *U, *Q, *Tmp1;
+    double* partial;
+
+    int perm(double* dst, const double* src, std::initializer_list<int64_t> dims,
+             std::initializer_list<int64_t> strides, double alpha = 1.0, double beta = 0.0) {
+        Perm4 p;
+        int k = 0;
+        for (auto d : dims) p.d[k++] = d;
+        k = 0;
+        for (auto s : strides) p.s[k++] = s;
+        const int64_t total = p.d[0] * p.d[1] * p.d[2] * p.d[3];
+        hipLaunchKernelGGL(permute4_kernel, dim3(grid_for(total)), dim3(256), 0, ctx->stream, dst, src, p,
+                           alpha, beta);
+        CORR_LAUNCHED(ctx, "permute4_kernel");
+        return CORR_OK;
+    }
+    int mm(int ta, int tb, int64_t M, int64_t N_, int64_t K, double al, const double* A, int64_t lda,
+           int64_t sA, const double* B, int64_t ldb, int64_t sB, double be, double* C, int64_t ldc,
+           int64_t sC, int64_t batch = 1) {
+        return gemm(ctx, ta, tb, M, N_, K, al, A, lda, sA, B, ldb, sB, be, C, ldc, sC, batch, prec);
+    }
+    int dot(const double* x, const double* y, int64_t len, double* out) {
+        const int nb = 512;
+        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, ctx->stream, x, y, len, partial);
+        CORR_LAUNCHED(ctx, "dot_kernel");
+        return reduce_sum(ctx, partial, nb, 1, out);
+    }
+
+    int alloc(double** p, const char* name, int64_t count) {
+        return scratch(ctx, name, sizeof(double) * (size_t)count, (void**)p);
+    }
+
+    // Extracts the constant blocks from ints_so (n^4, [p][q][r][s]).
+    int setup(const double* V) {
+        const int64_t s0 = n * n * n, s1 = n * n, s2 = n;
+        const int64_t O4 = o * o * o * o, O3V = o * o * o * v, O2V2 = o * o * v * v, OV3 = o * v * v * v;
+        CORR_TRY(alloc(&Voooo, "cc_Voooo", O4));
+        CORR_TRY(alloc(&Vooov, "cc_Vooov", O3V));
+        CORR_TRY(alloc(&Voovo, "cc_Voovo", O3V));
+        CORR_TRY(alloc(&Voovv, "cc_Voovv", O2V2));
+        CORR_TRY(alloc(&Vovoo, "cc_Vovoo", O3V));
+        CORR_TRY(alloc(&Vovvo, "cc_Vovvo", O2V2));
+        CORR_TRY(alloc(&Vovvv, "cc_Vovvv", OV3));
+        CORR_TRY(alloc(&Vvovv, "cc_Vvovv", OV3));
+        CORR_TRY(alloc(&Vvvvv, "cc_Vvvvv", v * v * v * v));
+        CORR_TRY(alloc(&PeOOf, "cc_PeOOf", O2V2));
+        CORR_TRY(alloc(&Pmenf, "cc_Pmenf", O2V2));
+        CORR_TRY(alloc(&Pmine, "cc_Pmine", O3V));
+        CORR_TRY(alloc(&Pianf, "cc_Pianf", O2V2));
+        CORR_TRY(alloc(&Pmbje, "cc_Pmbje", O2V2));
+        CORR_TRY(alloc(&Pabje, "cc_Pabje", OV3));
+        const int64_t ob = o;   // offset of the virtual block
+        auto at = [&](int64_t p, int64_t q, int64_t r, int64_t s) { return V + p * s0 + q * s1 + r * s2 + s; };
+        CORR_TRY(perm(Voooo, at(0, 0, 0, 0), {o, o, o, o}, {s0, s1, s2, 1}));
+        CORR_TRY(perm(Vooov, at(0, 0, 0, ob), {o, o, o, v}, {s0, s1, s2, 1}));
+        CORR_TRY(perm(Voovo, at(0, 0, ob, 0), {o, o, v, o}, {s0, s1, s2, 1}));
+        CORR_TRY(perm(Voovv, at(0, 0, ob, ob), {o, o, v, v}, {s0, s1, s2, 1}));
+        CORR_TRY(perm(Vovoo, at(0, ob, 0, 0), {o, v, o, o}, {s0, s1, s2, 1}));
+        CORR_TRY(perm(Vovvo, at(0, ob, ob, 0), {o, v, v, o}, {s0, s1, s2, 1}));
+        CORR_TRY(perm(Vovvv, at(0, ob, ob, ob), {o, v, v, v}, {s0, s1, s2, 1}));
+        CORR_TRY(perm(Vvovv, at(ob, 0, ob, ob), {v, o, v, v}, {s0, s1, s2, 1}));
+        CORR_TRY(perm(Vvvvv, at(ob, ob, ob, ob), {v, v, v, v}, {s0, s1, s2, 1}));
+        // PeOOf[e][m][n][f] = <mn||ef>
+        CORR_TRY(perm(PeOOf, at(0, 0, ob, ob), {v, o, o, v}, {s2, s0, s1, 1}));
+        // Pmenf[m][e][n][f] = <mn||ef>
+        CORR_TRY(perm(Pmenf, at(0, 0, ob, ob), {o, v, o, v}, {s0, s2, s1, 1}));
+        // Pmine[m][i][n][e] = <mn||ie>
+        CORR_TRY(perm(Pmine, at(0, 0, 0, ob), {o, o, o, v}, {s0, s2, s1, 1}));
+        // Pianf[i][a][n][f] = <na||if>
+        CORR_TRY(perm(Pianf, at(0, ob, 0, ob), {o, v, o, v}, {s2, s1, s0, 1}));
+        // Pmbje[m][b][j][e] = <mb||ej>
+        CORR_TRY(perm(Pmbje, at(0, ob, ob, 0), {o, v, o, v}, {s0, s1, 1, s2}));
+        // Pabje[a][b][j][e] = <ab||ej>
+        CORR_TRY(perm(Pabje, at(ob, ob, ob, 0), {v, v, o, v}, {s0, s1, 1, s2}));
+
+        CORR_TRY(alloc(&tau_s, "cc_tau_s", O2V2));
+        CORR_TRY(alloc(&tau, "cc_tau", O2V2));
+        CORR_TRY(alloc(&tauS_a, "cc_tauS_a", O2V2));
+        CORR_TRY(alloc(&Fae, "cc_Fae", v * v));
+        CORR_TRY(alloc(&Faep, "cc_Faep", v * v));
+        CORR_TRY(alloc(&Fmi, "cc_Fmi", o * o));
+        CORR_TRY(alloc(&Fmip, "cc_Fmip", o * o));
+        CORR_TRY(alloc(&Fme, "cc_Fme", o * v));
+        CORR_TRY(alloc(&Yk, "cc_Yk", o * v));
+        CORR_TRY(alloc(&Wmnij, "cc_Wmnij", O4));
+        CORR_TRY(alloc(&Wabef, "cc_Wabef", v * v * v * v));
+        CORR_TRY(alloc(&Wmbej, "cc_Wmbej", O2V2));
+        CORR_TRY(alloc(&Zp, "cc_Zp", O2V2));
+        CORR_TRY(alloc(&W2, "cc_W2", O2V2));
+        CORR_TRY(alloc(&Wp, "cc_Wp", O2V2));
+        CORR_TRY(alloc(&T2p, "cc_T2p", O2V2));
+        CORR_TRY(alloc(&T2a, "cc_T2a", O2V2));
+        CORR_TRY(alloc(&X1, "cc_X1", O2V2));
+        CORR_TRY(alloc(&X2, "cc_X2", O2V2 > O3V ? O2V2 : O3V));
+        CORR_TRY(alloc(&Zr, "cc_Zr", O2V2));
+        CORR_TRY(alloc(&U, "cc_U", O3V));
+        CORR_TRY(alloc(&Q, "cc_Q", O2V2));
+        CORR_TRY(alloc(&Tmp1, "cc_Tmp1", o * v));
+        CORR_TRY(alloc(&partial, "cc_partial", 512));
+        return CORR_OK;
+    }
+
+    // ccsd_energy (ccsd.cc:147-219): E = f_kc t_kc + 1/4 t2.<kl||dc> + 1/2 t_kd Y_kd
+    int energy(const double* f, const double* T1, const double* T2, double* E) {
+        const int64_t ov = o * v;
+        // f_ov block as a contiguous copy in Tmp1
+        CORR_TRY(perm(Tmp1, f + o, {1, 1, o, v}, {0, 0, n, 1}));
+        // Y[(k d)] = sum_{(l c)} <kl||dc> t_lc  (FP64: energy terms stay FP64)
+        CORR_TRY(gemm(ctx, 0, 0, ov, 1, ov, 1.0, Pmenf, ov, 0, T1, 1, 0, 0.0, Yk, 1, 0, 1, CORR_PREC_FP64));
+        double e1, e2, e3;
+        CORR_TRY(dot(Tmp1, T1, ov, &e1));
+        CORR_TRY(dot(T2, Voovv, o * o * v * v, &e2));
+        CORR_TRY(dot(T1, Yk, ov, &e3));
+        *E = e1 + 0.25 * e2 + 0.5 * e3;
+        return CORR_OK;
+    }
+
+    // One update (T1, T2) -> (T1n, T2n)  (ccsd.cc:221-467, ccsd_intermediates.cc)
+    int update(const double* f, const double* T1, const double* T2, double* T1n, double* T2n) {
+        const int64_t ov = o * v, o2 = o * o, v2 = v * v, O2V2 = o2 * v2;
+        const unsigned g2 = grid_for(O2V2);
+        hipLaunchKernelGGL(taus_kernel, dim3(g2), dim3(256), 0, ctx->stream, T1, T2, o, v, tau_s, tau);
+        CORR_LAUNCHED(ctx, "taus_kernel");
+        // ---- Fae, Fmi, Fme
+        hipLaunchKernelGGL(f_init_kernel, dim3(grid_for(v2 + o2)), dim3(256), 0, ctx->stream, f, T1, o, v, Fae, Fmi);
+        CORR_LAUNCHED(ctx, "f_init_kernel");
+        // Fae[a][e] += sum_mf t_mf <ma||fe> = - sum_(mf) t_(mf) <am||fe>
+        CORR_TRY(mm(0, 0, 1, v, ov, -1.0, T1, ov, 0, Vvovv, v, ov * v, 1.0, Fae, v, v, v));
+        // Fae[a][e] -= 1/2 sum_(mn f) tau~[mn a f] <mn||ef>
+        CORR_TRY(perm(tauS_a, tau_s, {v, o, o, v}, {v, o * v2, v2, 1}));
+        CORR_TRY(mm(0, 1, v, v, o2 * v, -0.5, tauS_a, o2 * v, 0, PeOOf, o2 * v, 0, 1.0, Fae, v, 0));
+        // Fmi[m][i] += 1/2 sum_e f_me t_ie
+        CORR_TRY(perm(Tmp1, f + o, {1, 1, o, v}, {0, 0, n, 1}));
+        CORR_TRY(mm(0, 1, o, o, v, 0.5, Tmp1, v, 0, T1, v, 0, 1.0, Fmi, o, 0));
+        // Fmi[(m i)] += sum_(n e) <mn||ie> t_ne
+        CORR_TRY(mm(0, 0, o2, 1, ov, 1.0, Pmine, ov, 0, T1, 1, 0, 1.0, Fmi, 1, 0));
+        // Fmi[m][i] += 1/2 sum_(n e f) <mn||ef> tau~[i n e f]
+        CORR_TRY(mm(0, 1, o, o, o * v2, 0.5, Voovv, o * v2, 0, tau_s, o * v2, 0, 1.0, Fmi, o, 0));
+        // Fme[(m e)] = f_me + sum_(n f) <mn||ef> t_nf
+        CORR_TRY(mm(0, 0, ov, 1, ov, 1.0, Pmenf, ov, 0, T1, 1, 0, 0.0, Fme, 1, 0));
+        hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(ov)), dim3(256), 0, ctx->stream, Fme, Tmp1, ov, 1.0, 1.0);
+        CORR_LAUNCHED(ctx, "axpby_kernel");
+        // ---- Wmnij = <mn||ij> + P(ij) t_je <mn||ie> + 1/4 tau_ijef <mn||ef>
+        CORR_TRY(perm(Wmnij, Voooo, {o, o, o, o}, {o * o2, o2, o, 1}));
+        CORR_TRY(mm(0, 1, o * o2, o, v, 1.0, Vooov, v, 0, T1, v, 0, 1.0, Wmnij, o, 0));
+        CORR_TRY(mm(0, 1, o, o, v, -1.0, T1, v, 0, Vooov, v, o * v, 1.0, Wmnij, o, o2, o2));
+        CORR_TRY(mm(0, 1, o2, o2, v2, 0.25, Voovv, v2, 0, tau, v2, 0, 1.0, Wmnij, o2, 0));
+        // ---- Wabef = <ab||ef> - P(ab) t_mb <am||ef> + 1/4 tau_mnab <mn||ef>
+        CORR_TRY(perm(Wabef, Vvvvv, {v, v, v, v}, {v * v2, v2, v, 1}));
+        CORR_TRY(mm(1, 0, v, v2, o, -1.0, T1, v, 0, Vvovv, v2, o * v2, 1.0, Wabef, v2, v * v2, v));
+        CORR_TRY(mm(1, 0, v, v2, o, 1.0, T1, v, 0, Vvovv, v2, o * v2, 1.0, Wabef, v * v2, v2, v));
+        CORR_TRY(mm(1, 0, v2, v2, o2, 0.25, tau, v2, 0, Voovv, v2, 0, 1.0, Wabef, v2, 0));
+        // ---- Wmbej = <mb||ej> + t_jf <mb||ef> - t_nb <mn||ej> - (1/2 t_jnfb + t_jf t_nb) <mn||ef>
+        CORR_TRY(perm(Wmbej, Vovvo, {o, v, v, o}, {v2 * o, v * o, o, 1}));
+        CORR_TRY(mm(0, 1, ov * v, o, v, 1.0, Vovvv, v, 0, T1, v, 0, 1.0, Wmbej, o, 0));
+        CORR_TRY(mm(1, 0, v, v * o, o, -1.0, T1, v, 0, Voovo, v * o, o * v * o, 1.0, Wmbej, v * o, v * v * o, o));
+        hipLaunchKernelGGL(zp_kernel, dim3(g2), dim3(256), 0, ctx->stream, T1, T2, o, v, Zp);
+        CORR_LAUNCHED(ctx, "zp_kernel");
+        CORR_TRY(mm(0, 0, ov, ov, ov, 1.0, Pmenf, ov, 0, Zp, ov, 0, 0.0, W2, ov, 0));  // W2[m][e][j][b]
+        CORR_TRY(perm(Wmbej, W2, {o, v, v, o}, {v * ov, 1, ov, v}, -1.0, 1.0));
+
+        // ---- T1 update (ccsd.cc:266-333)
+        CORR_TRY(perm(T1n, f + o, {1, 1, o, v}, {0, 0, n, 1}));
+        CORR_TRY(mm(0, 1, o, v, v, 1.0, T1, v, 0, Fae, v, 0, 1.0, T1n, v, 0));
+        CORR_TRY(mm(1, 0, o, v, o, -1.0, Fmi, o, 0, T1, v, 0, 1.0, T1n, v, 0));
+        CORR_TRY(perm(T2p, T2, {o, v, o, v}, {o * v2, v, v2, 1}));      // T2p[i][a][m][e] = t_imae
+        CORR_TRY(mm(0, 0, ov, 1, ov, 1.0, T2p, ov, 0, Fme, 1, 0, 1.0, T1n, 1, 0));
+        CORR_TRY(mm(0, 0, ov, 1, ov, -1.0, Pianf, ov, 0, T1, 1, 0, 1.0, T1n, 1, 0));
+        // -1/2 t_imef <ma||ef> = +1/2 t_i(mef) <am||ef>
+        CORR_TRY(mm(0, 1, o, v, o * v2, 0.5, T2, o * v2, 0, Vvovv, o * v2, 0, 1.0, T1n, v, 0));
+        // -1/2 t_mnae <nm||ei> = +1/2 sum_(mn e) t_mnae <mn||ei>
+        CORR_TRY(perm(T2a, T2, {v, o, o, v}, {v, o * v2, v2, 1}));      // T2a[a][m][n][e]
+        CORR_TRY(mm(0, 0, v, o, o2 * v, 1.0, T2a, o2 * v, 0, Voovo, o, 0, 0.0, Q, o, 0));  // Q[a][i]
+        CORR_TRY(perm(T1n, Q, {1, 1, o, v}, {0, 0, 1, o}, 0.5, 1.0));
+
+        // ---- T2 update (ccsd.cc:336-467)
+        CORR_TRY(perm(T2n, Voovv, {o, o, v, v}, {o * v2, v2, v, 1}));
+        // Fae'[b][e] = Fae - 1/2 t_mb Fme ; P(ab) t_ijae Fae'_be
+        CORR_TRY(perm(Faep, Fae, {1, 1, v, v}, {0, 0, v, 1}));
+        CORR_TRY(mm(1, 0, v, v, o, -0.5, T1, v, 0, Fme, v, 0, 1.0, Faep, v, 0));
+        CORR_TRY(mm(0, 1, o2 * v, v, v, 1.0, T2, v, 0, Faep, v, 0, 0.0, X1, v, 0));
+        CORR_TRY(perm(T2n, X1, {o, o, v, v}, {o * v2, v2, v, 1}, 1.0, 1.0));
+        CORR_TRY(perm(T2n, X1, {o, o, v, v}, {o * v2, v2, 1, v}, -1.0, 1.0));
+        // Fmi'[m][j] = Fmi + 1/2 Fme t_je ; -P(ij) t_imab Fmi'_mj
+        CORR_TRY(perm(Fmip, Fmi, {1, 1, o, o}, {0, 0, o, 1}));
+        CORR_TRY(mm(0, 1, o, o, v, 0.5, Fme, v, 0, T1, v, 0, 1.0, Fmip, o, 0));
+        CORR_TRY(mm(1, 0, o, v2, o, 1.0, Fmip, o, 0, T2, v2, o * v2, 0.0, X2, v2, o * v2, o));
+        CORR_TRY(perm(T2n, X2, {o, o, v, v}, {o * v2, v2, v, 1}, -1.0, 1.0));
+        CORR_TRY(perm(T2n, X2, {o, o, v, v}, {v2, o * v2, v, 1}, 1.0, 1.0));
+        // hh ladder: 1/2 tau_mnab W_mnij
+        CORR_TRY(mm(1, 0, o2, v2, o2, 0.5, Wmnij, o2, 0, tau, v2, 0, 1.0, T2n, v2, 0));
+        // pp ladder: 1/2 tau_ijef W_abef
+        CORR_TRY(mm(0, 1, o2, v2, v2, 0.5, tau, v2, 0, Wabef, v2, 0, 1.0, T2n, v2, 0));
+        // ring: Zr[(ia)][(jb)] = t_imae W_mbej - t_ie t_ma <mb||ej>
+        CORR_TRY(perm(Wp, Wmbej, {o, v, o, v}, {v2 * o, o, 1, v * o}));   // Wp[m][e][j][b]
+        CORR_TRY(mm(0, 0, ov, ov, ov, 1.0, T2p, ov, 0, Wp, ov, 0, 0.0, Zr, ov, 0));
+        CORR_TRY(mm(0, 1, ov * o, o, v, 1.0, Pmbje, v, 0, T1, v, 0, 0.0, U, o, 0));   // U[m][b][j][i]
+        CORR_TRY(mm(1, 0, v, v * o2, o, 1.0, T1, v, 0, U, v * o2, 0, 0.0, Q, v * o2, 0));  // Q[a][b][j][i]
+        CORR_TRY(perm(Zr, Q, {o, v, o, v}, {1, v * o2, o, o2}, -1.0, 1.0));
+        CORR_TRY(perm(T2n, Zr, {o, o, v, v}, {v * ov, v, ov, 1}, 1.0, 1.0));   // +Zr[i a j b]
+        CORR_TRY(perm(T2n, Zr, {o, o, v, v}, {v * ov, v, 1, ov}, -1.0, 1.0));  // -Zr[i b j a]
+        CORR_TRY(perm(T2n, Zr, {o, o, v, v}, {v, v * ov, ov, 1}, -1.0, 1.0));  // -Zr[j a i b]
+        CORR_TRY(perm(T2n, Zr, {o, o, v, v}, {v, v * ov, 1, ov}, 1.0, 1.0));   // +Zr[j b i a]
+        // P(ij) t_ie <ab||ej>
+        CORR_TRY(mm(0, 1, v2 * o, o, v, 1.0, Pabje, v, 0, T1, v, 0, 0.0, X1, o, 0));   // X1[a][b][j][i]
+        CORR_TRY(perm(T2n, X1, {o, o, v, v}, {1, o, v * o2, o2}, 1.0, 1.0));
+        CORR_TRY(perm(T2n, X1, {o, o, v, v}, {o, 1, v * o2, o2}, -1.0, 1.0));
+        // -P(ab) t_ma <mb||ij>
+        CORR_TRY(mm(1, 0, v, v * o2, o, 1.0, T1, v, 0, Vovoo, v * o2, 0, 0.0, X1, v * o2, 0));  // X1[a][b][i][j]
+        CORR_TRY(perm(T2n, X1, {o, o, v, v}, {o, 1, v * o2, o2}, -1.0, 1.0));
+        CORR_TRY(perm(T2n, X1, {o, o, v, v}, {o, 1, o2, v * o2}, 1.0, 1.0));
+
+        hipLaunchKernelGGL(denom_kernel, dim3(grid_for(ov + O2V2)), dim3(256), 0, ctx->stream, T1n, T2n, f, o, v);
+        CORR_LAUNCHED(ctx, "denom_kernel");
+        return CORR_OK;
+    }
+};
+
+static int check_sizes(corr_ctx* ctx, int64_t o, int64_t v) {
+    CORR_REQUIRE(ctx, o > 0 && v > 0, "ccsd: nocc and nvir must be positive");
+    CORR_REQUIRE(ctx, v * v * v * v < ((int64_t)1 << 40), "ccsd: v^4 too large");
+    return CORR_OK;
+}
+
+int ccsd_ints_so(corr_ctx* ctx, int64_t N, const double* I, double* out) {
+    CORR_REQUIRE(ctx, N > 0 && I && out, "ccsd_ints_so: bad arguments");
+    const int64_t n = 2 * N;
+    hipLaunchKernelGGL(ints_so_kernel, dim3(grid_for(n * n * n * n)), dim3(256), 0, ctx->stream, I, N, out);
+    CORR_LAUNCHED(ctx, "ints_so_kernel");
+    return CORR_OK;
+}
+
+int ccsd_update(corr_ctx* ctx, int64_t o, int64_t v, const double* V, const double* f, const double* T1,
+                const double* T2, double* T1n, double* T2n, int prec) {
+    CORR_TRY(check_sizes(ctx, o, v));
+    CORR_REQUIRE(ctx, V && f && T1 && T2 && T1n && T2n, "ccsd_update: null pointer");
+    CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32, "ccsd: precision must be FP64 or FP32");
+    CCSD cc{ctx, o, v, o + v, prec};
+    CORR_TRY(cc.setup(V));
+    KernelTimer t(ctx, "ccsd_update");
+    return cc.update(f, T1, T2, T1n, T2n);
+}
+
+int ccsd_energy(corr_ctx* ctx, int64_t o, int64_t v, const double* V, const double* f, const double* T1,
+                const double* T2, double* E) {
+    CORR_TRY(check_sizes(ctx, o, v));
+    CORR_REQUIRE(ctx, V && f && T1 && T2 && E, "ccsd_energy: null pointer");
+    CCSD cc{ctx, o, v, o + v, CORR_PREC_FP64};
+    CORR_TRY(cc.setup(V));
+    return cc.energy(f, T1, T2, E);
+}
+
+// Solves the (m+1) x (m+1) DIIS system by Gaussian elimination with partial pivoting.
+static bool diis_solve(std::vector<double> B, int m, std::vector<double>& c) {
+    const int k = m + 1;
+    std::vector<double> rhs(k, 0.0);
+    rhs[m] = -1.0;
+    for (int col = 0; col < k; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < k; ++r)
+            if (std::fabs(B[r * k + col]) > std::fabs(B[piv * k + col])) piv = r;
+        if (std::fabs(B[piv * k + col]) < 1e-300) return false;
+        if (piv != col) {
+            for (int x = 0; x < k; ++x) std::swap(B[col * k + x], B[piv * k + x]);
+            std::swap(rhs[col], rhs[piv]);
+        }
+        for (int r = col + 1; r < k; ++r) {
+            const double fct = B[r * k + col] / B[col * k + col];
+            for (int x = col; x < k; ++x) B[r * k + x] -= fct * B[col * k + x];
+            rhs[r] -= fct * rhs[col];
+        }
+    }
+    c.assign(k, 0.0);
+    for (int r = k - 1; r >= 0; --r) {
+        double s = rhs[r];
+        for (int x = r + 1; x < k; ++x) s -= B[r * k + x] * c[x];
+        c[r] = s / B[r * k + r];
+    }
+    c.resize(m);
+    return true;
+}
+
+int ccsd_run(corr_ctx* ctx, int64_t N, int64_t nroe, const double* I, const double* C, const double* H,
+             double ion_rep, const corr_ccsd_opts* opts, double* h_energies, double* T1out, double* T2out,
+             corr_ccsd_result* res) {
+    CORR_REQUIRE(ctx, I && C && H && opts && res, "ccsd: null pointer");
+    CORR_REQUIRE(ctx, N > 0 && nroe > 0 && nroe < 2 * N && nroe % 2 == 0, "ccsd: bad nroao/nroe");
+    CORR_REQUIRE(ctx, opts->mode == CORR_CCSD_REFERENCE || opts->mode == CORR_CCSD_DIIS, "ccsd: bad mode");
+    CORR_REQUIRE(ctx, opts->maxiter >= 1, "ccsd: maxiter must be >= 1");
+    CORR_REQUIRE(ctx, opts->prec == CORR_PREC_FP64 || opts->prec == CORR_PREC_FP32,
+                 "ccsd: precision must be FP64 or FP32");
+    const int64_t n = 2 * N, o = nroe, v = n - o;
+    CORR_TRY(check_sizes(ctx, o, v));
+    const int64_t L1 = o * v, L2 = o * o * v * v, L = L1 + L2;
+    double *V, *hs, *hso, *f, *tmp;
+    CORR_TRY(scratch(ctx, "cc_ints_so", sizeof(double) * (size_t)(n * n * n * n), (void**)&V));
+    CORR_TRY(scratch(ctx, "cc_hs", sizeof(double) * N * N * 2, (void**)&hs));
+    tmp = hs + N * N;
+    CORR_TRY(scratch(ctx, "cc_hso", sizeof(double) * n * n, (void**)&hso));
+    CORR_TRY(scratch(ctx, "cc_f", sizeof(double) * n * n, (void**)&f));
+    CORR_TRY(ccsd_ints_so(ctx, N, I, V));
+    // h = C H C^T (spatial), then spin-blocked; f = h + sum_j <pj||qj>
+    CORR_TRY(gemm(ctx, 0, 0, N, N, N, 1.0, C, N, 0, H, N, 0, 0.0, tmp, N, 0, 1, CORR_PREC_FP64));
+    CORR_TRY(gemm(ctx, 0, 1, N, N, N, 1.0, tmp, N, 0, C, N, 0, 0.0, hs, N, 0, 1, CORR_PREC_FP64));
+    hipLaunchKernelGGL(form_fock_kernel, dim3(grid_for(n * n)), dim3(256), 0, ctx->stream, hs, V, N, o, hso, f);
+    CORR_LAUNCHED(ctx, "form_fock_kernel");
+
+    CCSD cc{ctx, o, v, n, opts->prec};
+    CORR_TRY(cc.setup(V));
+    // HF check
+    {
+        const int nb = 64;
+        hipLaunchKernelGGL(ehf_kernel, dim3(nb), dim3(256), 0, ctx->stream, I, hso, N, o, cc.partial);
+        CORR_LAUNCHED(ctx, "ehf_kernel");
+        double e;
+        CORR_TRY(reduce_sum(ctx, cc.partial, nb, 1, &e));
+        res->e_hf_check = e + ion_rep;
+    }
+    // amplitude vectors x = [T1 | T2]
+    const int nd = opts->mode == CORR_CCSD_DIIS ? std::max(2, opts->diis_size) : 0;
+    double *X, *Xn;
+    CORR_TRY(scratch(ctx, "cc_X", sizeof(double) * L, (void**)&X));
+    CORR_TRY(scratch(ctx, "cc_Xn", sizeof(double) * L, (void**)&Xn));
+    double *hist = nullptr, *errs = nullptr;
+    if (nd) {
+        CORR_TRY(scratch(ctx, "cc_diis_x", sizeof(double) * L * nd, (void**)&hist));
+        CORR_TRY(scratch(ctx, "cc_diis_r", sizeof(double) * L * nd, (void**)&errs));
+    }
+    // guess: T1 = 0, T2 = <ij||ab>/D  (ccsd.cc:84-144)
+    CORR_HIP(ctx, hipMemsetAsync(X, 0, sizeof(double) * L1, ctx->stream));
+    CORR_HIP(ctx, hipMemcpyAsync(X + L1, cc.Voovv, sizeof(double) * L2, hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(denom_kernel, dim3(grid_for(L)), dim3(256), 0, ctx->stream, (double*)nullptr, X + L1, f, o, v);
+    CORR_LAUNCHED(ctx, "denom_kernel");
+    {
+        double e;
+        CORR_TRY(cc.dot(X + L1, cc.Voovv, L2, &e));
+        res->e_mp2 = 0.25 * e;
+    }
+    res->iterations = 0;
+    res->converged = 0;
+    res->r_norm = 0.0;
+    double e_old = 0.0, e = 0.0;
+    std::vector<double> Bm;   // DIIS overlaps of the stored residuals (nd x nd)
+    if (nd) Bm.assign((size_t)nd * nd, 0.0);
+    int nstored = 0, head = 0;
+    for (int it = 0; it < opts->maxiter; ++it) {
+        CORR_TRY(cc.energy(f, X, X + L1, &e));
+        if (h_energies) h_energies[it] = e;
+        res->iterations = it + 1;
+        res->d_e = e_old - e;
+        res->e_ccsd = e;
+        if (opts->mode == CORR_CCSD_DIIS && it > 0 && std::fabs(e - e_old) < opts->e_tol &&
+            res->r_norm < opts->r_tol) {
+            res->converged = 1;
+            break;
+        }
+        e_old = e;
+        {
+            KernelTimer t(ctx, "ccsd_update");
+            CORR_TRY(cc.update(f, X, X + L1, Xn, Xn + L1));
+        }
+        if (!nd) {
+            std::swap(X, Xn);
+            // the swapped buffers keep their scratch registration by address only
+            continue;
+        }
+        // residual r = Xn - X, stored with Xn in slot head
+        double* xs = hist + (int64_t)head * L;
+        double* rs = errs + (int64_t)head * L;
+        hipLaunchKernelGGL(sub_kernel, dim3(grid_for(L)), dim3(256), 0, ctx->stream, rs, Xn, X, L);
+        CORR_LAUNCHED(ctx, "sub_kernel");
+        CORR_HIP(ctx, hipMemcpyAsync(xs, Xn, sizeof(double) * L, hipMemcpyDeviceToDevice, ctx->stream));
+        if (nstored < nd) ++nstored;
+        for (int k = 0; k < nstored; ++k) {
+            double d;
+            CORR_TRY(cc.dot(rs, errs + (int64_t)k * L, L, &d));
+            Bm[(size_t)head * nd + k] = Bm[(size_t)k * nd + head] = d;
+        }
+        res->r_norm = std::sqrt(Bm[(size_t)head * nd + head]);
+        head = (head + 1) % nd;
+        std::vector<double> c;
+        bool ok = false;
+        if (nstored >= 2) {
+            const int m = nstored;
+            std::vector<double> Bs((size_t)(m + 1) * (m + 1), -1.0);
+            Bs[(size_t)m * (m + 1) + m] = 0.0;
+            for (int a = 0; a < m; ++a)
+                for (int b = 0; b < m; ++b) Bs[(size_t)a * (m + 1) + b] = Bm[(size_t)a * nd + b];
+            ok = diis_solve(Bs, m, c);
+        }
+        if (ok) {
+            for (int k = 0; k < nstored; ++k) {
+                hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(L)), dim3(256), 0, ctx->stream, X,
+                                   hist + (int64_t)k * L, L, c[k], k == 0 ? 0.0 : 1.0);
+                CORR_LAUNCHED(ctx, "axpby_kernel");
+            }
+        } else {
+            std::swap(X, Xn);
+        }
+    }
+    if (T1out) CORR_HIP(ctx, hipMemcpyAsync(T1out, X, sizeof(double) * L1, hipMemcpyDeviceToDevice, ctx->stream));
+    if (T2out) CORR_HIP(ctx, hipMemcpyAsync(T2out, X + L1, sizeof(double) * L2, hipMemcpyDeviceToDevice, ctx->stream));
+    CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (opts->mode == CORR_CCSD_DIIS && !res->converged) return set_error(ctx, CORR_ENOTCONV, "ccsd: not converged");
+    return CORR_OK;
+}
+
+}  // namespace corr
+
+extern "C" {
+
+int corr_ccsd_ints_so(corr_ctx* ctx, int64_t nroao, const double* d_prec_ints, double* d_ints_so) {
+    CORR_CHECK_CTX(ctx);
+    return corr::ccsd_ints_so(ctx, nroao, d_prec_ints, d_ints_so);
+}
+
+int corr_ccsd_update(corr_ctx* ctx, int64_t nocc, int64_t nvir, const double* d_ints_so, const double* d_f,
+                     const double* d_T1, const double* d_T2, double* d_T1n, double* d_T2n, int prec) {
+    CORR_CHECK_CTX(ctx);
+    return corr::ccsd_update(ctx, nocc, nvir, d_ints_so, d_f, d_T1, d_T2, d_T1n, d_T2n, prec);
+}
+
+int corr_ccsd_energy(corr_ctx* ctx, int64_t nocc, int64_t nvir, const double* d_ints_so, const double* d_f,
+                     const double* d_T1, const double* d_T2, double* h_e) {
+    CORR_CHECK_CTX(ctx);
+    return corr::ccsd_energy(ctx, nocc, nvir, d_ints_so, d_f, d_T1, d_T2, h_e);
+}
+
+int corr_ccsd(corr_ctx* ctx, int64_t nroao, int64_t nroe, const double* d_prec_ints, const double* d_MOs,
+              const double* d_Hmat, double ion_rep, const corr_ccsd_opts* opts, double* h_energies,
+              double* d_T1, double* d_T2, corr_ccsd_result* h_out) {
+    CORR_CHECK_CTX(ctx);
+    return corr::ccsd_run(ctx, nroao, nroe, d_prec_ints, d_MOs, d_Hmat, ion_rep, opts, h_energies, d_T1, d_T2,
+                          h_out);
+}
+
+}  // extern "C"
