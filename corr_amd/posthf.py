@@ -154,3 +154,43 @@ def run_canonical_mp2_ri(Bia: torch.Tensor, FMo: torch.Tensor, nocc: int, prec: 
     (can_mp2.cc:83-91)."""
     e = torch.diagonal(FMo).contiguous()
     return rimp2_energy(Bia, e[:nocc].contiguous(), e[nocc:].contiguous(), prec)
+
+
+def rimp2_gradient(Bia: torch.Tensor, eps_occ: torch.Tensor, eps_vir: torch.Tensor, c_os: float = 1.0,
+                   c_ss: float = 1.0, prec: str | int = "fp64"):
+    """(E, dE/dBia, dE/deps_occ, dE/deps_vir) for E = c_os EMP2_OS + c_ss EMP2_SS."""
+    _dev(Bia, eps_occ, eps_vir)
+    naux, o, v = Bia.shape
+    dB = torch.empty_like(Bia)
+    deo, dev = torch.empty_like(eps_occ), torch.empty_like(eps_vir)
+    e = MP2Energy()
+    _ctx(Bia).call("corr_rimp2_gradient", naux, o, v, ptr(Bia), ptr(eps_occ), ptr(eps_vir), c_os, c_ss,
+                   _prec(prec), ptr(dB), ptr(deo), ptr(dev), C.byref(e))
+    return c_os * e.e_os + c_ss * e.e_ss, dB, deo, dev
+
+
+class RIMP2Energy(torch.autograd.Function):
+    """E = c_os EMP2_OS + c_ss EMP2_SS as a differentiable function of
+    (Bia, eps_occ, eps_vir).  Forward: fused pair-block energy kernel.
+    Backward: pair-block recomputation + gradient kernels (no o^2 v^2
+    amplitudes kept between forward and backward)."""
+
+    @staticmethod
+    def forward(ctx, Bia, eps_occ, eps_vir, prec="fp64", c_os=1.0, c_ss=1.0, grad_prec=None):
+        e = rimp2_energy(Bia, eps_occ, eps_vir, prec)
+        ctx.save_for_backward(Bia, eps_occ, eps_vir)
+        ctx.c = (c_os, c_ss, grad_prec if grad_prec is not None else prec)
+        return torch.tensor(c_os * e["EMP2_OS"] + c_ss * e["EMP2_SS"], dtype=torch.float64, device=Bia.device)
+
+    @staticmethod
+    def backward(ctx, gE):
+        Bia, eo, ev = ctx.saved_tensors
+        c_os, c_ss, prec = ctx.c
+        _, dB, deo, dev = rimp2_gradient(Bia, eo, ev, c_os, c_ss, prec)
+        g = gE.to(torch.float64)
+        return dB * g, deo * g, dev * g, None, None, None, None
+
+
+def rimp2(Bia, eps_occ, eps_vir, prec="fp64", c_os=1.0, c_ss=1.0, grad_prec=None):
+    """Differentiable RI-MP2 energy (see RIMP2Energy)."""
+    return RIMP2Energy.apply(Bia, eps_occ, eps_vir, prec, c_os, c_ss, grad_prec)
