@@ -55,7 +55,8 @@ class CCSDOpts(C.Structure):
 
 
 class CCSDResult(C.Structure):
-    _fields_ = [("e_hf_check", C.c_double), ("e_mp2", C.c_double), ("e_ccsd", C.c_double),
+    _fields_ = [("e_hf_check", C.c_double), ("e_one_check", C.c_double), ("e_fsum_check", C.c_double),
+                ("e_mp2", C.c_double), ("e_ccsd", C.c_double),
                 ("d_e", C.c_double), ("r_norm", C.c_double), ("iterations", C.c_int),
                 ("converged", C.c_int)]
 

@@ -17,6 +17,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libcorr_gpu.so"
+DROPIN_LIB = PKG / "libcorr_dropin.so"
+DROPIN_CHECK = PKG / "corr_dropin_check"
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [
@@ -63,9 +65,31 @@ def build(verbose: bool = True) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr[-8000:]}")
     os.replace(tmp, LIB)
+    build_dropin()
     if verbose:
-        print(f"built {LIB} from {len(sources)} sources")
+        print(f"built {LIB} from {len(sources)} sources, {DROPIN_LIB.name}, {DROPIN_CHECK.name}")
     return LIB
+
+
+def _run(cmd: list[str], what: str) -> None:
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{res.stderr[-8000:]}")
+
+
+def build_dropin() -> None:
+    """Host-only C++ drop-in layer (reference signatures) over libcorr_gpu.so,
+    and the driver that runs the reference's post-HF sequence through it."""
+    cxx = os.environ.get("CXX", "g++")
+    src = PKG / "dropin"
+    tmp = DROPIN_LIB.with_suffix(".so.tmp")
+    _run([cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", str(src / "corr_dropin.cpp"), "-o", str(tmp),
+          f"-L{PKG}", "-lcorr_gpu", "-Wl,-rpath,$ORIGIN"], "g++ (libcorr_dropin.so)")
+    os.replace(tmp, DROPIN_LIB)
+    tmp = DROPIN_CHECK.with_suffix(".tmp")
+    _run([cxx, "-O2", "-std=c++17", "-Wall", str(src / "dropin_check.cpp"), "-o", str(tmp), f"-L{PKG}",
+          "-lcorr_dropin", "-lcorr_gpu", "-Wl,-rpath,$ORIGIN"], "g++ (corr_dropin_check)")
+    os.replace(tmp, DROPIN_CHECK)
 
 
 if __name__ == "__main__":

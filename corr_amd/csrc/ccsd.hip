@@ -136,26 +136,31 @@ __global__ void dot_kernel(const double* __restrict__ x, const double* __restric
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-// El of calc_e_check (ccsd.cc:50-81): sum_i h_ii + 1/2 sum_ij [(ii|jj) - (ij|ij) d(si,sj)]
-__global__ void ehf_kernel(const double* __restrict__ I, const double* __restrict__ h, int64_t N,
-                           int64_t nocc, double* __restrict__ partial) {
-    __shared__ double sh[256];
+// calc_e_check (ccsd.cc:50-81): partial sums of [sum_i h_ii, 1/2 twoe, sum_i f_ii]
+// with twoe = sum_ij [(ii|jj) - (ij|ij) d(si,sj)] over occupied spin orbitals.
+__global__ void ehf_kernel(const double* __restrict__ I, const double* __restrict__ h,
+                           const double* __restrict__ f, int64_t N, int64_t nocc, double* __restrict__ partial) {
+    __shared__ double sh[3][256];
     const int64_t n = 2 * N;
-    double acc = 0.0;
+    double acc[3] = {0.0, 0.0, 0.0};
     GRID_STRIDE(e, nocc * nocc) {
         const int64_t i = e / nocc, j = e % nocc, si = i >> 1, sj = j >> 1;
         double t = I[((si * N + si) * N + sj) * N + sj];
         if ((i & 1) == (j & 1)) t -= I[((si * N + sj) * N + si) * N + sj];
-        acc += 0.5 * t;
-        if (j == 0) acc += h[i * n + i];
+        acc[1] += 0.5 * t;
+        if (j == 0) {
+            acc[0] += h[i * n + i];
+            acc[2] += f[i * n + i];
+        }
     }
-    sh[threadIdx.x] = acc;
+    for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+    if (threadIdx.x < 3) partial[blockIdx.x * 3 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 __global__ void axpby_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n, double a,
@@ -493,11 +498,13 @@ int ccsd_run(corr_ctx* ctx, int64_t N, int64_t nroe, const double* I, const doub
     // HF check
     {
         const int nb = 64;
-        hipLaunchKernelGGL(ehf_kernel, dim3(nb), dim3(256), 0, ctx->stream, I, hso, N, o, cc.partial);
+        hipLaunchKernelGGL(ehf_kernel, dim3(nb), dim3(256), 0, ctx->stream, I, hso, f, N, o, cc.partial);
         CORR_LAUNCHED(ctx, "ehf_kernel");
-        double e;
-        CORR_TRY(reduce_sum(ctx, cc.partial, nb, 1, &e));
-        res->e_hf_check = e + ion_rep;
+        double e[3];
+        CORR_TRY(reduce_sum(ctx, cc.partial, nb, 3, e));
+        res->e_hf_check = e[0] + e[1] + ion_rep;
+        res->e_one_check = e[0] + ion_rep;
+        res->e_fsum_check = e[2] - e[1] + ion_rep;
     }
     // amplitude vectors x = [T1 | T2]
     const int nd = opts->mode == CORR_CCSD_DIIS ? std::max(2, opts->diis_size) : 0;

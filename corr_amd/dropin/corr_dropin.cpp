@@ -1,0 +1,257 @@
+// Drop-in replacements for the reference's post-HF entry points (the C++
+// signatures of src/POSTHF/{motrans,can_mp2,ri,ccsd}.h).  Host code only: all
+// arithmetic is done by libcorr_gpu.so through the C-ABI.  Ownership follows
+// the reference (callee new[] for FMo, prec_ints, ints_so; caller-allocated
+// BPQ and Bia), and results are printed with the reference's labels and
+// widths.  Errors end the process with a message, as the reference's own
+// failure paths do (exit() in ops_io.cc / ops_rhf.cc).
+//
+// Environment overrides (defaults reproduce the reference):
+//   CORR_PREC       fp64 | fp32 | bf16x3   arithmetic of the bulk contractions
+//   CORR_CCSD_MODE  reference | diis        20 fixed iterations, or DIIS to 1e-8
+#include "../../include/corr_dropin.h"
+#include "../../include/corr_abi.h"
+
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <vector>
+
+#define PWIDTH_L 12
+#define PWIDTH_R 16
+
+namespace {
+
+corr_ctx* g_ctx = nullptr;
+
+[[noreturn]] void die(const char* what, int st) {
+    std::cerr << "corr_dropin: " << what << " failed with status " << st << ": "
+              << (g_ctx ? corr_ctx_last_error(g_ctx) : "") << std::endl;
+    std::exit(3);
+}
+
+corr_ctx* ctx() {
+    if (!g_ctx) {
+        const int st = corr_ctx_create(0, &g_ctx);
+        if (st != CORR_OK) die("corr_ctx_create", st);
+    }
+    return g_ctx;
+}
+
+void check(const char* what, int st) {
+    if (st != CORR_OK) die(what, st);
+}
+
+int prec_from_env() {
+    const char* p = std::getenv("CORR_PREC");
+    if (!p || !std::strcmp(p, "fp64")) return CORR_PREC_FP64;
+    if (!std::strcmp(p, "fp32")) return CORR_PREC_FP32;
+    if (!std::strcmp(p, "bf16x3")) return CORR_PREC_BF16X3;
+    std::cerr << "corr_dropin: unknown CORR_PREC '" << p << "'" << std::endl;
+    std::exit(1);
+}
+
+// Device buffer owned by the wrapper call.
+struct DBuf {
+    void* p = nullptr;
+    explicit DBuf(int64_t bytes) { check("corr_dmalloc", corr_dmalloc(ctx(), &p, bytes)); }
+    DBuf(const void* host, int64_t bytes) : DBuf(bytes) { check("corr_h2d", corr_h2d(ctx(), p, host, bytes)); }
+    ~DBuf() { corr_dfree(ctx(), p); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    double* d() const { return static_cast<double*>(p); }
+    void to_host(void* host, int64_t bytes) const { check("corr_d2h", corr_d2h(ctx(), host, p, bytes)); }
+};
+
+// Device copies of prec_ints made by MOtrans, keyed by the host array they
+// were downloaded to (the reference never modifies prec_ints afterwards), so
+// that run_canonical_mp2 and ccsd_ur do not upload N^4 values again.
+std::map<const double*, DBuf*>& prec_cache() {
+    static std::map<const double*, DBuf*> m;
+    return m;
+}
+
+const double* device_prec_ints(const double* host, int64_t N, DBuf** owned) {
+    auto it = prec_cache().find(host);
+    if (it != prec_cache().end()) {
+        *owned = nullptr;
+        return it->second->d();
+    }
+    *owned = new DBuf(host, sizeof(double) * N * N * N * N);
+    return (*owned)->d();
+}
+
+void print_mp2(const char* title, const char* rule, const char* scs_title, const char* scs_rule,
+               const corr_mp2_energy& e) {
+    std::cout << title;
+    std::cout << rule;
+    std::cout << std::setw(PWIDTH_L) << "EMP2_SS:" << std::setw(PWIDTH_R) << e.e_ss << '\n';
+    std::cout << std::setw(PWIDTH_L) << "EMP2_OS:" << std::setw(PWIDTH_R) << e.e_os << '\n';
+    std::cout << std::setw(PWIDTH_L) << "EMP2:" << std::setw(PWIDTH_R) << e.e_mp2 << '\n';
+    std::cout << scs_title;
+    std::cout << scs_rule;
+    std::cout << std::setw(PWIDTH_L) << "EMP2_SS:" << std::setw(PWIDTH_R) << e.e_ss / 3.0 << '\n';
+    std::cout << std::setw(PWIDTH_L) << "EMP2_OS:" << std::setw(PWIDTH_R) << e.e_os * 6.0 / 5.0 << '\n';
+    std::cout << std::setw(PWIDTH_L) << "EMP2(SCS):" << std::setw(PWIDTH_R) << e.e_ss / 3. + e.e_os * 6.0 / 5.0
+              << '\n';
+}
+
+void transform_ri_device(int naux, int N, int nroe, const double* MOs, const double* d_BPQ, double* Bia) {
+    const int o = nroe / 2, v = N - o;
+    DBuf dC(MOs, sizeof(double) * N * N);
+    DBuf dB(sizeof(double) * (int64_t)naux * o * v);
+    check("corr_transform_ri", corr_transform_ri(ctx(), naux, N, o, dC.d(), d_BPQ, dB.d(), CORR_PREC_FP64));
+    dB.to_host(Bia, sizeof(double) * (int64_t)naux * o * v);
+}
+
+}  // namespace
+
+// motrans.cc:8-32
+void build_FMo(systeminfo* sysinfo, OEints* onemats, pHF* postHF) {
+    const int N = sysinfo->nroao;
+    postHF->FMo = new double[(size_t)N * N];
+    DBuf dC(onemats->MOs, sizeof(double) * N * N), dF(onemats->Fmat, sizeof(double) * N * N);
+    DBuf dFMo(sizeof(double) * N * N);
+    check("corr_build_fmo", corr_build_fmo(ctx(), N, dC.d(), dF.d(), dFMo.d()));
+    dFMo.to_host(postHF->FMo, sizeof(double) * N * N);
+}
+
+// motrans.cc:35-211 (the full transform; the reference's debug transform and
+// progress prints are not reproduced)
+void MOtrans(systeminfo* sysinfo, OEints* onemats, TEints* twomats, pHF* postHF) {
+    const int64_t N = sysinfo->nroao, n4 = N * N * N * N, nint = sysinfo->nrofint;
+    std::cout << "Need " << n4 * (int64_t)sizeof(double) << " bytes ("
+              << n4 * (int64_t)sizeof(double) / 1024 / 1024 << " MB) for precalculation\n";
+    postHF->prec_ints = new double[(size_t)n4];
+    DBuf dC(onemats->MOs, sizeof(double) * N * N);
+    DBuf dval(twomats->intval, sizeof(double) * (nint > 0 ? nint : 1));
+    DBuf dnum(twomats->intnums, sizeof(unsigned short) * 4 * (nint > 0 ? nint : 1));
+    DBuf dense(sizeof(double) * n4), work(sizeof(double) * n4);
+    DBuf* out = new DBuf(sizeof(double) * n4);
+    const int64_t sc[4] = {twomats->sortcount[0], twomats->sortcount[1], twomats->sortcount[2],
+                           twomats->sortcount[3]};
+    check("corr_ao_unfold", corr_ao_unfold(ctx(), N, nint, sc, dval.d(), (const uint16_t*)dnum.p, dense.d()));
+    check("corr_motrans", corr_motrans(ctx(), N, dC.d(), dense.d(), work.d(), out->d(), prec_from_env() == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_PREC_FP32));
+    out->to_host(postHF->prec_ints, sizeof(double) * n4);
+    prec_cache()[postHF->prec_ints] = out;
+}
+
+// can_mp2.cc:11-49
+void run_canonical_mp2(systeminfo* sysinfo, pHF* postHF) {
+    const int N = sysinfo->nroao, o = sysinfo->nroe / 2;
+    DBuf* owned = nullptr;
+    const double* dI = device_prec_ints(postHF->prec_ints, N, &owned);
+    DBuf dF(postHF->FMo, sizeof(double) * N * N);
+    corr_mp2_energy e;
+    check("corr_canonical_mp2", corr_canonical_mp2(ctx(), N, o, dI, dF.d(), &e));
+    delete owned;
+    print_mp2("\nMP2-Results:\n", "------------\n", "\nSCS-MP2-Results:\n", "------------\n", e);
+}
+
+// can_mp2.cc:53-132
+void run_canonical_mp2_ri(systeminfo* sysinfo, pHF* postHF) {
+    const int N = sysinfo->nroao, o = sysinfo->nroe / 2, v = N - o, naux = sysinfo->naux_2;
+    std::vector<double> eo(o), ev(v);
+    for (int i = 0; i < o; ++i) eo[i] = postHF->FMo[(size_t)i * N + i];
+    for (int a = 0; a < v; ++a) ev[a] = postHF->FMo[(size_t)(a + o) * N + (a + o)];
+    DBuf dB(postHF->Bia, sizeof(double) * (int64_t)naux * o * v);
+    DBuf deo(eo.data(), sizeof(double) * o), dev(ev.data(), sizeof(double) * v);
+    corr_mp2_energy e;
+    check("corr_rimp2_energy", corr_rimp2_energy(ctx(), naux, o, v, dB.d(), deo.d(), dev.d(), prec_from_env(), &e));
+    print_mp2("\nRI-MP2-Results:\n", "---------------\n", "\nSCS-RI-MP2-Results:\n", "-------------------\n", e);
+}
+
+// ri.cc:50-69, 102-103 without the libint2 integral evaluation
+void calculate_ri_arith(int naux, int nroao, const double* J, const double* Pmn, double* BPQ) {
+    const int64_t N2 = (int64_t)nroao * nroao;
+    DBuf dJ(J, sizeof(double) * naux * naux), dM(sizeof(double) * naux * naux);
+    DBuf dP(Pmn, sizeof(double) * naux * N2), dBPQ(sizeof(double) * naux * N2);
+    int64_t kept = 0;
+    check("corr_ri_metric", corr_ri_metric(ctx(), naux, dJ.d(), CORR_METRIC_EIGEN, dM.d(), &kept));
+    check("corr_ri_bpq", corr_ri_bpq(ctx(), naux, nroao, dM.d(), dP.d(), dBPQ.d(), CORR_PREC_FP64));
+    dBPQ.to_host(BPQ, sizeof(double) * naux * N2);
+}
+
+// ri.cc:147-172
+void transform_ri(systeminfo* sysinfo, OEints* onemats, pHF* postHF) {
+    const int N = sysinfo->nroao, naux = sysinfo->naux_2;
+    DBuf dBPQ(postHF->BPQ, sizeof(double) * (int64_t)naux * N * N);
+    transform_ri_device(naux, N, sysinfo->nroe, onemats->MOs, dBPQ.d(), postHF->Bia);
+}
+
+// ri.cc:109-145: BPQ read from PREFIX.rimp2
+void read_transform_ri(std::string prefix, int nroe, int nroao, int naux_2, double* MOs, double* Bia) {
+    const int64_t count = (int64_t)naux_2 * nroao * nroao;
+    std::vector<double> BPQ(count);
+    std::ifstream datf(prefix + ".rimp2", std::ios::binary);
+    datf.read(reinterpret_cast<char*>(BPQ.data()), count * sizeof(double));
+    if (!datf) {
+        std::cerr << "corr_dropin: cannot read " << count << " doubles from " << prefix << ".rimp2" << std::endl;
+        std::exit(3);
+    }
+    DBuf dBPQ(BPQ.data(), sizeof(double) * count);
+    transform_ri_device(naux_2, nroao, nroe, MOs, dBPQ.d(), Bia);
+}
+
+// ccsd.cc:6-9 (index accessor over the spatial (ij|kl) array)
+double get_integral(double* ints, long long int& istep, long long int& jstep, long long int& kstep, int& i, int& j,
+                    int& k, int& l) {
+    return ints[(i / 2) * istep + (k / 2) * jstep + (j / 2) * kstep + l / 2] * (i % 2 == k % 2) * (j % 2 == l % 2) -
+           ints[(i / 2) * istep + (l / 2) * jstep + (j / 2) * kstep + k / 2] * (i % 2 == l % 2) * (j % 2 == k % 2);
+}
+
+// ccsd.cc:538-580
+void ccsd_ur(systeminfo* sysinfo, OEints* onemats, pHF* postHF) {
+    const int N = sysinfo->nroao, nocc = sysinfo->nroe, norb = 2 * N, nvir = norb - nocc;
+    const char* mode = std::getenv("CORR_CCSD_MODE");
+    const bool diis = mode && !std::strcmp(mode, "diis");
+    std::cout << "\nCCSD:\n----" << '\n';
+    const long long mem_a = 2LL * nocc * nvir + 2LL * nocc * nocc * nvir * nvir;
+    const long long mem_int = 1LL * nvir * nvir + 1LL * nocc * nocc + 1LL * nocc * nvir + 1LL * nocc * nocc * nocc * nocc +
+                              1LL * nvir * nvir * nvir * nvir + 1LL * nocc * nvir * nvir * nocc +
+                              2LL * nocc * nocc * nvir * nvir;
+    std::cout << "\n Allocating " << mem_a / 1024 / 1024 << "MB(" << mem_a / 1024 / 1024 / 1024 << "Gb) for amplitues\n";
+    std::cout << "Allocating " << mem_int / 1024 / 1024 << "MB(" << mem_int / 1024 / 1024 / 1024 << "Gb) for intermediates\n";
+    std::cout.flush();
+
+    DBuf* owned = nullptr;
+    const double* dI = device_prec_ints(postHF->prec_ints, N, &owned);
+    DBuf dC(onemats->MOs, sizeof(double) * N * N), dH(onemats->Hmat, sizeof(double) * N * N);
+    const int64_t L1 = (int64_t)nocc * nvir, L2 = L1 * L1;
+    DBuf dT1(sizeof(double) * L1), dT2(sizeof(double) * L2);
+    corr_ccsd_opts opts;
+    opts.mode = diis ? CORR_CCSD_DIIS : CORR_CCSD_REFERENCE;
+    opts.maxiter = diis ? 200 : 20;
+    opts.diis_size = 8;
+    const int p = prec_from_env();
+    opts.prec = p == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_PREC_FP32;
+    opts.e_tol = 1e-8;
+    opts.r_tol = 1e-6;
+    std::vector<double> energies(opts.maxiter);
+    corr_ccsd_result r;
+    const int st = corr_ccsd(ctx(), N, nocc, dI, dC.d(), dH.d(), sysinfo->ion_rep, &opts, energies.data(), dT1.d(),
+                             dT2.d(), &r);
+    if (st != CORR_OK && st != CORR_ENOTCONV) die("corr_ccsd", st);
+    // <pq||rs> as the reference leaves it in postHF->ints_so (ccsd.cc:514)
+    {
+        const int64_t n4 = (int64_t)norb * norb * norb * norb;
+        postHF->ints_so = new double[(size_t)n4];
+        DBuf dV(sizeof(double) * n4);
+        check("corr_ccsd_ints_so", corr_ccsd_ints_so(ctx(), N, dI, dV.d()));
+        dV.to_host(postHF->ints_so, sizeof(double) * n4);
+    }
+    delete owned;
+    std::cout << "El:" << r.e_hf_check << std::endl;
+    std::cout << "one:" << r.e_one_check << std::endl;
+    std::cout << "fsum-0.5twoe:" << r.e_fsum_check;
+    std::cout << "\n\nEMP2: " << r.e_mp2;
+    double e_old = 0.0;
+    for (int it = 0; it < r.iterations; ++it) {
+        std::cout << it << "E(CCSD)= " << energies[it] << " dE:" << e_old - energies[it] << "  " << "\n";
+        e_old = energies[it];
+    }
+}

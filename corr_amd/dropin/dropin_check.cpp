@@ -1,0 +1,74 @@
+// Runs the reference's post-HF sequence (src/main.cc:130-175) through the
+// drop-in entry points on inputs read from a binary file:
+//   int64 N, nroe, naux, nrofint, sortcount[4]; double ion_rep;
+//   double MOs[N*N], Fmat[N*N], Hmat[N*N], intval[nrofint];
+//   uint16 intnums[4*nrofint]; double J[naux*naux], Pmn[naux*N*N]
+// and prints the reference's result lines.
+#include "../../include/corr_dropin.h"
+
+#include <cstdint>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <vector>
+
+template <typename T>
+static void rd(std::ifstream& f, T* p, int64_t n) {
+    f.read(reinterpret_cast<char*>(p), sizeof(T) * n);
+    if (!f) {
+        std::cerr << "dropin_check: short input file" << std::endl;
+        std::exit(2);
+    }
+}
+
+int main(int argc, char** argv) {
+    if (argc != 2) {
+        std::cerr << "usage: corr_dropin_check INPUT" << std::endl;
+        return 1;
+    }
+    std::ifstream f(argv[1], std::ios::binary);
+    int64_t hdr[8];
+    rd(f, hdr, 8);
+    double ion_rep;
+    rd(f, &ion_rep, 1);
+    const int64_t N = hdr[0], nroe = hdr[1], naux = hdr[2], nint = hdr[3];
+    std::vector<double> MOs(N * N), Fmat(N * N), Hmat(N * N), intval(nint);
+    std::vector<unsigned short> intnums(4 * nint);
+    std::vector<double> J(naux * naux), Pmn(naux * N * N);
+    rd(f, MOs.data(), N * N);
+    rd(f, Fmat.data(), N * N);
+    rd(f, Hmat.data(), N * N);
+    rd(f, intval.data(), nint);
+    rd(f, intnums.data(), 4 * nint);
+    rd(f, J.data(), naux * naux);
+    rd(f, Pmn.data(), naux * N * N);
+
+    systeminfo* sys = new systeminfo;
+    sys->nroe = (int)nroe;
+    sys->nroao = (int)N;
+    sys->naux_2 = (int)naux;
+    sys->nrofint = nint;
+    sys->ion_rep = ion_rep;
+    OEints* one = new OEints;
+    one->MOs = MOs.data();
+    one->Fmat = Fmat.data();
+    one->Hmat = Hmat.data();
+    TEints* two = new TEints;
+    two->intval = intval.data();
+    two->intnums = intnums.data();
+    for (int k = 0; k < 4; ++k) two->sortcount[k] = hdr[4 + k];
+    pHF* post = new pHF;
+    std::cout << std::fixed << std::setprecision(10);   // the reference's stream state after SCF
+
+    build_FMo(sys, one, post);
+    MOtrans(sys, one, two, post);
+    run_canonical_mp2(sys, post);
+    post->BPQ = new double[naux * N * N];
+    calculate_ri_arith((int)naux, (int)N, J.data(), Pmn.data(), post->BPQ);
+    post->Bia = new double[naux * (nroe / 2) * (N - nroe / 2)];
+    transform_ri(sys, one, post);
+    run_canonical_mp2_ri(sys, post);
+    ccsd_ur(sys, one, post);
+    std::cout << "\n--END--" << std::endl;
+    return 0;
+}

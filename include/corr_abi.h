@@ -190,6 +190,8 @@ typedef struct corr_ccsd_opts {
 
 typedef struct corr_ccsd_result {
     double e_hf_check;    /* El of calc_e_check (ccsd.cc:50-81), + ion_rep  */
+    double e_one_check;   /* "one:" of calc_e_check: sum_i h_ii + ion_rep  */
+    double e_fsum_check;  /* "fsum-0.5twoe:": sum_i f_ii - twoe/2 + ion_rep */
     double e_mp2;         /* spin-orbital MP2 of the guess (ccsd.cc:131-143) */
     double e_ccsd;        /* last energy evaluated                           */
     double d_e;           /* last energy change (E_old - E)                  */
