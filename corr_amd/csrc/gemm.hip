@@ -40,55 +40,72 @@ template <> struct Mfma<float> {
     }
 };
 
-// Loads one BK x BDIM slab of an operand into registers.
-//  CONTIG_ALONG_K == false: element (k, x) at P[k*ld + x]  (x contiguous)
-//  CONTIG_ALONG_K == true : element (k, x) at P[x*ld + k]  (k contiguous)
-template <int BDIM, int BK, bool CONTIG_ALONG_K, int NT, typename T>
-__device__ inline void load_slab(const double* __restrict__ P, int64_t ld, int64_t x0,
-                                 int64_t xmax, int64_t k0, int64_t kmax, T* regs) {
-    constexpr int ELEMS = BDIM * BK / NT;
-    const int tid = threadIdx.x;
-#pragma unroll
-    for (int u = 0; u < ELEMS; ++u) {
-        const int e = u * NT + tid;
-        int kk, xx;
-        if (CONTIG_ALONG_K) { xx = e / BK; kk = e % BK; }
-        else                { kk = e / BDIM; xx = e % BDIM; }
-        const int64_t k = k0 + kk, x = x0 + xx;
-        double v = 0.0;
-        if (k < kmax && x < xmax)
-            v = CONTIG_ALONG_K ? P[x * ld + k] : P[k * ld + x];
-        regs[u] = (T)v;
-    }
-}
+// One operand slab (BK deep, BDIM wide) of a GEMM tile.
+//  CONTIG_ALONG_K == false: element (k, x) at P[k*ld + x] (x contiguous); LDS
+//    tile [k][x] with rows padded by 16 elements, so a fragment read (16 x
+//    values per lane group) and a store (consecutive x per lane) both touch
+//    distinct banks.
+//  CONTIG_ALONG_K == true : element (k, x) at P[x*ld + k] (k contiguous); LDS
+//    tile [x][k] with rows of BK + KPAD elements (36 floats / 18 doubles:
+//    a row stride of 36 dwords), so the 16 rows of a fragment read and the
+//    consecutive-k stores of a lane group fall in distinct banks.
+template <typename T> struct KPad;
+template <> struct KPad<float> { static constexpr int v = 4; };
+template <> struct KPad<double> { static constexpr int v = 2; };
 
-template <int BDIM, int BK, bool CONTIG_ALONG_K, int LDW, int NT, typename T>
-__device__ inline void store_slab(T (*S)[LDW], const T* regs) {
-    constexpr int ELEMS = BDIM * BK / NT;
-    const int tid = threadIdx.x;
-#pragma unroll
-    for (int u = 0; u < ELEMS; ++u) {
-        const int e = u * NT + tid;
-        int kk, xx;
+template <int BDIM, int BK, bool CONTIG_ALONG_K, int NT, typename T>
+struct Slab {
+    static constexpr int ROWS = CONTIG_ALONG_K ? BDIM : BK;
+    static constexpr int LDW = CONTIG_ALONG_K ? BK + KPad<T>::v : BDIM + 16;
+    static constexpr int ELEMS = (BDIM * BK + NT - 1) / NT;
+
+    __device__ static inline void coords(int e, int& kk, int& xx) {
         if (CONTIG_ALONG_K) { xx = e / BK; kk = e % BK; }
         else                { kk = e / BDIM; xx = e % BDIM; }
-        S[kk][xx] = regs[u];
     }
-}
+    __device__ static inline void load(const double* __restrict__ P, int64_t ld, int64_t x0, int64_t xmax,
+                                       int64_t k0, int64_t kmax, T* regs) {
+#pragma unroll
+        for (int u = 0; u < ELEMS; ++u) {
+            const int e = u * NT + (int)threadIdx.x;
+            int kk, xx;
+            coords(e, kk, xx);
+            const int64_t k = k0 + kk, x = x0 + xx;
+            double v = 0.0;
+            if (e < BDIM * BK && k < kmax && x < xmax) v = CONTIG_ALONG_K ? P[x * ld + k] : P[k * ld + x];
+            regs[u] = (T)v;
+        }
+    }
+    __device__ static inline void store(T (*S)[LDW], const T* regs) {
+#pragma unroll
+        for (int u = 0; u < ELEMS; ++u) {
+            const int e = u * NT + (int)threadIdx.x;
+            if (BDIM * BK % NT != 0 && e >= BDIM * BK) break;
+            int kk, xx;
+            coords(e, kk, xx);
+            if (CONTIG_ALONG_K) S[xx][kk] = regs[u];
+            else S[kk][xx] = regs[u];
+        }
+    }
+    __device__ static inline T frag(const T (*S)[LDW], int k, int x) {
+        return CONTIG_ALONG_K ? S[x][k] : S[k][x];
+    }
+};
 
 template <typename T> struct GemmSlab;
 template <> struct GemmSlab<double> { static constexpr int BK = 16; };
 template <> struct GemmSlab<float> { static constexpr int BK = 32; };
 
-// 8 waves (two per SIMD) in a WM x WN grid; each wave owns 64 x 32 of C
+// WM x WN waves (8 or 10, two or more per SIMD); each wave owns 64 x 32 of C
 // (4 x 2 MFMA tiles).  BM = 64 WM, BN = 32 WN.
 template <int TA, int TB, typename T, int WM, int WN>
-__global__ __launch_bounds__(512) void gemm_kernel(GemmArgs g) {
-    constexpr int NT = 512;
-    constexpr int BM = 64 * WM, BN = 32 * WN, BK = GemmSlab<T>::BK, PAD = 16;
-    constexpr int EA = BM * BK / NT, EB = BN * BK / NT;
-    __shared__ T As[BK][BM + PAD];
-    __shared__ T Bs[BK][BN + PAD];
+__global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
+    constexpr int NT = 64 * WM * WN;
+    constexpr int BM = 64 * WM, BN = 32 * WN, BK = GemmSlab<T>::BK;
+    typedef Slab<BM, BK, TA == 0, NT, T> SA;   // op(A)(m,k): TA==0 -> A[m*lda+k]; TA==1 -> A[k*lda+m]
+    typedef Slab<BN, BK, TB == 1, NT, T> SB;   // op(B)(k,n): TB==0 -> B[k*ldb+n]; TB==1 -> B[n*ldb+k]
+    __shared__ T As[SA::ROWS][SA::LDW];
+    __shared__ T Bs[SB::ROWS][SB::LDW];
     typedef typename Mfma<T>::acc_t acc_t;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -110,30 +127,28 @@ __global__ __launch_bounds__(512) void gemm_kernel(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    T ra[EA], rb[EB];
-    // op(A)(m,k): TA==0 -> A[m*lda+k] (k contiguous); TA==1 -> A[k*lda+m]
-    // op(B)(k,n): TB==0 -> B[k*ldb+n] (n contiguous); TB==1 -> B[n*ldb+k]
+    T ra[SA::ELEMS], rb[SB::ELEMS];
     if (kTiles > 0) {
-        load_slab<BM, BK, TA == 0, NT>(A, g.lda, m0, g.M, 0, kEnd, ra);
-        load_slab<BN, BK, TB == 1, NT>(B, g.ldb, n0, g.N, 0, kEnd, rb);
+        SA::load(A, g.lda, m0, g.M, 0, kEnd, ra);
+        SB::load(B, g.ldb, n0, g.N, 0, kEnd, rb);
     }
     for (int64_t kt = 0; kt < kTiles; ++kt) {
         __syncthreads();
-        store_slab<BM, BK, TA == 0, BM + PAD, NT>(As, ra);
-        store_slab<BN, BK, TB == 1, BN + PAD, NT>(Bs, rb);
+        SA::store(As, ra);
+        SB::store(Bs, rb);
         __syncthreads();
         if (kt + 1 < kTiles) {
-            load_slab<BM, BK, TA == 0, NT>(A, g.lda, m0, g.M, (kt + 1) * BK, kEnd, ra);
-            load_slab<BN, BK, TB == 1, NT>(B, g.ldb, n0, g.N, (kt + 1) * BK, kEnd, rb);
+            SA::load(A, g.lda, m0, g.M, (kt + 1) * BK, kEnd, ra);
+            SB::load(B, g.ldb, n0, g.N, (kt + 1) * BK, kEnd, rb);
         }
 #pragma unroll 2
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + (lane >> 4);
             T fa[4], fb[2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) fa[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+            for (int i = 0; i < 4; ++i) fa[i] = SA::frag(As, kr, wm * 64 + i * 16 + (lane & 15));
 #pragma unroll
-            for (int j = 0; j < 2; ++j) fb[j] = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
+            for (int j = 0; j < 2; ++j) fb[j] = SB::frag(Bs, kr, wn * 32 + j * 16 + (lane & 15));
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -160,10 +175,14 @@ __global__ __launch_bounds__(512) void gemm_kernel(GemmArgs g) {
 
 template <int TA, int TB, typename T>
 static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
-    // N <= 64, or a multiple of 64 that 128 does not divide: 256 x 64 tiles;
+    // Tile shapes: 128 x 160 when N is in (128, 160] (o = 160 of (H2O)32 in one
+    // tile), 256 x 64 for N <= 64 or a multiple of 64 that 128 does not divide,
     // otherwise 128 x 128.
-    const bool tall = (a.N <= 64) || (a.N % 128 != 0 && a.N % 64 == 0);
-    const int BM = tall ? 256 : 128, BN = tall ? 64 : 128;
+    int cfg = 0;
+    if (a.N > 128 && a.N <= 160) cfg = 2;
+    else if ((a.N <= 64) || (a.N % 128 != 0 && a.N % 64 == 0)) cfg = 1;
+    const int BM = cfg == 1 ? 256 : 128, BN = cfg == 0 ? 128 : (cfg == 1 ? 64 : 160);
+    const int nt = cfg == 2 ? 640 : 512;
     a.tilesM = cdiv(a.M, BM);
     const int64_t tiles = a.tilesM * cdiv(a.N, BN);
     CORR_REQUIRE(ctx, tiles < (int64_t)1 << 31, "gemm: too many tiles");
@@ -173,10 +192,12 @@ static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
         s.A += b0 * a.sA; s.B += b0 * a.sB; s.C += b0 * a.sC;
         dim3 grid((unsigned)tiles, (unsigned)nb);
         KernelTimer timer(ctx, "gemm");
-        if (tall)
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2>), grid, dim3(512), 0, ctx->stream, s);
+        if (cfg == 1)
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2>), grid, dim3(nt), 0, ctx->stream, s);
+        else if (cfg == 2)
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 5>), grid, dim3(nt), 0, ctx->stream, s);
         else
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 4>), grid, dim3(512), 0, ctx->stream, s);
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 4>), grid, dim3(nt), 0, ctx->stream, s);
         CORR_LAUNCHED(ctx, "gemm_kernel");
     }
     return CORR_OK;

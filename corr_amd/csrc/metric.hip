@@ -22,68 +22,105 @@ constexpr int NB = 64;
 
 // Factor the kb x kb diagonal block at A[k0,k0] (lda) in place (lower) and
 // write its inverse (lower, kb x kb, ld NB) to Linv.  Sets *notpd on failure.
-// Right-looking, unscaled: step j subtracts S[i][j] S[l][j] / S[j][j] from the
-// trailing triangle (it only reads column j, which step j does not write, so
-// one barrier per step suffices); the columns are scaled by 1/sqrt(S[j][j])
-// at the end.
+//
+// The 2080 lower-triangle entries are owned by the threads (9 slots each) and
+// live in registers.  Factorisation is right-looking and unscaled: step j
+// subtracts S[i][j] S[l][j] / S[j][j] from every owned (i, l) with l > j,
+// reading column j from a double-buffered LDS column that the owners of
+// column j published at the end of step j-1; the scaling by 1/sqrt(S[l][l])
+// is applied at the end.  The inverse is a right-looking forward
+// substitution of L X = I in the same ownership: step k finalises row k of X
+// (X[k][c] = R[k][c] / L[k][k]) and subtracts L[i][k] X[k][c] from the rows
+// below.  One barrier per step in both phases.
+constexpr int TRI = NB * (NB + 1) / 2;
+constexpr int SLOTS = (TRI + 255) / 256;
+
 __global__ __launch_bounds__(256) void chol_diag_kernel(double* __restrict__ A, int64_t lda,
                                                         int64_t k0, int kb,
                                                         double* __restrict__ Linv,
                                                         int* __restrict__ notpd) {
-    __shared__ double S[NB][NB + 1];
-    __shared__ double X[NB][NB + 1];
-    __shared__ double dsq[NB];
+    __shared__ double col[2][NB];
+    __shared__ double Ls[NB][NB + 1];
     __shared__ int bad;
     const int tid = threadIdx.x;
     if (tid == 0) bad = 0;
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int r = e / NB, c = e % NB;
-        S[r][c] = (r < kb && c < kb) ? A[(k0 + r) * lda + (k0 + c)] : 0.0;
-        X[r][c] = 0.0;
+    int si[SLOTS], sl[SLOTS];
+    bool act[SLOTS];
+    double val[SLOTS];
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u) {
+        const int e = tid + 256 * u;
+        int i = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+        while (i * (i + 1) / 2 > e) --i;
+        while ((i + 1) * (i + 2) / 2 <= e) ++i;
+        si[u] = i;
+        sl[u] = e - i * (i + 1) / 2;
+        act[u] = e < TRI && si[u] < kb;
+        val[u] = act[u] ? A[(k0 + si[u]) * lda + (k0 + sl[u])] : 0.0;
+        if (act[u] && sl[u] == 0) col[0][si[u]] = val[u];
     }
-    __syncthreads();
+    // factorisation
     for (int j = 0; j < kb; ++j) {
-        double dj = S[j][j];
+        __syncthreads();
+        const double* cj = col[j & 1];
+        double dj = cj[j];
         if (!(dj > 0.0)) {
             if (tid == 0) bad = 1;
             dj = 1.0;
         }
         const double inv = 1.0 / dj;
-        const int m = kb - j - 1;
-        for (int e = tid; e < m * m; e += 256) {
-            const int i = j + 1 + e / m, l = j + 1 + e % m;
-            if (l <= i) S[i][l] -= S[i][j] * S[l][j] * inv;
+#pragma unroll
+        for (int u = 0; u < SLOTS; ++u) {
+            if (act[u] && sl[u] > j) {
+                val[u] -= cj[si[u]] * cj[sl[u]] * inv;
+                if (sl[u] == j + 1) col[(j + 1) & 1][si[u]] = val[u];
+            }
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u)
+        if (act[u] && si[u] == sl[u]) col[0][si[u]] = val[u] > 0.0 ? sqrt(val[u]) : 1.0;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u) {
+        if (act[u]) {
+            const double d = col[0][sl[u]];
+            val[u] = (si[u] == sl[u]) ? d : val[u] / d;
+            Ls[si[u]][sl[u]] = val[u];
+            A[(k0 + si[u]) * lda + (k0 + sl[u])] = val[u];
+        }
+    }
+    // inverse: R = I, then row by row
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u) val[u] = (act[u] && si[u] == sl[u]) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int k = 0; k < kb; ++k) {
+        double* rk = col[k & 1];
+        const double dkk = Ls[k][k];
+#pragma unroll
+        for (int u = 0; u < SLOTS; ++u)
+            if (act[u] && si[u] == k) {
+                val[u] /= dkk;
+                rk[sl[u]] = val[u];
+            }
         __syncthreads();
+#pragma unroll
+        for (int u = 0; u < SLOTS; ++u)
+            if (act[u] && si[u] > k && sl[u] <= k) val[u] -= Ls[si[u]][k] * rk[sl[u]];
     }
-    if (tid < kb) {
-        const double d = S[tid][tid];
-        dsq[tid] = d > 0.0 ? sqrt(d) : 1.0;
-    }
-    __syncthreads();
-    for (int e = tid; e < kb * kb; e += 256) {
-        const int r = e / kb, c = e % kb;
-        if (c < r) S[r][c] /= dsq[c];
-    }
-    __syncthreads();
-    if (tid < kb) S[tid][tid] = dsq[tid];
-    __syncthreads();
-    // Inverse of the lower factor, one column per thread.
-    if (tid < kb) {
-        const int c = tid;
-        X[c][c] = 1.0 / S[c][c];
-        for (int i = c + 1; i < kb; ++i) {
-            double acc = 0.0;
-            for (int k = c; k < i; ++k) acc += S[i][k] * X[k][c];
-            X[i][c] = -acc / S[i][i];
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int r = e / NB, c = e % NB;
+        if (r < kb && c < kb) {
+            if (c > r) {
+                A[(k0 + r) * lda + (k0 + c)] = 0.0;
+                Linv[r * NB + c] = 0.0;
+            }
         }
     }
-    __syncthreads();
-    for (int e = tid; e < kb * kb; e += 256) {
-        const int r = e / kb, c = e % kb;
-        A[(k0 + r) * lda + (k0 + c)] = (c <= r) ? S[r][c] : 0.0;
-        Linv[r * NB + c] = (c <= r) ? X[r][c] : 0.0;
-    }
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u)
+        if (act[u]) Linv[si[u] * NB + sl[u]] = val[u];
     if (tid == 0 && bad) *notpd = 1;
 }
 

@@ -18,9 +18,10 @@
 //   A == B: x1/2 (T1 and T2 cover the same elements)
 //
 // B is repacked once into panels: panel (I, A) is a contiguous
-// [Kp][128] slab (FP64/FP32), or a [Kp/32][hi/lo][128][32] slab of BF16
-// halves (BF16x3), so every K-slab a workgroup stages into LDS is one
-// contiguous, fully coalesced read.
+// [Kp][128] slab (FP64/FP32), or a [Kp/32][hi/lo][128][4][8] slab of BF16
+// halves (BF16x3) already in the energy kernel's LDS order, so every K-slab a
+// workgroup stages into LDS is one contiguous, fully coalesced read (for BF16x3
+// a direct global->LDS copy).
 #include "common.hpp"
 
 namespace corr {
@@ -58,6 +59,10 @@ __device__ inline uint16_t f32_to_bf16_rn(float f) {
 }
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
+// Slot of 16-byte k-chunk ch in a 64-byte LDS row r of the BF16 energy kernel
+// (an involution in ch): keeps its ds_read_b128 fragment reads conflict-free.
+__device__ inline int bf16_chunk(int row, int ch) { return ch ^ (((row >> 2) & 1) << 1); }
+
 }  // namespace
 
 // ---- repack ----------------------------------------------------------------
@@ -78,7 +83,8 @@ __global__ void repack_real(const double* __restrict__ Bia, T* __restrict__ Bp, 
 }
 
 // One block per (panel, 32-row K slab): read [32 q][128 c] coalesced along a,
-// write [hi/lo][128 c][32 q] halves.
+// write [hi/lo][128 c][4 chunks][8 q] halves in the energy kernel's LDS order
+// (chunk slot s of row c holds k-chunk bf16_chunk(c, s)).
 __global__ __launch_bounds__(256) void repack_bf16(const double* __restrict__ Bia,
                                                    uint16_t* __restrict__ Bp, int64_t naux, int o,
                                                    int v, int nAC, int64_t Kc) {
@@ -95,7 +101,7 @@ __global__ __launch_bounds__(256) void repack_bf16(const double* __restrict__ Bi
     __syncthreads();
     uint16_t* out = Bp + (blk * Kc + kc) * (2 * PW * 32);
     for (int e = threadIdx.x; e < 2 * PW * 4; e += 256) {
-        const int hl = e / (PW * 4), c = (e % (PW * 4)) / 4, kq = e % 4;
+        const int hl = e / (PW * 4), c = (e % (PW * 4)) / 4, kq = bf16_chunk(c, e % 4);
         uint16_t h[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -308,17 +314,103 @@ __device__ inline corr_f4 mma3(const corr_u4& ah, const corr_u4& al, const corr_
     return mma_bf16(ah, bh, c);
 }
 
-// One slab = two 32-deep K sub-slabs (each [hi/lo][128][32] contiguous in
-// the panel); LDS rows hold 64 k values + 8 pad (conflict-free 16 B reads).
+// K advances in 32-deep stages; a stage of one panel is [hi/lo][128 rows][32 k] bf16,
+// stored in the panel exactly as in LDS: a row is four 16-byte k-chunks, chunk ch
+// of row r in slot bf16_chunk(r, ch), which keeps the ds_read_b128 fragment reads
+// (16-lane groups) free of bank conflicts on the 64-bank LDS.  Stages move
+// global -> LDS with global_load_lds_dwordx4 (no VGPR round trip, no ds_write):
+// each wave copies one (panel, hi/lo) half, 8 contiguous 1 KiB segments.  Two
+// stage buffers (2 x 64 KiB, separate LDS objects) alternate: stage kc+2 is
+// copied into the buffer stage kc has just released while stage kc+1 computes.
+typedef __attribute__((address_space(3))) void* corr_lds_ptr;
+typedef __attribute__((address_space(1))) const void* corr_gbl_ptr;
+
+typedef uint16_t Bf16Stage[4][2][PW][32];   // [panel][hi/lo][row][k slot]
+
+struct T1Frag {
+    corr_u4 ah[4], al[4], bh[2], bl[2];
+};
+struct T2Frag {
+    corr_u4 ah[2], al[2], bh[2][2], bl[2][2];
+};
+
+__device__ inline corr_u4 lds_frag(const Bf16Stage& S, int p, int hl, int row, int ko) {
+    return *reinterpret_cast<const corr_u4*>(&S[p][hl][row][ko]);
+}
+
+// T1 = (I,A) x (J,B)^T: rows c1[rb] of panel 0, columns c2 + 16 y of panel 1.
+__device__ inline void read_t1(const Bf16Stage& S, const int (&c1)[4], int c2, int ko, T1Frag& f) {
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        f.ah[rb] = lds_frag(S, 0, 0, c1[rb], ko);
+        f.al[rb] = lds_frag(S, 0, 1, c1[rb], ko);
+    }
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+        f.bh[y] = lds_frag(S, 1, 0, c2 + y * 16, ko);
+        f.bl[y] = lds_frag(S, 1, 1, c2 + y * 16, ko);
+    }
+}
+// T2 = (J,A) x (I,B)^T: rows c2 + 16 y of panel 2, columns c4[x][y] of panel 3.
+__device__ inline void read_t2(const Bf16Stage& S, const int (&c4)[2][2], int c2, int ko, T2Frag& f) {
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+        f.ah[y] = lds_frag(S, 2, 0, c2 + y * 16, ko);
+        f.al[y] = lds_frag(S, 2, 1, c2 + y * 16, ko);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            f.bh[x][y] = lds_frag(S, 3, 0, c4[x][y], ko);
+            f.bl[x][y] = lds_frag(S, 3, 1, c4[x][y], ko);
+        }
+}
+__device__ inline void mma_t1(const T1Frag& f, corr_f4 (&acc)[4][2]) {
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = mma3(f.ah[rb], f.al[rb], f.bh[cb], f.bl[cb], acc[rb][cb]);
+}
+__device__ inline void mma_t2(const T2Frag& f, corr_f4 (&acc)[4][2]) {
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+            acc[rb][cb] = mma3(f.ah[rb % 2], f.al[rb % 2], f.bh[rb / 2][cb], f.bl[rb / 2][cb], acc[rb][cb]);
+}
+
+// Copies this wave's half-panel of stage kc into S (8 x 1 KiB, one instruction each).
+__device__ inline void bf16_stage_copy(Bf16Stage& S, const uint16_t* Pw, int pw, int hw, int64_t kc,
+                                       int lane) {
+    constexpr int SUB = 2 * PW * 32;
+    const uint16_t* src = Pw + kc * SUB + hw * (PW * 32) + lane * 8;
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+        __builtin_amdgcn_global_load_lds((corr_gbl_ptr)(src + s * 512), (corr_lds_ptr)&S[pw][hw][16 * s][0],
+                                         16, 0, 0);
+}
+
+__device__ inline void bf16_stage_compute(const Bf16Stage& S, const int (&c1)[4], const int (&c4)[2][2], int c2,
+                                          int ko, corr_f4 (&acc1)[4][2], corr_f4 (&acc2)[4][2]) {
+    T1Frag f1;
+    T2Frag f2;
+    read_t1(S, c1, c2, ko, f1);
+    read_t2(S, c4, c2, ko, f2);
+    mma_t1(f1, acc1);
+    mma_t2(f2, acc2);
+}
+
+// Waits until this wave's outstanding global->LDS copies number at most N.
+#define CORR_WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt((N) | (0x7 << 4) | (0xF << 8))
+
 __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_bf16x3(
     const uint16_t* __restrict__ Bp, const double* __restrict__ eo, const double* __restrict__ ev,
     int o, int v, int nIB, int nAC, int64_t Kc, int64_t nWG, int64_t per,
     double* __restrict__ partial) {
-    constexpr int KS = 2;                // 32-deep sub-slabs per slab
-    constexpr int LDK = 32 * KS + 8;
-    constexpr int SUB = 2 * PW * 32;     // bf16 per panel per sub-slab (hi + lo)
-    constexpr int UPT = SUB / 8 / WG_THREADS;   // 16-byte units per thread per sub-slab (=2)
-    __shared__ __attribute__((aligned(16))) uint16_t L[4][2][PW][LDK];
+    constexpr int SUB = 2 * PW * 32;            // bf16 per panel per stage (hi + lo)
+    __shared__ __attribute__((aligned(16))) Bf16Stage L0;
+    __shared__ __attribute__((aligned(16))) Bf16Stage L1;
     __shared__ double sh[WG_THREADS / 64];
 
     PairTile t;
@@ -327,27 +419,17 @@ __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_bf16x3(
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = wave >> 2, wc = wave & 3, cl = lane & 15, kq = lane >> 4;
 
-    const uint16_t* P[4];
-    P[0] = Bp + ((int64_t)t.IB * nAC + t.AC) * Kc * SUB;
-    P[1] = Bp + ((int64_t)t.JB * nAC + t.BC) * Kc * SUB;
-    P[2] = Bp + ((int64_t)t.JB * nAC + t.AC) * Kc * SUB;
-    P[3] = Bp + ((int64_t)t.IB * nAC + t.BC) * Kc * SUB;
+    // Panels: 0 = (I,A), 1 = (J,B), 2 = (J,A), 3 = (I,B).  Wave w copies panel w/2, half w%2.
+    const int pw = wave >> 1, hw = wave & 1;
+    const int ib = (pw == 0 || pw == 3) ? t.IB : t.JB;
+    const int ac = (pw == 0 || pw == 2) ? t.AC : t.BC;
+    const uint16_t* Pw = Bp + ((int64_t)ib * nAC + ac) * Kc * SUB;
 
     corr_f4 acc1[4][2], acc2[4][2];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) { acc1[x][y] = corr_f4{0, 0, 0, 0}; acc2[x][y] = corr_f4{0, 0, 0, 0}; }
-
-    const int64_t nK = Kc / KS;   // Kc is a multiple of KS
-    corr_u4 reg[4][KS][UPT];
-#define CORR_BF16_LOAD(kc_)                                                                     \
-    _Pragma("unroll") for (int p = 0; p < 4; ++p)                                               \
-    _Pragma("unroll") for (int s = 0; s < KS; ++s)                                              \
-    _Pragma("unroll") for (int u = 0; u < UPT; ++u)                                             \
-        reg[p][s][u] = *reinterpret_cast<const corr_u4*>(                                          \
-            P[p] + ((kc_) * KS + s) * SUB + (int64_t)(u * WG_THREADS + threadIdx.x) * 8);
-    CORR_BF16_LOAD(0)
 
     int c1[4], c4[2][2];
 #pragma unroll
@@ -357,58 +439,35 @@ __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_bf16x3(
 #pragma unroll
         for (int y = 0; y < 2; ++y) c4[x][y] = (2 * wr + x) * PA + y * 16 + cl;
     const int c2 = wc * PA + cl;
+    // Rows read by this lane all share r & 15 = cl, so they share the chunk slot.
+    const int ko = bf16_chunk(cl, kq) * 8;
 
-    for (int64_t kc = 0; kc < nK; ++kc) {
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-            for (int s = 0; s < KS; ++s)
-#pragma unroll
-                for (int u = 0; u < UPT; ++u) {
-                    const int e = u * WG_THREADS + threadIdx.x;
-                    const int hl = e / (PW * 4), c = (e % (PW * 4)) / 4, q = e % 4;
-                    *reinterpret_cast<corr_u4*>(&L[p][hl][c][s * 32 + q * 8]) = reg[p][s][u];
-                }
-        __syncthreads();
-        if (kc + 1 < nK) { CORR_BF16_LOAD(kc + 1) }
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int ko = s * 32 + kq * 8;
-#define frag(p_, hl_, c_) (*reinterpret_cast<const corr_u4*>(&L[p_][hl_][c_][ko]))
-            {
-                corr_u4 ah[4], al[4], bh[2], bl[2];
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb) { ah[rb] = frag(0, 0, c1[rb]); al[rb] = frag(0, 1, c1[rb]); }
-#pragma unroll
-                for (int y = 0; y < 2; ++y) { bh[y] = frag(1, 0, c2 + y * 16); bl[y] = frag(1, 1, c2 + y * 16); }
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-                    for (int cb = 0; cb < 2; ++cb) acc1[rb][cb] = mma3(ah[rb], al[rb], bh[cb], bl[cb], acc1[rb][cb]);
-            }
-            {
-                corr_u4 ah[2], al[2], bh[2][2], bl[2][2];
-#pragma unroll
-                for (int y = 0; y < 2; ++y) { ah[y] = frag(2, 0, c2 + y * 16); al[y] = frag(2, 1, c2 + y * 16); }
-#pragma unroll
-                for (int x = 0; x < 2; ++x)
-#pragma unroll
-                    for (int y = 0; y < 2; ++y) { bh[x][y] = frag(3, 0, c4[x][y]); bl[x][y] = frag(3, 1, c4[x][y]); }
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-                    for (int cb = 0; cb < 2; ++cb)
-                        acc2[rb][cb] = mma3(ah[rb % 2], al[rb % 2], bh[rb / 2][cb], bl[rb / 2][cb], acc2[rb][cb]);
-            }
+    // Each loop iteration starts with no copy in flight, so the compiler's own
+    // waits before LDS reads see only the copy into the other buffer.
+    bf16_stage_copy(L0, Pw, pw, hw, 0, lane);
+    CORR_WAIT_VMCNT(0);
+    __syncthreads();
+    if (Kc > 1) bf16_stage_copy(L1, Pw, pw, hw, 1, lane);
+    bf16_stage_compute(L0, c1, c4, c2, ko, acc1, acc2);
+    CORR_WAIT_VMCNT(0);
+    __syncthreads();   // L0 released; stage 1 complete in L1
+    for (int64_t kc = 1; kc < Kc; kc += 2) {
+        if (kc + 1 < Kc) bf16_stage_copy(L0, Pw, pw, hw, kc + 1, lane);
+        bf16_stage_compute(L1, c1, c4, c2, ko, acc1, acc2);
+        CORR_WAIT_VMCNT(0);
+        __syncthreads();   // L1 released; stage kc+1 complete in L0
+        if (kc + 1 < Kc) {
+            if (kc + 2 < Kc) bf16_stage_copy(L1, Pw, pw, hw, kc + 2, lane);
+            bf16_stage_compute(L0, c1, c4, c2, ko, acc1, acc2);
+            CORR_WAIT_VMCNT(0);
+            __syncthreads();   // L0 released; stage kc+2 complete in L1
         }
     }
-#undef frag
-#undef CORR_BF16_LOAD
     double sos = 0.0, sx = 0.0;
     energy_epilogue(acc1, acc2, t, o, v, eo, ev, sos, sx);
     write_partial(sos, sx, t, id, sh, partial);
 }
+#undef CORR_WAIT_VMCNT
 
 // ---- host side -------------------------------------------------------------------
 struct RiLayout {

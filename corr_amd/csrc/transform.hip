@@ -153,23 +153,61 @@ int ri_bpq(corr_ctx* ctx, int64_t naux, int64_t N, const double* M, const double
                 gemm_prec(prec));
 }
 
-// X[Q][i][a] = sum_{mu nu} C[i,mu] C[o+a,nu] Y[Q][mu][nu]
-static int ao2ov(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Y,
+// X[Q][a][i] = sum_{mu nu} C[o+a,nu] C[i,mu] Y[Q][mu][nu]: the first AO index
+// is contracted with the occupied block, as in transform_ri, into
+// Z[Q][nu][i] (one tile column holds all o <= 160), then the second with the
+// virtual block.  Output layout [Q][a][i].
+static int ao2vo(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Y,
                  double* X, int p) {
     const int64_t v = N - o;
-    double* BiQ = nullptr;
-    CORR_TRY(scratch(ctx, "ri_BiQ", sizeof(double) * naux * o * N, (void**)&BiQ));
-    // BiQ[Q] (o x N) = C_occ (o x N) * Y[Q] (N x N)
-    CORR_TRY(gemm(ctx, 0, 0, o, N, N, 1.0, C, N, 0, Y, N, N * N, 0.0, BiQ, N, o * N, naux, p));
-    // X (naux*o x v) = BiQ (naux*o x N) * C_vir^T
-    CORR_TRY(gemm(ctx, 0, 1, naux * o, v, N, 1.0, BiQ, N, 0, C + o * N, N, 0, 0.0, X, v, 0, 1, p));
+    double* Z = nullptr;
+    CORR_TRY(scratch(ctx, "ri_Znui", sizeof(double) * naux * N * o, (void**)&Z));
+    // Z[Q] (N x o) = Y[Q]^T (N x N) * C_occ^T (N x o)
+    CORR_TRY(gemm(ctx, 1, 1, N, o, N, 1.0, Y, N, N * N, C, N, 0, 0.0, Z, o, N * o, naux, p));
+    // X[Q] (v x o) = C_vir (v x N) * Z[Q] (N x o)
+    CORR_TRY(gemm(ctx, 0, 0, v, o, N, 1.0, C + o * N, N, 0, Z, o, N * o, 0.0, X, o, v * o, naux, p));
+    return CORR_OK;
+}
+
+// dst[q][i][a] = src[q][a][i]  (batched transpose through LDS)
+__global__ __launch_bounds__(256) void transpose_vo_kernel(const double* __restrict__ src,
+                                                           double* __restrict__ dst, int64_t v, int64_t o,
+                                                           int64_t tv, int64_t to) {
+    __shared__ double t[32][33];
+    const int64_t q = blockIdx.y;
+    const int64_t ta = (blockIdx.x / to) * 32, ti = (blockIdx.x % to) * 32;
+    const double* s = src + q * v * o;
+    double* d = dst + q * v * o;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t a = ta + r, i = ti + tx;
+        t[r][tx] = (a < v && i < o) ? s[a * o + i] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t i = ti + r, a = ta + tx;
+        if (a < v && i < o) d[i * v + a] = t[tx][r];
+    }
+}
+
+static int transpose_vo(corr_ctx* ctx, int64_t naux, int64_t v, int64_t o, const double* src, double* dst) {
+    const int64_t tv = cdiv(v, 32), to = cdiv(o, 32);
+    for (int64_t q0 = 0; q0 < naux; q0 += 65535) {
+        const int64_t nq = std::min<int64_t>(65535, naux - q0);
+        hipLaunchKernelGGL(transpose_vo_kernel, dim3((unsigned)(tv * to), (unsigned)nq), dim3(256), 0, ctx->stream,
+                           src + q0 * v * o, dst + q0 * v * o, v, o, tv, to);
+        CORR_LAUNCHED(ctx, "transpose_vo_kernel");
+    }
     return CORR_OK;
 }
 
 int transform_ri(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C,
                  const double* BPQ, double* Bia, int prec) {
     CORR_REQUIRE(ctx, naux > 0 && N > 0 && o > 0 && o < N && C && BPQ && Bia, "transform_ri: bad arguments");
-    return ao2ov(ctx, naux, N, o, C, BPQ, Bia, gemm_prec(prec));
+    double* X = nullptr;
+    CORR_TRY(scratch(ctx, "ri_Xvo", sizeof(double) * naux * o * (N - o), (void**)&X));
+    CORR_TRY(ao2vo(ctx, naux, N, o, C, BPQ, X, gemm_prec(prec)));
+    return transpose_vo(ctx, naux, N - o, o, X, Bia);
 }
 
 int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Pmn,
@@ -179,11 +217,14 @@ int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double*
     CORR_REQUIRE(ctx, mode == CORR_METRIC_EIGEN || mode == CORR_METRIC_CHOLESKY, "ri_form_bia: unknown metric mode");
     const int64_t v = N - o;
     const int p = gemm_prec(prec);
-    double* X = nullptr;
-    CORR_TRY(scratch(ctx, "ri_Pia", sizeof(double) * naux * o * v, (void**)&X));
-    CORR_TRY(ao2ov(ctx, naux, N, o, C, Pmn, X, p));
-    return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1, p,
-                mode == CORR_METRIC_CHOLESKY);
+    double *X = nullptr, *Y = nullptr;
+    CORR_TRY(scratch(ctx, "ri_Xvo", sizeof(double) * naux * o * v, (void**)&X));
+    CORR_TRY(scratch(ctx, "ri_Yvo", sizeof(double) * naux * o * v, (void**)&Y));
+    CORR_TRY(ao2vo(ctx, naux, N, o, C, Pmn, X, p));
+    // Y[Q][(a i)] = sum_P M[Q][P] X[P][(a i)]   (M lower triangular for CHOLESKY)
+    CORR_TRY(gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Y, o * v, 0, 1, p,
+                  mode == CORR_METRIC_CHOLESKY));
+    return transpose_vo(ctx, naux, v, o, Y, Bia);
 }
 
 }  // namespace corr
