@@ -6,12 +6,14 @@
 // FP32 path: operands rounded to FP32 when staged into LDS,
 //            v_mfma_f32_16x16x4_f32, FP32 accumulate, epilogue in FP64.
 //
-// A workgroup is 8 wave64s (two per SIMD); each wave owns a 64x32 block of C
-// built from 4x2 MFMA 16x16 tiles.  Tiles of A and B are staged through LDS as [k][m]
-// (m contiguous, padded so that a wave's fragment read touches all 64 banks
-// once) with the next K-slab prefetched into registers while the current one
-// is consumed.  Global loads are lane-contiguous along whichever dimension
-// the operand is contiguous in.
+// A workgroup is 8 or 10 wave64s; each wave owns a 64x32 block of C built from
+// 4x2 MFMA 16x16 tiles.  A K-slab is BK deep (32 for FP32, 16 for FP64); the
+// four k-rows of one MFMA step are spread over the slab as k = g * BK/4 + t
+// (lane group g = lane / 16, step t), so that a lane's k values within a slab
+// are contiguous.  An operand that is contiguous along K is staged as [x][k]
+// rows and read as two ds_read_b128 per fragment row; an operand contiguous
+// along x is staged as [k][x] and read as scalars.  Global loads move pairs of
+// doubles; LDS holds two slabs, so each slab costs one barrier.
 #include "common.hpp"
 
 namespace corr {
@@ -40,76 +42,111 @@ template <> struct Mfma<float> {
     }
 };
 
-// One operand slab (BK deep, BDIM wide) of a GEMM tile.
-//  CONTIG_ALONG_K == false: element (k, x) at P[k*ld + x] (x contiguous); LDS
-//    tile [k][x] with rows padded by 16 elements, so a fragment read (16 x
-//    values per lane group) and a store (consecutive x per lane) both touch
-//    distinct banks.
-//  CONTIG_ALONG_K == true : element (k, x) at P[x*ld + k] (k contiguous); LDS
-//    tile [x][k] with rows of BK + KPAD elements (36 floats / 18 doubles:
-//    a row stride of 36 dwords), so the 16 rows of a fragment read and the
-//    consecutive-k stores of a lane group fall in distinct banks.
-template <typename T> struct KPad;
-template <> struct KPad<float> { static constexpr int v = 4; };
-template <> struct KPad<double> { static constexpr int v = 2; };
+typedef double corr_d2 __attribute__((ext_vector_type(2)));
 
-template <int BDIM, int BK, bool CONTIG_ALONG_K, int NT, typename T>
-struct Slab {
-    static constexpr int ROWS = CONTIG_ALONG_K ? BDIM : BK;
-    static constexpr int LDW = CONTIG_ALONG_K ? BK + KPad<T>::v : BDIM + 16;
-    static constexpr int ELEMS = (BDIM * BK + NT - 1) / NT;
+// Slab geometry: BK elements of k per slab, KV = BK/4 per lane group, CH
+// elements per 16-byte LDS chunk; XPAD pads [k][x] rows to a stride of
+// 2 (FP32) or 4 (FP64) modulo 8 elements (4 lane groups in distinct banks).
+template <typename T> struct GemmSlab;
+template <> struct GemmSlab<double> {
+    static constexpr int BK = 16, KV = 4, CH = 2, XPAD = 4;
+};
+template <> struct GemmSlab<float> {
+    static constexpr int BK = 32, KV = 8, CH = 4, XPAD = 2;
+};
+
+// Slot of 16-byte chunk ch in a 128-byte [x][k] row x (an involution in ch):
+// keeps the ds_read_b128 fragment reads of 16 consecutive rows conflict-free.
+__device__ inline int gemm_chunk(int x, int ch) {
+    return ch ^ (((x >> 1) & 1) | (((x >> 3) & 1) << 2));
+}
+
+// One operand slab (BK deep, BDIM wide) of a GEMM tile, staged as element pairs.
+//  KC == true : element (k, x) at P[x*ld + k]; LDS [BDIM][BK], chunk-swizzled rows.
+//  KC == false: element (k, x) at P[k*ld + x]; LDS [BK][BDIM + XPAD].
+template <int BDIM, bool KC, int NT, typename T>
+struct Opnd {
+    typedef GemmSlab<T> G;
+    static constexpr int BK = G::BK, KV = G::KV;
+    static constexpr int LDW = KC ? BK : BDIM + G::XPAD;
+    static constexpr int SIZE = KC ? BDIM * BK : BK * LDW;   // elements of one LDS slab
+    static constexpr int PAIRS = BDIM * BK / 2;
+    static constexpr int PPT = (PAIRS + NT - 1) / NT;
 
     __device__ static inline void coords(int e, int& kk, int& xx) {
-        if (CONTIG_ALONG_K) { xx = e / BK; kk = e % BK; }
-        else                { kk = e / BDIM; xx = e % BDIM; }
+        if (KC) { xx = e / (BK / 2); kk = (e % (BK / 2)) * 2; }
+        else    { kk = e / (BDIM / 2); xx = (e % (BDIM / 2)) * 2; }
     }
     __device__ static inline void load(const double* __restrict__ P, int64_t ld, int64_t x0, int64_t xmax,
-                                       int64_t k0, int64_t kmax, T* regs) {
+                                       int64_t k0, int64_t kmax, corr_d2* regs) {
 #pragma unroll
-        for (int u = 0; u < ELEMS; ++u) {
+        for (int u = 0; u < PPT; ++u) {
             const int e = u * NT + (int)threadIdx.x;
             int kk, xx;
             coords(e, kk, xx);
             const int64_t k = k0 + kk, x = x0 + xx;
-            double v = 0.0;
-            if (e < BDIM * BK && k < kmax && x < xmax) v = CONTIG_ALONG_K ? P[x * ld + k] : P[k * ld + x];
-            regs[u] = (T)v;
+            corr_d2 v = {0.0, 0.0};
+            if (PAIRS % NT == 0 || e < PAIRS) {
+                // The pair runs along the contiguous dimension: (k, k+1) or (x, x+1).
+                const int64_t i0 = KC ? x * ld + k : k * ld + x;
+                const bool in0 = k < kmax && x < xmax;
+                const bool in1 = KC ? (k + 1 < kmax && x < xmax) : (k < kmax && x + 1 < xmax);
+                if (in0 && in1 && ((reinterpret_cast<uintptr_t>(P + i0) & 15) == 0)) {
+                    v = *reinterpret_cast<const corr_d2*>(P + i0);
+                } else {
+                    if (in0) v.x = P[i0];
+                    if (in1) v.y = P[i0 + 1];
+                }
+            }
+            regs[u] = v;
         }
     }
-    __device__ static inline void store(T (*S)[LDW], const T* regs) {
+    __device__ static inline void store(T* S, const corr_d2* regs) {
 #pragma unroll
-        for (int u = 0; u < ELEMS; ++u) {
+        for (int u = 0; u < PPT; ++u) {
             const int e = u * NT + (int)threadIdx.x;
-            if (BDIM * BK % NT != 0 && e >= BDIM * BK) break;
+            if (PAIRS % NT != 0 && e >= PAIRS) break;
             int kk, xx;
             coords(e, kk, xx);
-            if (CONTIG_ALONG_K) S[xx][kk] = regs[u];
-            else S[kk][xx] = regs[u];
+            const int off = KC ? xx * BK + gemm_chunk(xx, kk / G::CH) * G::CH + kk % G::CH : kk * LDW + xx;
+            if (sizeof(T) == 4) {
+                float2 f = make_float2((float)regs[u].x, (float)regs[u].y);
+                *reinterpret_cast<float2*>(S + off) = f;
+            } else {
+                *reinterpret_cast<corr_d2*>(S + off) = regs[u];
+            }
         }
     }
-    __device__ static inline T frag(const T (*S)[LDW], int k, int x) {
-        return CONTIG_ALONG_K ? S[x][k] : S[k][x];
+    // Values h*PER .. h*PER+PER-1 of the lane group's KV k positions (slab
+    // positions g*KV + ...) for row/column x; PER = elements per 16 bytes.
+    static constexpr int PER = 16 / sizeof(T);
+    __device__ static inline void frag(const T* S, int g, int h, int x, T (&f)[PER]) {
+        if (KC) {
+            typedef T vec_t __attribute__((ext_vector_type(PER)));
+            const vec_t q = *reinterpret_cast<const vec_t*>(S + x * BK + gemm_chunk(x, g * (KV / PER) + h) * PER);
+#pragma unroll
+            for (int e = 0; e < PER; ++e) f[e] = q[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < PER; ++e) f[e] = S[(g * KV + h * PER + e) * LDW + x];
+        }
     }
 };
-
-template <typename T> struct GemmSlab;
-template <> struct GemmSlab<double> { static constexpr int BK = 16; };
-template <> struct GemmSlab<float> { static constexpr int BK = 32; };
 
 // WM x WN waves (8 or 10, two or more per SIMD); each wave owns 64 x 32 of C
 // (4 x 2 MFMA tiles).  BM = 64 WM, BN = 32 WN.
 template <int TA, int TB, typename T, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
     constexpr int NT = 64 * WM * WN;
-    constexpr int BM = 64 * WM, BN = 32 * WN, BK = GemmSlab<T>::BK;
-    typedef Slab<BM, BK, TA == 0, NT, T> SA;   // op(A)(m,k): TA==0 -> A[m*lda+k]; TA==1 -> A[k*lda+m]
-    typedef Slab<BN, BK, TB == 1, NT, T> SB;   // op(B)(k,n): TB==0 -> B[k*ldb+n]; TB==1 -> B[n*ldb+k]
-    __shared__ T As[SA::ROWS][SA::LDW];
-    __shared__ T Bs[SB::ROWS][SB::LDW];
+    constexpr int BM = 64 * WM, BN = 32 * WN, BK = GemmSlab<T>::BK, KV = GemmSlab<T>::KV;
+    typedef Opnd<BM, TA == 0, NT, T> OA;   // op(A)(m,k): TA==0 -> A[m*lda+k]; TA==1 -> A[k*lda+m]
+    typedef Opnd<BN, TB == 1, NT, T> OB;   // op(B)(k,n): TB==0 -> B[k*ldb+n]; TB==1 -> B[n*ldb+k]
+    __shared__ __attribute__((aligned(16))) T As[2][OA::SIZE];
+    __shared__ __attribute__((aligned(16))) T Bs[2][OB::SIZE];
     typedef typename Mfma<T>::acc_t acc_t;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave / WN, wn = wave % WN;
+    const int wm = wave / WN, wn = wave % WN, lg = lane >> 4, lc = lane & 15;
     const int64_t tm = blockIdx.x % g.tilesM, tn = blockIdx.x / g.tilesM;
     const int64_t m0 = tm * BM, n0 = tn * BN;
     const int64_t b = blockIdx.y;
@@ -127,40 +164,51 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    T ra[SA::ELEMS], rb[SB::ELEMS];
+    corr_d2 ra[OA::PPT], rb[OB::PPT];
     if (kTiles > 0) {
-        SA::load(A, g.lda, m0, g.M, 0, kEnd, ra);
-        SB::load(B, g.ldb, n0, g.N, 0, kEnd, rb);
+        OA::load(A, g.lda, m0, g.M, 0, kEnd, ra);
+        OB::load(B, g.ldb, n0, g.N, 0, kEnd, rb);
+        OA::store(As[0], ra);
+        OB::store(Bs[0], rb);
+        if (kTiles > 1) {
+            OA::load(A, g.lda, m0, g.M, BK, kEnd, ra);
+            OB::load(B, g.ldb, n0, g.N, BK, kEnd, rb);
+        }
     }
+    __syncthreads();
     for (int64_t kt = 0; kt < kTiles; ++kt) {
-        __syncthreads();
-        SA::store(As, ra);
-        SB::store(Bs, rb);
-        __syncthreads();
+        const int buf = (int)(kt & 1);
+        constexpr int PER = OA::PER;
         if (kt + 1 < kTiles) {
-            SA::load(A, g.lda, m0, g.M, (kt + 1) * BK, kEnd, ra);
-            SB::load(B, g.ldb, n0, g.N, (kt + 1) * BK, kEnd, rb);
+            OA::store(As[buf ^ 1], ra);
+            OB::store(Bs[buf ^ 1], rb);
+            if (kt + 2 < kTiles) {
+                OA::load(A, g.lda, m0, g.M, (kt + 2) * BK, kEnd, ra);
+                OB::load(B, g.ldb, n0, g.N, (kt + 2) * BK, kEnd, rb);
+            }
         }
-#pragma unroll 2
-        for (int kk = 0; kk < BK; kk += 4) {
-            const int kr = kk + (lane >> 4);
-            T fa[4], fb[2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) fa[i] = SA::frag(As, kr, wm * 64 + i * 16 + (lane & 15));
+        for (int h = 0; h < KV / PER; ++h) {
+            T fa[4][PER], fb[2][PER];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) fb[j] = SB::frag(Bs, kr, wn * 32 + j * 16 + (lane & 15));
+            for (int i = 0; i < 4; ++i) OA::frag(As[buf], lg, h, wm * 64 + i * 16 + lc, fa[i]);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 2; ++j) OB::frag(Bs[buf], lg, h, wn * 32 + j * 16 + lc, fb[j]);
 #pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(fa[i], fb[j], acc[i][j]);
+            for (int t = 0; t < PER; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(fa[i][t], fb[j][t], acc[i][j]);
         }
+        __syncthreads();
     }
 
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
+            const int64_t col = n0 + wn * 32 + j * 16 + lc;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t row = m0 + wm * 64 + i * 16 + AccLayout<acc_t>::row(lane, r);

@@ -42,10 +42,10 @@ def test_gemm_fp64(corr, ta, tb, M, N, K):
     assert np.allclose(out, ref, rtol=1e-12, atol=1e-12 * np.sqrt(K))
 
 
-@pytest.mark.parametrize("ta,tb", [(0, 1), (1, 0)])
-def test_gemm_fp32(corr, ta, tb):
-    M, N, K = 200, 150, 300
-    _say(f"gemm fp32 ta={ta} tb={tb}")
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(200, 150, 300), (37, 53, 29), (130, 257, 70), (300, 64, 33)])
+def test_gemm_fp32(corr, ta, tb, M, N, K):
+    _say(f"gemm fp32 ta={ta} tb={tb} M={M} N={N} K={K}")
     rng = np.random.default_rng(3)
     A = rng.standard_normal((K, M) if ta else (M, K))
     B = rng.standard_normal((N, K) if tb else (K, N))
