@@ -15,7 +15,12 @@ value = algorithmic TFLOP/s of the mixed path, (W_B + W_E) / t_step, with
 W_E = 2 [o(o+1)/2] v^2 naux and W_B = 2 naux o N^2 + 2 naux o v N + 2 naux^2 o v.
 Inputs are seeded synthetic tensors of the real shapes (no integral files).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--suite]
+
+--suite adds a "suite" object with the other rows of the hot path, each at its
+own shape: build_FMo (N=768), AO unfold + 4-index transform + canonical MP2
+((H2O)8, N=192), metric and B formation ((H2O)32), RI-MP2 gradient ((H2O)16)
+and CCSD per iteration ((H2O)4).
 """
 from __future__ import annotations
 
@@ -98,6 +103,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (a known-size kernel for scaling HBM counters)")
+    ap.add_argument("--suite", action="store_true",
+                    help="also time every other row of the hot path at its own shape (adds a 'suite' key)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -194,8 +201,207 @@ def main() -> int:
                           "frac": round(W_E / t_k64 / 1e12 / 78.6, 4)},
         "cpu_baseline": cpu,
     }
+    if args.suite:
+        del C, J, Pmn
+        torch.cuda.empty_cache()
+        line["suite"] = run_suite(dev, args.seed)
     print(json.dumps(line), flush=True)
     return 0
+
+
+# ---------------------------------------------------------------------------
+# Suite: the other rows of the hot path, each at its own shape.
+# ---------------------------------------------------------------------------
+
+def _timed(fn, reps: int = 3):
+    """Median wall time (s) of fn() over reps runs after one warm-up, with the
+    device synchronised around each run."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+        del out
+    return float(np.median(ts))
+
+
+def unique_list_gpu(N: int, device, seed: int):
+    """Canonical AO ERI quadruples (a>=b, c>=d, ab>=cd) in the reference's
+    stored form: resort step 1 (ops_rhf.cc:766-818) and the five permutation
+    classes (ops_rhf.cc:820-896), with seeded random values.  Returns
+    (intval, intnums int16 [n,4], sortcount)."""
+    import torch
+    P = N * (N + 1) // 2
+    T = P * (P + 1) // 2
+    k = torch.arange(T, dtype=torch.int64, device=device)
+
+    def tri(x):
+        r = torch.floor((torch.sqrt(8.0 * x.double() + 1.0) - 1.0) / 2.0).long()
+        r = torch.where(r * (r + 1) // 2 > x, r - 1, r)
+        r = torch.where((r + 1) * (r + 2) // 2 <= x, r + 1, r)
+        return r, x - r * (r + 1) // 2
+
+    p, q = tri(k)
+    del k
+    a, b = tri(p)
+    c, d = tri(q)
+    del p, q
+    oa, ob, oc, od = a.clone(), b.clone(), c.clone(), d.clone()
+
+    def assign(m, w, x, y, z):
+        a[m], b[m], c[m], d[m] = w[m], x[m], y[m], z[m]
+
+    assign((oa == ob) & (ob == od) & (oc != od), oa, oa, oa, oc)
+    assign((oa != ob) & (oa == oc) & (oc == od), oa, oa, oa, ob)
+    assign((oa != ob) & (ob == oc) & (oc == od), ob, ob, ob, oa)
+    assign((oa == od) & (ob == oc) & (oa != ob), oa, ob, od, oc)
+    assign((ob == oc) & (ob != oa) & (ob != od) & (oa != od), ob, oa, oc, od)
+    assign((oc == od) & (oc != oa) & (oc != ob) & (oa != ob), oc, oc, oa, ob)
+    assign((oa == od) & (oa != ob) & (oa != oc) & (ob != oc), oa, ob, od, oc)
+    assign((ob == od) & (ob != oa) & (ob != oc) & (oa != oc), ob, oa, od, oc)
+    del oa, ob, oc, od
+    cls = torch.full_like(a, 4)
+    m0 = (a == b) & (b == c) & (c == d)
+    m1 = ~m0 & (a == b) & (c == d)
+    m2 = ~m0 & ~m1 & (a == c) & (b == d)
+    m3 = ~m0 & ~m1 & ~m2 & (a == b)
+    cls[m3] = 3
+    cls[m2] = 2
+    cls[m1] = 1
+    cls[m0] = 0
+    order = torch.sort(cls, stable=True).indices
+    sortcount = [int((cls <= j).sum()) for j in range(4)]
+    nums = torch.stack([a, b, c, d], dim=1)[order].to(torch.int16)
+    del a, b, c, d, cls, order
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    val = torch.randn(T, dtype=torch.float64, device=device, generator=g) * 1e-2
+    return val, nums.contiguous(), sortcount
+
+
+def run_suite(dev, seed: int) -> dict:
+    import torch
+    import corr_amd
+    out = {"peaks_TFLOPs": {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}}
+
+    def rec(name, fn):
+        try:
+            out[name] = fn()
+            log(f"suite {name}: {out[name]}")
+        except Exception as exc:   # a failing row is reported, not fatal
+            out[name] = {"error": f"{type(exc).__name__}: {exc}"}
+            log(f"suite {name}: FAILED {exc}")
+        torch.cuda.empty_cache()
+
+    rng = np.random.default_rng(seed)
+
+    # a1: FMo = C F C^T at N = 768
+    def fmo():
+        N = 768
+        Cm = random_orthogonal(N, rng, dev)
+        F = torch.randn(N, N, dtype=torch.float64, device=dev)
+        t = _timed(lambda: corr_amd.build_FMo(Cm, F))
+        return {"shape": "N=768", "ms": round(t * 1e3, 3), "TFLOPs": round(4 * N ** 3 / t / 1e12, 2)}
+    rec("build_fmo_h2o32", fmo)
+
+    # a2-a4, a6: unfold + 4-index transform + canonical MP2 at (H2O)8, N = 192, o = 40
+    def fourindex():
+        N, o = 192, 40
+        val, nums, sc = unique_list_gpu(N, dev, seed)
+        n_unique = val.numel()
+        t_unf = _timed(lambda: corr_amd.ao_unfold(N, val, nums, sc), reps=2)
+        dense = corr_amd.ao_unfold(N, val, nums, sc)
+        nbytes_unf = n_unique * 16 + dense.numel() * 8
+        del val, nums
+        torch.cuda.empty_cache()
+        Cm = random_orthogonal(N, rng, dev)
+        W4 = 8.0 * N ** 5
+        res = {"shape": "N=192 (H2O)8", "unique_integrals": n_unique,
+               "unfold_ms": round(t_unf * 1e3, 2),
+               "unfold_GBs": round(nbytes_unf / t_unf / 1e9, 1), "W_4_TFLOP": round(W4 / 1e12, 3)}
+        for prec in ("fp64", "fp32"):
+            t = _timed(lambda: corr_amd.MOtrans(Cm, dense, prec), reps=2)
+            res[f"motrans_{prec}_ms"] = round(t * 1e3, 2)
+            res[f"motrans_{prec}_TFLOPs"] = round(W4 / t / 1e12, 2)
+        ints = corr_amd.MOtrans(Cm, dense, "fp64")
+        del dense
+        torch.cuda.empty_cache()
+        eo, ev = water_spectrum(8, rng)
+        FMo = torch.diag(torch.from_numpy(np.concatenate([eo, ev])).to(dev)).contiguous()
+        t = _timed(lambda: corr_amd.run_canonical_mp2(ints, FMo, o))
+        res["canonical_mp2_ms"] = round(t * 1e3, 3)
+        res["canonical_mp2_GBs"] = round(2 * o * o * (N - o) ** 2 * 8 / t / 1e9, 1)
+        return res
+    rec("fourindex_h2o8", fourindex)
+
+    # a7-a8: metric and B formation at (H2O)32
+    def bform():
+        C, J, Pmn, eo, ev = make_inputs(32, seed, dev)
+        N, o, v, naux = sizes(32)
+        res = {"shape": "(H2O)32 N=768 naux=2432"}
+        res["metric_eigen_ms"] = round(_timed(lambda: corr_amd.ri_metric(J, "eigen")) * 1e3, 2)
+        res["metric_cholesky_ms"] = round(_timed(lambda: corr_amd.ri_metric(J, "cholesky")) * 1e3, 2)
+        M, _ = corr_amd.ri_metric(J, "cholesky")
+        WB = work(N, o, v, naux)[1]
+        for prec in ("fp64", "fp32"):
+            t = _timed(lambda: corr_amd.ri_form_bia(C, Pmn, M, o, "cholesky", prec))
+            res[f"form_bia_{prec}_ms"] = round(t * 1e3, 2)
+            res[f"form_bia_{prec}_TFLOPs"] = round(WB / t / 1e12, 2)
+        return res
+    rec("b_formation_h2o32", bform)
+
+    # new row: RI-MP2 gradient at (H2O)16
+    def grad():
+        N, o, v, naux = sizes(16)
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        B = 0.02 * torch.randn(naux, o, v, dtype=torch.float64, device=dev, generator=g)
+        eo, ev = water_spectrum(16, rng)
+        eo, ev = torch.from_numpy(eo).to(dev), torch.from_numpy(ev).to(dev)
+        WE = work(N, o, v, naux)[0]
+        res = {"shape": "(H2O)16 o=80 v=304 naux=1216", "W_E_TFLOP": round(WE / 1e12, 4)}
+        for prec in ("fp64", "fp32"):
+            te = _timed(lambda: corr_amd.rimp2_energy(B, eo, ev, prec))
+            tg = _timed(lambda: corr_amd.rimp2_gradient(B, eo, ev, 1.0, 1.0, prec))
+            res[f"energy_{prec}_ms"] = round(te * 1e3, 2)
+            res[f"gradient_{prec}_ms"] = round(tg * 1e3, 2)
+            res[f"gradient_{prec}_TFLOPs"] = round(3 * WE / tg / 1e12, 2)
+        return res
+    rec("rimp2_gradient_h2o16", grad)
+
+    # a10-a19: CCSD at (H2O)4, N = 96, 40 electrons (spin orbitals: o = 40, v = 152)
+    def ccsd():
+        N, nroe = 96, 40
+        o, v = nroe, 2 * N - nroe
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        h = torch.diag(torch.linspace(-2.0, 1.5, N, dtype=torch.float64, device=dev))
+        h = h + 0.02 * torch.randn(N, N, dtype=torch.float64, device=dev, generator=g)
+        h = 0.5 * (h + h.T)
+        X = torch.randn(2 * N, N, N, dtype=torch.float64, device=dev, generator=g) / N
+        X = 0.5 * (X + X.transpose(1, 2))
+        Xm = X.reshape(2 * N, N * N)
+        eri = (0.05 * (Xm.T @ Xm)).reshape(N, N, N, N).contiguous()
+        del X, Xm
+        Cm = torch.eye(N, dtype=torch.float64, device=dev)
+        W = 4.0 * o * o * v ** 4 + 4.0 * o ** 3 * v ** 3 + 4.0 * o ** 4 * v * v
+        res = {"shape": "(H2O)4 N=96 spin-orbital o=40 v=152", "W_CC_TFLOP_per_iter": round(W / 1e12, 3)}
+        for prec in ("fp64", "fp32"):
+            t1 = _timed(lambda: corr_amd.ccsd_ur(eri, Cm, h, nroe, 0.0, maxiter=1, prec=prec), reps=1)
+            t11 = _timed(lambda: corr_amd.ccsd_ur(eri, Cm, h, nroe, 0.0, maxiter=11, prec=prec), reps=1)
+            it = (t11 - t1) / 10
+            res[f"iteration_{prec}_ms"] = round(it * 1e3, 2)
+            res[f"iteration_{prec}_TFLOPs"] = round(W / it / 1e12, 2)
+            res[f"setup_{prec}_ms"] = round((t1 - it) * 1e3, 1)
+        r = corr_amd.ccsd_ur(eri, Cm, h, nroe, 0.0, maxiter=20)
+        res["e_ccsd_iter20"] = r["e_ccsd"]
+        return res
+    rec("ccsd_h2o4", ccsd)
+    return out
 
 
 def cpu_baseline(C, Pmn, J, o, eo, ev, W_E, budget_s):
