@@ -26,6 +26,7 @@ struct GemmArgs {
     double alpha, beta;
     int64_t tilesM;
     int a_lower;
+    int64_t ksplit;   // > 0: blockIdx.y is a K chunk of this depth (C = partial slice), not a batch index
 };
 
 template <typename T> struct Mfma;
@@ -150,12 +151,20 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
     const int64_t tm = blockIdx.x % g.tilesM, tn = blockIdx.x / g.tilesM;
     const int64_t m0 = tm * BM, n0 = tn * BN;
     const int64_t b = blockIdx.y;
-    const double* A = g.A + b * g.sA;
-    const double* B = g.B + b * g.sB;
-    double* C = g.C + b * g.sC;
-
+    const double* A;
+    const double* B;
     int64_t kEnd = g.K;
-    if (g.a_lower) kEnd = (m0 + BM < g.K) ? (m0 + BM) : g.K;
+    if (g.ksplit > 0) {
+        const int64_t k0 = b * g.ksplit;
+        A = g.A + (TA ? k0 * g.lda : k0);
+        B = g.B + (TB ? k0 : k0 * g.ldb);
+        kEnd = (g.K - k0 < g.ksplit) ? (g.K - k0) : g.ksplit;
+    } else {
+        A = g.A + b * g.sA;
+        B = g.B + b * g.sB;
+    }
+    double* C = g.C + b * g.sC;
+    if (g.a_lower) kEnd = (m0 + BM < kEnd) ? (m0 + BM) : kEnd;
     const int64_t kTiles = (kEnd + BK - 1) / BK;
 
     acc_t acc[4][2];
@@ -223,13 +232,16 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
 
 template <int TA, int TB, typename T>
 static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
-    // Tile shapes: 128 x 160 when N is in (128, 160] (o = 160 of (H2O)32 in one
-    // tile), 256 x 64 for N <= 64 or a multiple of 64 that 128 does not divide,
-    // otherwise 128 x 128.
+    // Tile shape with the least padded area of C: 128 x 128, 128 x 160 (o = 160
+    // of (H2O)32 in one tile) or 256 x 64 (narrow N); ties keep that order.
+    const int BMs[3] = {128, 256, 128}, BNs[3] = {128, 64, 160};
     int cfg = 0;
-    if (a.N > 128 && a.N <= 160) cfg = 2;
-    else if ((a.N <= 64) || (a.N % 128 != 0 && a.N % 64 == 0)) cfg = 1;
-    const int BM = cfg == 1 ? 256 : 128, BN = cfg == 0 ? 128 : (cfg == 1 ? 64 : 160);
+    int64_t best = -1;
+    for (int c : {0, 2, 1}) {
+        const int64_t area = cdiv(a.M, BMs[c]) * BMs[c] * (cdiv(a.N, BNs[c]) * BNs[c]);
+        if (best < 0 || area < best) { best = area; cfg = c; }
+    }
+    const int BM = BMs[cfg], BN = BNs[cfg];
     const int nt = cfg == 2 ? 640 : 512;
     a.tilesM = cdiv(a.M, BM);
     const int64_t tiles = a.tilesM * cdiv(a.N, BN);
@@ -237,7 +249,13 @@ static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
     for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
         const int64_t nb = (batch - b0 < 65535) ? (batch - b0) : 65535;
         GemmArgs s = a;
-        s.A += b0 * a.sA; s.B += b0 * a.sB; s.C += b0 * a.sC;
+        if (a.ksplit > 0) {
+            s.A = a.A; s.B = a.B;   // chunk offsets are applied in the kernel from blockIdx.y
+            CORR_REQUIRE(ctx, batch <= 65535, "gemm: too many K chunks");
+        } else {
+            s.A += b0 * a.sA; s.B += b0 * a.sB;
+        }
+        s.C += b0 * a.sC;
         dim3 grid((unsigned)tiles, (unsigned)nb);
         KernelTimer timer(ctx, "gemm");
         if (cfg == 1)
@@ -251,21 +269,21 @@ static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
     return CORR_OK;
 }
 
-int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
-         double alpha, const double* A, int64_t lda, int64_t sA,
-         const double* B, int64_t ldb, int64_t sB, double beta, double* C,
-         int64_t ldc, int64_t sC, int64_t batch, int prec, bool a_lower) {
-    CORR_REQUIRE(ctx, M >= 0 && N >= 0 && K >= 0 && batch >= 0, "gemm: negative size");
-    if (M == 0 || N == 0 || batch == 0) return CORR_OK;
-    CORR_REQUIRE(ctx, A && B && C, "gemm: null operand");
-    CORR_REQUIRE(ctx, transa == 0 || transa == 1, "gemm: transa must be 0 or 1");
-    CORR_REQUIRE(ctx, transb == 0 || transb == 1, "gemm: transb must be 0 or 1");
-    CORR_REQUIRE(ctx, lda >= (transa ? M : K) && ldb >= (transb ? K : N) && ldc >= N,
-                 "gemm: leading dimension too small");
-    CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
-                 "gemm: unknown precision");
-    GemmArgs a{M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, alpha, beta, 0, a_lower ? 1 : 0};
-    const bool f64 = (prec == CORR_PREC_FP64);
+// C = alpha * sum_s P[s] + beta * C, chunks summed in order s = 0, 1, ...
+__global__ void splitk_reduce_kernel(const double* __restrict__ P, int64_t S, int64_t M, int64_t N,
+                                     double alpha, double beta, double* __restrict__ C, int64_t ldc) {
+    const int64_t MN = M * N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < MN; e += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int64_t c = 0; c < S; ++c) s += P[c * MN + e];
+        const int64_t m = e / N, n = e % N;
+        double v = alpha * s;
+        if (beta != 0.0) v += beta * C[m * ldc + n];
+        C[m * ldc + n] = v;
+    }
+}
+
+static int dispatch(corr_ctx* ctx, int transa, int transb, int f64, GemmArgs& a, int64_t batch) {
     const int sel = transa * 2 + transb;
     if (f64) {
         switch (sel) {
@@ -281,6 +299,47 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
         case 2: return launch_t<1, 0, float>(ctx, a, batch);
         default: return launch_t<1, 1, float>(ctx, a, batch);
     }
+}
+
+int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
+         double alpha, const double* A, int64_t lda, int64_t sA,
+         const double* B, int64_t ldb, int64_t sB, double beta, double* C,
+         int64_t ldc, int64_t sC, int64_t batch, int prec, bool a_lower) {
+    CORR_REQUIRE(ctx, M >= 0 && N >= 0 && K >= 0 && batch >= 0, "gemm: negative size");
+    if (M == 0 || N == 0 || batch == 0) return CORR_OK;
+    CORR_REQUIRE(ctx, A && B && C, "gemm: null operand");
+    CORR_REQUIRE(ctx, transa == 0 || transa == 1, "gemm: transa must be 0 or 1");
+    CORR_REQUIRE(ctx, transb == 0 || transb == 1, "gemm: transb must be 0 or 1");
+    CORR_REQUIRE(ctx, lda >= (transa ? M : K) && ldb >= (transb ? K : N) && ldc >= N,
+                 "gemm: leading dimension too small");
+    CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
+                 "gemm: unknown precision");
+    const int f64 = (prec == CORR_PREC_FP64) ? 1 : 0;
+    // Split K when the output has too few tiles to fill the GPU and K is deep:
+    // chunks of depth kc write partial products to a workspace, then a
+    // fixed-order sum applies alpha and beta (deterministic).
+    const int64_t tiles_est = cdiv(M, 128) * cdiv(N, 128) * batch;
+    if (batch == 1 && !a_lower && tiles_est < 128 && K >= 8192) {
+        int64_t S = cdiv(512, tiles_est);
+        if (S > K / 1024) S = K / 1024;
+        const int64_t cap = (int64_t)(256 << 20) / (8 * M * N);   // workspace at most 256 MiB
+        if (S > cap) S = cap;
+        if (S >= 2) {
+            const int64_t kc = rup(cdiv(K, S), 32);
+            S = cdiv(K, kc);
+            double* P = nullptr;
+            CORR_TRY(scratch(ctx, "gemm_splitk", sizeof(double) * S * M * N, (void**)&P));
+            GemmArgs a{M, N, K, A, lda, 0, B, ldb, 0, P, N, M * N, 1.0, 0.0, 0, 0, kc};
+            CORR_TRY(dispatch(ctx, transa, transb, f64, a, S));
+            const int64_t grid = std::min<int64_t>(cdiv(M * N, 256), 4096);
+            hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, P, S, M, N,
+                               alpha, beta, C, ldc);
+            CORR_LAUNCHED(ctx, "splitk_reduce_kernel");
+            return CORR_OK;
+        }
+    }
+    GemmArgs a{M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, alpha, beta, 0, a_lower ? 1 : 0, 0};
+    return dispatch(ctx, transa, transb, f64, a, batch);
 }
 
 // Deterministic reduction: one workgroup, fixed strided order, fixed tree.

@@ -99,29 +99,52 @@ __global__ __launch_bounds__(256) void rimp2_grad_pairs_kernel(GradArgs g) {
 // Fixed-order accumulation of the block's partials:
 // deps_vir[a] += sum_{pair, tb} pv ; deps_occ[i] += sum_{j, tiles} pij[.][0]
 // energy[0..1] += sum pij[.][1..2]
-__global__ void rimp2_grad_accum_kernel(const double* __restrict__ pv, const double* __restrict__ pij, int o,
-                                        int v, int bo, int i0, int nT, double* __restrict__ deo,
-                                        double* __restrict__ dev, double* __restrict__ energy) {
+// Workgroups 0..nvb-1 own 64 consecutive virtuals each (lanes = a, coalesced;
+// the 4 waves take interleaved partial rows and are summed in a fixed order).
+// The next bo workgroups own one occupied each, the last two one energy each
+// (strided partial sums, fixed LDS tree).  Reruns are bit-identical.
+__global__ __launch_bounds__(256) void rimp2_grad_accum_kernel(const double* __restrict__ pv,
+                                                               const double* __restrict__ pij, int o, int v,
+                                                               int bo, int i0, int nT, double* __restrict__ deo,
+                                                               double* __restrict__ dev,
+                                                               double* __restrict__ energy) {
+    __shared__ double sh[256];
     const int nt2 = nT * nT;
     const int64_t npair = (int64_t)bo * o;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < v) {
-        double s = 0.0;
-        for (int64_t p = 0; p < npair; ++p)
-            for (int tb = 0; tb < nT; ++tb) s += pv[(p * nT + tb) * (int64_t)v + t];
-        dev[t] += s;
-    } else if (t < v + bo) {
-        const int ii = t - v;
-        double s = 0.0;
-        for (int j = 0; j < o; ++j)
-            for (int k = 0; k < nt2; ++k) s += pij[(((int64_t)ii * o + j) * nt2 + k) * 3 + 0];
-        deo[i0 + ii] += s;
-    } else if (t < v + bo + 2) {
-        const int which = t - v - bo;   // 0: E_OS, 1: E_SS
-        double s = 0.0;
-        for (int64_t p = 0; p < npair; ++p)
-            for (int k = 0; k < nt2; ++k) s += pij[(p * nt2 + k) * 3 + 1 + which];
-        energy[which] += s;
+    const int nvb = (v + 63) / 64;
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    if ((int)blockIdx.x < nvb) {
+        const int lane = tid & 63, w = tid >> 6;
+        const int a = blockIdx.x * 64 + lane;
+        const int64_t n = npair * nT;
+        if (a < v)
+            for (int64_t e = w; e < n; e += 4) s += pv[e * v + a];
+        sh[tid] = s;
+        __syncthreads();
+        if (w == 0 && a < v) dev[a] += (sh[lane] + sh[64 + lane]) + (sh[128 + lane] + sh[192 + lane]);
+        return;
+    }
+    const int r = (int)blockIdx.x - nvb;   // < bo: deps_occ of occupied i0 + r; bo, bo+1: E_OS, E_SS
+    const double* src;
+    int64_t n;
+    if (r < bo) {
+        src = pij + (int64_t)r * o * nt2 * 3;
+        n = (int64_t)o * nt2;
+    } else {
+        src = pij + 1 + (r - bo);
+        n = npair * nt2;
+    }
+    for (int64_t e = tid; e < n; e += 256) s += src[e * 3];
+    sh[tid] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) sh[tid] += sh[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (r < bo) deo[i0 + r] += sh[0];
+        else energy[r - bo] += sh[0];
     }
 }
 
@@ -141,8 +164,8 @@ int rimp2_gradient(corr_ctx* ctx, int64_t naux, int64_t o, int64_t v, const doub
                  "rimp2_gradient: unknown precision");
     const int gp = prec == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_PREC_FP32;
     const int64_t ov = o * v;
-    // occupied block size: G and R slabs of at most ~2 GB each
-    int64_t bo = (int64_t)(2.0e9 / (8.0 * (double)v * (double)ov));
+    // occupied block size: G and R slabs of at most ~6 GB each
+    int64_t bo = (int64_t)(6.0e9 / (8.0 * (double)v * (double)ov));
     if (bo < 1) bo = 1;
     if (bo > o) bo = o;
     const int nT = (int)cdiv(v, RT);
@@ -170,9 +193,9 @@ int rimp2_gradient(corr_ctx* ctx, int64_t naux, int64_t o, int64_t v, const doub
         CORR_REQUIRE(ctx, blocks < ((int64_t)1 << 31), "rimp2_gradient: too many tiles");
         hipLaunchKernelGGL(rimp2_grad_pairs_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a);
         CORR_LAUNCHED(ctx, "rimp2_grad_pairs_kernel");
-        const int nthr = (int)(v + nb + 2);
-        hipLaunchKernelGGL(rimp2_grad_accum_kernel, dim3((unsigned)cdiv(nthr, 256)), dim3(256), 0, ctx->stream, pv,
-                           pij, (int)o, (int)v, (int)nb, (int)i0, nT, deo, dev, acc);
+        const int nacc = (int)(cdiv(v, 64) + nb + 2);
+        hipLaunchKernelGGL(rimp2_grad_accum_kernel, dim3((unsigned)nacc), dim3(256), 0, ctx->stream, pv, pij,
+                           (int)o, (int)v, (int)nb, (int)i0, nT, deo, dev, acc);
         CORR_LAUNCHED(ctx, "rimp2_grad_accum_kernel");
         // dB[:, I, :] = -4 B R_I^T
         CORR_TRY(gemm(ctx, 0, 1, naux, nb * v, ov, -4.0, B, ov, 0, R, ov, 0, 0.0, dB + i0 * v, ov, 0, 1, gp));

@@ -54,6 +54,24 @@ def test_gemm_fp32(corr, ta, tb, M, N, K):
     assert np.max(np.abs(out - ref)) < 1e-5 * np.sqrt(K) * 4
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gemm_split_k(corr, ta, tb, prec):
+    """Few output tiles and a deep K take the split-K path (chunks of K into a
+    workspace, fixed-order sum); the last chunk is ragged."""
+    M, N, K = 40, 152, 20000
+    _say(f"gemm split-K {prec} ta={ta} tb={tb}")
+    rng = np.random.default_rng(11 + 2 * ta + tb)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    ref = -0.5 * ((A.T if ta else A) @ (B.T if tb else B))
+    out = corr.dgemm(dev(A), dev(B), bool(ta), bool(tb), alpha=-0.5, prec=prec).cpu().numpy()
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    assert np.max(np.abs(out - ref)) < tol * np.sqrt(K) * 4
+    again = corr.dgemm(dev(A), dev(B), bool(ta), bool(tb), alpha=-0.5, prec=prec).cpu().numpy()
+    assert np.array_equal(out, again)   # deterministic
+
+
 @pytest.mark.parametrize("naux,o,v", [(76, 5, 19), (50, 1, 1), (33, 9, 40), (140, 13, 70), (304, 20, 76)])
 def test_rimp2_fp64_matches_oracle(corr, naux, o, v):
     _say(f"rimp2 fp64 naux={naux} o={o} v={v}")
