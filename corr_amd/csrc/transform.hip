@@ -51,6 +51,40 @@ __global__ void unfold_kernel(int64_t N, int64_t nrofint, int64_t s0, int64_t s1
     }
 }
 
+// Pair-symmetric form of the same unfold.  Every image that a permutation class
+// produces is a permutation of (a b | c d) that keeps the pairs {a,b}, {c,d}
+// and swaps or keeps them, so the dense tensor is D[p][q][r][s] =
+// V[pair(p,q)][pair(r,s)] with pair(x,y) = max(x,y)(max(x,y)+1)/2 + min(x,y) and
+// V the symmetric P x P pair matrix (P = N(N+1)/2).  Step 1 puts each list entry
+// at V[ab][cd] and V[cd][ab]; step 2 fills one dense row (p,q) per workgroup
+// with coalesced writes, reading row pair(p,q) of V (L2-resident).
+__device__ inline int64_t sym_pair(int64_t x, int64_t y) {
+    return x >= y ? x * (x + 1) / 2 + y : y * (y + 1) / 2 + x;
+}
+
+__global__ void pair_scatter_kernel(int64_t N, int64_t nrofint, const double* __restrict__ val,
+                                    const uint16_t* __restrict__ idx, double* __restrict__ V) {
+    const int64_t P = N * (N + 1) / 2;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < nrofint;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = idx[4 * x + 0], b = idx[4 * x + 1], c = idx[4 * x + 2], d = idx[4 * x + 3];
+        if (a >= N || b >= N || c >= N || d >= N) continue;
+        const int64_t u = sym_pair(a, b), w = sym_pair(c, d);
+        V[u * P + w] = val[x];
+        V[w * P + u] = val[x];
+    }
+}
+
+__global__ __launch_bounds__(256) void pair_expand_kernel(int64_t N, const double* __restrict__ V,
+                                                          double* __restrict__ dense) {
+    const int64_t P = N * (N + 1) / 2;
+    const int64_t row = blockIdx.x;   // p * N + q
+    const int64_t p = row / N, q = row % N;
+    const double* Vu = V + sym_pair(p, q) * P;
+    double* out = dense + row * N * N;
+    for (int64_t e = threadIdx.x; e < N * N; e += 256) out[e] = Vu[sym_pair(e / N, e % N)];
+}
+
 // (ia|jb) = prec_ints[((i*N + o+a)*N + j)*N + o+b]; one partial pair per block.
 __global__ __launch_bounds__(256) void canonical_mp2_kernel(const double* __restrict__ I4,
                                                             const double* __restrict__ FMo,
@@ -106,6 +140,24 @@ int ao_unfold(corr_ctx* ctx, int64_t N, int64_t nrofint, const int64_t* sc, cons
     CORR_REQUIRE(ctx, 0 <= sc[0] && sc[0] <= sc[1] && sc[1] <= sc[2] && sc[2] <= sc[3] && sc[3] <= nrofint,
                  "ao_unfold: sortcount must be non-decreasing and <= nrofint");
     const int64_t n4 = N * N * N * N;
+    const int64_t P = N * (N + 1) / 2;
+    // Pair-symmetric expansion while the P x P pair matrix is at most 32 GiB
+    // (N <= ~480), else the per-class image scatter.
+    if (P * P * 8 <= ((int64_t)32 << 30)) {
+        double* V = nullptr;
+        CORR_TRY(scratch(ctx, "unfold_pairs", sizeof(double) * P * P, (void**)&V));
+        hipLaunchKernelGGL(zero_kernel, dim3(grid_for(P * P)), dim3(256), 0, ctx->stream, V, P * P);
+        CORR_LAUNCHED(ctx, "zero_kernel");
+        if (nrofint > 0) {
+            hipLaunchKernelGGL(pair_scatter_kernel, dim3(grid_for(nrofint)), dim3(256), 0, ctx->stream, N, nrofint,
+                               val, idx, V);
+            CORR_LAUNCHED(ctx, "pair_scatter_kernel");
+        }
+        CORR_REQUIRE(ctx, N * N < ((int64_t)1 << 31), "ao_unfold: N too large");
+        hipLaunchKernelGGL(pair_expand_kernel, dim3((unsigned)(N * N)), dim3(256), 0, ctx->stream, N, V, dense);
+        CORR_LAUNCHED(ctx, "pair_expand_kernel");
+        return CORR_OK;
+    }
     hipLaunchKernelGGL(zero_kernel, dim3(grid_for(n4)), dim3(256), 0, ctx->stream, dense, n4);
     CORR_LAUNCHED(ctx, "zero_kernel");
     if (nrofint > 0) {
