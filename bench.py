@@ -194,7 +194,9 @@ def main() -> int:
         "fp64_tflops": round((W_E + W_B) / f64["s_per_step"] / 1e12, 3),
         "roofline": {"bound": "mfma", "kernel": "rimp2_energy_bf16x3", "achieved": round(achieved, 2),
                      "peak": 2500.0, "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4),
-                     "traffic": None,
+                     "traffic": recorded_traffic("rimp2_energy_bf16x3"),
+                     "traffic_source": "profiles/rimp2_energy_h2o32.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
+                                       "scaled by a 1 GiB copy)",
                      "work": "3 x W_E (three BF16 products per FP32 product)"},
         "roofline_fp64": {"bound": "mfma", "kernel": "rimp2_energy_real<double>",
                           "achieved": round(W_E / t_k64 / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
@@ -402,6 +404,20 @@ def run_suite(dev, seed: int) -> dict:
         return res
     rec("ccsd_h2o4", ccsd)
     return out
+
+
+def recorded_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` as recorded from the FETCH_SIZE and
+    WRITE_SIZE counter passes in profiles/rimp2_energy_h2o32.md (a line
+    'traffic_bytes_per_launch <kernel>: <bytes>'); None if not recorded."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "rimp2_energy_h2o32.md")
+    try:
+        with open(path) as fh:
+            m = re.search(r"traffic_bytes_per_launch " + re.escape(kernel) + r":\s*(\d+)", fh.read())
+    except OSError:
+        return None
+    return int(m.group(1)) if m else None
 
 
 def cpu_baseline(C, Pmn, J, o, eo, ev, W_E, budget_s):
