@@ -18,7 +18,7 @@
 //   A == B: x1/2 (T1 and T2 cover the same elements)
 //
 // B is repacked once into panels: panel (I, A) is a contiguous
-// [Kp][128] slab (FP64/FP32), or a [Kp/32][hi/lo][128][4][8] slab of BF16
+// [Kp][128] slab (FP64/FP32), or a [Kp/16][hi/lo][128][2][8] slab of BF16
 // halves (BF16x3) already in the energy kernel's LDS order, so every K-slab a
 // workgroup stages into LDS is one contiguous, fully coalesced read (for BF16x3
 // a direct global->LDS copy).
@@ -59,9 +59,10 @@ __device__ inline uint16_t f32_to_bf16_rn(float f) {
 }
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-// Slot of 16-byte k-chunk ch in a 64-byte LDS row r of the BF16 energy kernel
-// (an involution in ch): keeps its ds_read_b128 fragment reads conflict-free.
-__device__ inline int bf16_chunk(int row, int ch) { return ch ^ (((row >> 2) & 1) << 1); }
+// Slot of 16-byte k-chunk ch (0 or 1) in a 32-byte LDS row r of the BF16 energy
+// kernel (an involution in ch): keeps its ds_read_b128 fragment reads, 32 rows
+// per instruction, free of bank conflicts on the 64-bank LDS.
+__device__ inline int bf16_slot(int row, int ch) { return ch ^ ((row >> 3) & 1); }
 
 }  // namespace
 
@@ -83,8 +84,8 @@ __global__ void repack_real(const double* __restrict__ Bia, T* __restrict__ Bp, 
 }
 
 // One block per (panel, 32-row K slab): read [32 q][128 c] coalesced along a,
-// write [hi/lo][128 c][4 chunks][8 q] halves in the energy kernel's LDS order
-// (chunk slot s of row c holds k-chunk bf16_chunk(c, s)).
+// write two 16-deep stages, each [hi/lo][128 c][2 slots][8 q], in the energy
+// kernel's LDS order (slot s of row c holds k-chunk bf16_slot(c, s)).
 __global__ __launch_bounds__(256) void repack_bf16(const double* __restrict__ Bia,
                                                    uint16_t* __restrict__ Bp, int64_t naux, int o,
                                                    int v, int nAC, int64_t Kc) {
@@ -99,22 +100,24 @@ __global__ __launch_bounds__(256) void repack_bf16(const double* __restrict__ Bi
         S[q][c] = (Q < naux && i < o && a < v) ? Bia[(Q * o + i) * (int64_t)v + a] : 0.0;
     }
     __syncthreads();
+    // the two stages of this slab are consecutive: 2 x [2][PW][2][8] bf16
     uint16_t* out = Bp + (blk * Kc + kc) * (2 * PW * 32);
-    for (int e = threadIdx.x; e < 2 * PW * 4; e += 256) {
-        const int hl = e / (PW * 4), c = (e % (PW * 4)) / 4, kq = bf16_chunk(c, e % 4);
-        uint16_t h[8];
+    for (int e = threadIdx.x; e < 2 * 2 * PW * 2; e += 256) {
+        const int h = e / (2 * PW * 2), r0 = e % (2 * PW * 2);
+        const int hl = r0 / (PW * 2), c = (r0 % (PW * 2)) / 2, kq = bf16_slot(c, r0 % 2);
+        uint16_t hv[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const double x = S[kq * 8 + t][c];
+            const double x = S[h * 16 + kq * 8 + t][c];
             const uint16_t hi = f32_to_bf16_rn((float)x);
-            if (hl == 0) h[t] = hi;
-            else h[t] = f32_to_bf16_rn((float)(x - (double)bf16_to_f32(hi)));
+            if (hl == 0) hv[t] = hi;
+            else hv[t] = f32_to_bf16_rn((float)(x - (double)bf16_to_f32(hi)));
         }
         uint4 pk;
-        pk.x = h[0] | ((uint32_t)h[1] << 16);
-        pk.y = h[2] | ((uint32_t)h[3] << 16);
-        pk.z = h[4] | ((uint32_t)h[5] << 16);
-        pk.w = h[6] | ((uint32_t)h[7] << 16);
+        pk.x = hv[0] | ((uint32_t)hv[1] << 16);
+        pk.y = hv[2] | ((uint32_t)hv[3] << 16);
+        pk.z = hv[4] | ((uint32_t)hv[5] << 16);
+        pk.w = hv[6] | ((uint32_t)hv[7] << 16);
         *reinterpret_cast<uint4*>(out + (int64_t)e * 8) = pk;
     }
 }
@@ -301,173 +304,176 @@ __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_real(
 }
 
 // ---- BF16x3 kernel -------------------------------------------------------------
-// M ~= hi.hi + hi.lo + lo.hi with hi = bf16(x), lo = bf16(x - hi), FP32 accumulate.
-__device__ inline corr_f4 mma_bf16(const corr_u4& a, const corr_u4& b, corr_f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(corr_bf8, a),
+// M ~= hi.hi + hi.lo + lo.hi with hi = bf16(x), lo = bf16(x - hi), FP32 accumulate,
+// on v_mfma_f32_32x32x16_bf16: lane l holds A[l%32][8(l/32) + t] (t < 8), and
+// accumulator element r of lane l is C[8(r/4) + 4(l/32) + r%4][l%32].
+typedef float corr_f16v __attribute__((ext_vector_type(16)));
+
+__device__ inline corr_f16v mma_bf16_32(const corr_u4& a, const corr_u4& b, corr_f16v c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(corr_bf8, a),
                                                    __builtin_bit_cast(corr_bf8, b), c, 0, 0, 0);
 }
 
-__device__ inline corr_f4 mma3(const corr_u4& ah, const corr_u4& al, const corr_u4& bh, const corr_u4& bl,
-                               corr_f4 c) {
-    c = mma_bf16(al, bh, c);
-    c = mma_bf16(ah, bl, c);
-    return mma_bf16(ah, bh, c);
-}
-
-// K advances in 32-deep stages; a stage of one panel is [hi/lo][128 rows][32 k] bf16,
-// stored in the panel exactly as in LDS: a row is four 16-byte k-chunks, chunk ch
-// of row r in slot bf16_chunk(r, ch), which keeps the ds_read_b128 fragment reads
-// (16-lane groups) free of bank conflicts on the 64-bank LDS.  Stages move
-// global -> LDS with global_load_lds_dwordx4 (no VGPR round trip, no ds_write):
-// each wave copies one (panel, hi/lo) half, 8 contiguous 1 KiB segments.  Two
-// stage buffers (2 x 64 KiB, separate LDS objects) alternate: stage kc+2 is
-// copied into the buffer stage kc has just released while stage kc+1 computes.
+// K advances in 16-deep stages; a stage of one panel is [hi/lo][128 rows][16 k]
+// bf16 (8 KiB), stored in the panel exactly as in LDS: a row is two 16-byte
+// k-chunks, chunk ch of row r in slot bf16_slot(r, ch).  Stages move global ->
+// LDS with global_load_lds_dwordx4 (no VGPR round trip, no ds_write): each wave
+// copies one (panel, hi/lo) half, 4 contiguous 1 KiB segments.  Four stage
+// buffers (4 x 32 KiB, separate LDS objects) keep the copies three stages ahead
+// of the stage the matrix cores consume, with one barrier per stage.
 typedef __attribute__((address_space(3))) void* corr_lds_ptr;
 typedef __attribute__((address_space(1))) const void* corr_gbl_ptr;
 
-typedef uint16_t Bf16Stage[4][2][PW][32];   // [panel][hi/lo][row][k slot]
-
-struct T1Frag {
-    corr_u4 ah[4], al[4], bh[2], bl[2];
-};
-struct T2Frag {
-    corr_u4 ah[2], al[2], bh[2][2], bl[2][2];
-};
+typedef uint16_t Bf16Stage[4][2][PW][16];   // [panel][hi/lo][row][k slot]
 
 __device__ inline corr_u4 lds_frag(const Bf16Stage& S, int p, int hl, int row, int ko) {
     return *reinterpret_cast<const corr_u4*>(&S[p][hl][row][ko]);
 }
 
-// T1 = (I,A) x (J,B)^T: rows c1[rb] of panel 0, columns c2 + 16 y of panel 1.
-__device__ inline void read_t1(const Bf16Stage& S, const int (&c1)[4], int c2, int ko, T1Frag& f) {
+// Copies this wave's half-panel of stage st into S (4 x 1 KiB, one instruction each).
+__device__ inline void bf16_stage_copy(Bf16Stage& S, const uint16_t* Pw, int pw, int hw, int64_t st, int lane) {
+    constexpr int SUB = 2 * PW * 16;
+    const uint16_t* src = Pw + st * SUB + hw * (PW * 16) + lane * 8;
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) {
-        f.ah[rb] = lds_frag(S, 0, 0, c1[rb], ko);
-        f.al[rb] = lds_frag(S, 0, 1, c1[rb], ko);
-    }
-#pragma unroll
-    for (int y = 0; y < 2; ++y) {
-        f.bh[y] = lds_frag(S, 1, 0, c2 + y * 16, ko);
-        f.bl[y] = lds_frag(S, 1, 1, c2 + y * 16, ko);
-    }
-}
-// T2 = (J,A) x (I,B)^T: rows c2 + 16 y of panel 2, columns c4[x][y] of panel 3.
-__device__ inline void read_t2(const Bf16Stage& S, const int (&c4)[2][2], int c2, int ko, T2Frag& f) {
-#pragma unroll
-    for (int y = 0; y < 2; ++y) {
-        f.ah[y] = lds_frag(S, 2, 0, c2 + y * 16, ko);
-        f.al[y] = lds_frag(S, 2, 1, c2 + y * 16, ko);
-    }
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) {
-            f.bh[x][y] = lds_frag(S, 3, 0, c4[x][y], ko);
-            f.bl[x][y] = lds_frag(S, 3, 1, c4[x][y], ko);
-        }
-}
-__device__ inline void mma_t1(const T1Frag& f, corr_f4 (&acc)[4][2]) {
-#pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = mma3(f.ah[rb], f.al[rb], f.bh[cb], f.bl[cb], acc[rb][cb]);
-}
-__device__ inline void mma_t2(const T2Frag& f, corr_f4 (&acc)[4][2]) {
-#pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb)
-            acc[rb][cb] = mma3(f.ah[rb % 2], f.al[rb % 2], f.bh[rb / 2][cb], f.bl[rb / 2][cb], acc[rb][cb]);
-}
-
-// Copies this wave's half-panel of stage kc into S (8 x 1 KiB, one instruction each).
-__device__ inline void bf16_stage_copy(Bf16Stage& S, const uint16_t* Pw, int pw, int hw, int64_t kc,
-                                       int lane) {
-    constexpr int SUB = 2 * PW * 32;
-    const uint16_t* src = Pw + kc * SUB + hw * (PW * 32) + lane * 8;
-#pragma unroll
-    for (int s = 0; s < 8; ++s)
-        __builtin_amdgcn_global_load_lds((corr_gbl_ptr)(src + s * 512), (corr_lds_ptr)&S[pw][hw][16 * s][0],
+    for (int s = 0; s < 4; ++s)
+        __builtin_amdgcn_global_load_lds((corr_gbl_ptr)(src + s * 512), (corr_lds_ptr)&S[pw][hw][32 * s][0],
                                          16, 0, 0);
 }
 
-__device__ inline void bf16_stage_compute(const Bf16Stage& S, const int (&c1)[4], const int (&c4)[2][2], int c2,
-                                          int ko, corr_f4 (&acc1)[4][2], corr_f4 (&acc2)[4][2]) {
-    T1Frag f1;
-    T2Frag f2;
-    read_t1(S, c1, c2, ko, f1);
-    read_t2(S, c4, c2, ko, f2);
-    mma_t1(f1, acc1);
-    mma_t2(f2, acc2);
+// One stage of this wave's four 32 x 32 tiles: T1_x = rows ra[x] of panel 0 x
+// rows rb of panel 1, T2_x = rows rb of panel 2 x rows ra[x] of panel 3.
+__device__ inline void bf16_stage_compute(const Bf16Stage& S, const int (&ra)[2], int rb, int ko,
+                                          corr_f16v (&acc1)[2], corr_f16v (&acc2)[2]) {
+    corr_u4 ah[2], al[2], dh[2], dl[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        ah[x] = lds_frag(S, 0, 0, ra[x], ko);
+        al[x] = lds_frag(S, 0, 1, ra[x], ko);
+        dh[x] = lds_frag(S, 3, 0, ra[x], ko);
+        dl[x] = lds_frag(S, 3, 1, ra[x], ko);
+    }
+    const corr_u4 bh = lds_frag(S, 1, 0, rb, ko), bl = lds_frag(S, 1, 1, rb, ko);
+    const corr_u4 ch = lds_frag(S, 2, 0, rb, ko), cl = lds_frag(S, 2, 1, rb, ko);
+    // The three products of one accumulator are dependent: issue them term by
+    // term across the four accumulators, so dependent MFMAs are four apart.
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        acc1[x] = mma_bf16_32(al[x], bh, acc1[x]);
+        acc2[x] = mma_bf16_32(cl, dh[x], acc2[x]);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        acc1[x] = mma_bf16_32(ah[x], bl, acc1[x]);
+        acc2[x] = mma_bf16_32(ch, dl[x], acc2[x]);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        acc1[x] = mma_bf16_32(ah[x], bh, acc1[x]);
+        acc2[x] = mma_bf16_32(ch, dh[x], acc2[x]);
+    }
 }
 
 // Waits until this wave's outstanding global->LDS copies number at most N.
 #define CORR_WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt((N) | (0x7 << 4) | (0xF << 8))
+// Stage barrier: this wave's LDS reads are complete and at most N of its copies
+// are outstanding (s_waitcnt vmcnt(N) lgkmcnt(0)), then s_barrier.  Unlike
+// __syncthreads() it does not wait for the younger copies still in flight; the
+// empty asm statements keep the compiler from moving LDS accesses across it.
+#define CORR_STAGE_BARRIER(N)                                   \
+    {                                                           \
+        __builtin_amdgcn_s_waitcnt((N) | (0x7 << 4));           \
+        __asm__ volatile("" ::: "memory");                      \
+        __builtin_amdgcn_s_barrier();                           \
+        __asm__ volatile("" ::: "memory");                      \
+    }
 
+// Wave (wr, wc), wr in 0..1, wc in 0..3: T1_x = M^{ij}_ab for i = 2 wr + x,
+// j = wc (rows a, columns b, all 32 x 32 of the tile's virtual blocks), and
+// T2_x = B[:, (J,A) row j]^T B[:, (I,B) row i] = M^{ji}_ab = M^{ij}_ba, so the
+// exchange partner of every element sits in the same lane and register.
 __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_bf16x3(
     const uint16_t* __restrict__ Bp, const double* __restrict__ eo, const double* __restrict__ ev,
     int o, int v, int nIB, int nAC, int64_t Kc, int64_t nWG, int64_t per,
     double* __restrict__ partial) {
-    constexpr int SUB = 2 * PW * 32;            // bf16 per panel per stage (hi + lo)
+    constexpr int SUB = 2 * PW * 16;            // bf16 per panel per stage (hi + lo)
     __shared__ __attribute__((aligned(16))) Bf16Stage L0;
     __shared__ __attribute__((aligned(16))) Bf16Stage L1;
+    __shared__ __attribute__((aligned(16))) Bf16Stage L2;
+    __shared__ __attribute__((aligned(16))) Bf16Stage L3;
     __shared__ double sh[WG_THREADS / 64];
 
     PairTile t;
     int64_t id;
     if (!decode_tile(nIB, nAC, nWG, per, t, id)) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave >> 2, wc = wave & 3, cl = lane & 15, kq = lane >> 4;
+    const int wr = wave >> 2, wc = wave & 3, r32 = lane & 31, kh = lane >> 5;
 
     // Panels: 0 = (I,A), 1 = (J,B), 2 = (J,A), 3 = (I,B).  Wave w copies panel w/2, half w%2.
     const int pw = wave >> 1, hw = wave & 1;
-    const int ib = (pw == 0 || pw == 3) ? t.IB : t.JB;
-    const int ac = (pw == 0 || pw == 2) ? t.AC : t.BC;
-    const uint16_t* Pw = Bp + ((int64_t)ib * nAC + ac) * Kc * SUB;
+    const int ibw = (pw == 0 || pw == 3) ? t.IB : t.JB;
+    const int acw = (pw == 0 || pw == 2) ? t.AC : t.BC;
+    const uint16_t* Pw = Bp + ((int64_t)ibw * nAC + acw) * Kc * SUB;
 
-    corr_f4 acc1[4][2], acc2[4][2];
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) { acc1[x][y] = corr_f4{0, 0, 0, 0}; acc2[x][y] = corr_f4{0, 0, 0, 0}; }
+    const int ra[2] = {(2 * wr) * PA + r32, (2 * wr + 1) * PA + r32};
+    const int rb = wc * PA + r32;
+    // Rows read by this lane all share r % 32 = r32, so they share the chunk slot.
+    const int ko = bf16_slot(r32, kh) * 8;
 
-    int c1[4], c4[2][2];
-#pragma unroll
-    for (int rb = 0; rb < 4; ++rb) c1[rb] = (2 * wr + rb / 2) * PA + (rb % 2) * 16 + cl;
+    corr_f16v acc1[2], acc2[2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) c4[x][y] = (2 * wr + x) * PA + y * 16 + cl;
-    const int c2 = wc * PA + cl;
-    // Rows read by this lane all share r & 15 = cl, so they share the chunk slot.
-    const int ko = bf16_chunk(cl, kq) * 8;
+        for (int r = 0; r < 16; ++r) { acc1[x][r] = 0.0f; acc2[x][r] = 0.0f; }
 
-    // Each loop iteration starts with no copy in flight, so the compiler's own
-    // waits before LDS reads see only the copy into the other buffer.
+    // Stage s lives in buffer s % 4.  Before the barrier that opens stage s,
+    // every wave has finished reading stage s-1 and has landed its copy of
+    // stage s; after it, the buffer of stage s-1 takes the copy of stage s+3.
+    // Kc is a multiple of 4 (naux is padded to 64).
+    // Copies in flight at each barrier: this wave's 4 instructions per stage;
+    // stage s+1 must have landed while s+2 and s+3 may still run (vmcnt 8).
+    // The copy of each step is unconditional (no branch for the compiler's
+    // wait insertion to merge): the last three steps re-copy stage Kc-1 into
+    // the buffer just released, which is never read again.
     bf16_stage_copy(L0, Pw, pw, hw, 0, lane);
-    CORR_WAIT_VMCNT(0);
-    __syncthreads();
-    if (Kc > 1) bf16_stage_copy(L1, Pw, pw, hw, 1, lane);
-    bf16_stage_compute(L0, c1, c4, c2, ko, acc1, acc2);
-    CORR_WAIT_VMCNT(0);
-    __syncthreads();   // L0 released; stage 1 complete in L1
-    for (int64_t kc = 1; kc < Kc; kc += 2) {
-        if (kc + 1 < Kc) bf16_stage_copy(L0, Pw, pw, hw, kc + 1, lane);
-        bf16_stage_compute(L1, c1, c4, c2, ko, acc1, acc2);
-        CORR_WAIT_VMCNT(0);
-        __syncthreads();   // L1 released; stage kc+1 complete in L0
-        if (kc + 1 < Kc) {
-            if (kc + 2 < Kc) bf16_stage_copy(L1, Pw, pw, hw, kc + 2, lane);
-            bf16_stage_compute(L0, c1, c4, c2, ko, acc1, acc2);
-            CORR_WAIT_VMCNT(0);
-            __syncthreads();   // L0 released; stage kc+2 complete in L1
+    bf16_stage_copy(L1, Pw, pw, hw, Kc > 1 ? 1 : Kc - 1, lane);
+    bf16_stage_copy(L2, Pw, pw, hw, Kc > 2 ? 2 : Kc - 1, lane);
+    CORR_STAGE_BARRIER(8)
+#define CORR_STAGE(CUR, NEXT3, s_)                                                       \
+    {                                                                                    \
+        bf16_stage_copy(NEXT3, Pw, pw, hw, ((s_) + 3 < Kc) ? (s_) + 3 : Kc - 1, lane);   \
+        bf16_stage_compute(CUR, ra, rb, ko, acc1, acc2);                                 \
+        CORR_STAGE_BARRIER(8)                                                            \
+    }
+    for (int64_t s = 0; s < Kc; s += 4) {
+        CORR_STAGE(L0, L3, s)
+        CORR_STAGE(L1, L0, s + 1)
+        CORR_STAGE(L2, L1, s + 2)
+        CORR_STAGE(L3, L2, s + 3)
+    }
+#undef CORR_STAGE
+    CORR_WAIT_VMCNT(0);   // no copy outlives the loop
+
+    double sos = 0.0, sx = 0.0;
+    const int j = t.JB * PI + wc;
+    const int b = t.BC * PA + r32;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        const int i = t.IB * PI + 2 * wr + x;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int a = t.AC * PA + 8 * (r / 4) + 4 * kh + (r % 4);
+            if (i < o && j < o && a < v && b < v) {
+                const double m1 = (double)acc1[x][r], m2 = (double)acc2[x][r];
+                const double dinv = 1.0 / (ev[a] + ev[b] - eo[i] - eo[j]);
+                sos += (m1 * m1 + m2 * m2) * dinv;
+                sx += m1 * m2 * dinv;
+            }
         }
     }
-    double sos = 0.0, sx = 0.0;
-    energy_epilogue(acc1, acc2, t, o, v, eo, ev, sos, sx);
     write_partial(sos, sx, t, id, sh, partial);
 }
 #undef CORR_WAIT_VMCNT
+#undef CORR_STAGE_BARRIER
 
 // ---- host side -------------------------------------------------------------------
 struct RiLayout {
@@ -535,7 +541,7 @@ int rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const 
                            eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp, l.nWG, l.per, partial);
     else
         hipLaunchKernelGGL(rimp2_energy_bf16x3, grid, block, 0, ctx->stream, (const uint16_t*)Bp,
-                           eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp / 32, l.nWG, l.per,
+                           eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp / 16, l.nWG, l.per,
                            partial);
     CORR_LAUNCHED(ctx, "rimp2_energy");
     timer.stop();
