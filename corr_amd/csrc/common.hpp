@@ -17,6 +17,8 @@ struct corr_ctx {
     struct Buf { void* p = nullptr; size_t bytes = 0; };
     std::map<std::string, Buf> scratch;
     void* blas_handle = nullptr;   // rocBLAS handle used by the eigen metric
+    hipStream_t aux_stream = nullptr;   // second stream (default priority) for overlapped work
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool timing = false;
     struct Timed { std::string name; hipEvent_t start, stop; };
     std::vector<Timed> pending;                  // recorded, not yet read
@@ -61,6 +63,9 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
          double alpha, const double* A, int64_t lda, int64_t sA,
          const double* B, int64_t ldb, int64_t sB, double beta, double* C,
          int64_t ldc, int64_t sC, int64_t batch, int prec, bool a_lower = false);
+
+// J^-1/2 (EIGEN) or L^-1 of J = L L^T (CHOLESKY) into M (metric.hip).
+int ri_metric(corr_ctx* ctx, int64_t naux, const double* J, int mode, double* M, int64_t* nkept);
 
 // Deterministic sum of n doubles (device) into *h_out (host), n2 interleaved
 // series: data[i*nser + s].

@@ -84,6 +84,12 @@ int corr_ctx_destroy(corr_ctx* ctx) {
     for (auto& kv : ctx->scratch)
         if (kv.second.p) (void)hipFree(kv.second.p);
     corr_blas_release(ctx);
+    if (ctx->aux_stream) {
+        (void)hipStreamSynchronize(ctx->aux_stream);
+        (void)hipStreamDestroy(ctx->aux_stream);
+    }
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return CORR_OK;

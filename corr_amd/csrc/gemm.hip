@@ -328,7 +328,9 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
             const int64_t kc = rup(cdiv(K, S), 32);
             S = cdiv(K, kc);
             double* P = nullptr;
-            CORR_TRY(scratch(ctx, "gemm_splitk", sizeof(double) * S * M * N, (void**)&P));
+            // one workspace per stream: the auxiliary stream may run GEMMs concurrently
+            const char* ws = (ctx->aux_stream && ctx->stream == ctx->aux_stream) ? "gemm_splitk_aux" : "gemm_splitk";
+            CORR_TRY(scratch(ctx, ws, sizeof(double) * S * M * N, (void**)&P));
             GemmArgs a{M, N, K, A, lda, 0, B, ldb, 0, P, N, M * N, 1.0, 0.0, 0, 0, kc};
             CORR_TRY(dispatch(ctx, transa, transb, f64, a, S));
             const int64_t grid = std::min<int64_t>(cdiv(M * N, 256), 4096);

@@ -279,9 +279,54 @@ int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double*
     return transpose_vo(ctx, naux, v, o, Y, Bia);
 }
 
+// B from J and (P|mu nu) with the metric factorisation overlapped: the AO->MO
+// GEMMs are queued on the context stream first, the metric runs on the
+// context's auxiliary stream (ordered after the caller's prior work by an
+// event), and the metric GEMM waits for it by a second event.
+int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Pmn,
+                  const double* J, int mode, double* M, double* Bia, int prec, int64_t* nkept) {
+    CORR_REQUIRE(ctx, naux > 0 && N > 0 && o > 0 && o < N && C && Pmn && J && Bia,
+                 "ri_bia_from_j: bad arguments");
+    CORR_REQUIRE(ctx, mode == CORR_METRIC_EIGEN || mode == CORR_METRIC_CHOLESKY, "ri_bia_from_j: unknown metric mode");
+    CORR_REQUIRE(ctx, J != M, "ri_bia_from_j: J and M must not alias");
+    const int64_t v = N - o;
+    const int p = gemm_prec(prec);
+    double *X = nullptr, *Y = nullptr;
+    CORR_TRY(scratch(ctx, "ri_Xvo", sizeof(double) * naux * o * v, (void**)&X));
+    CORR_TRY(scratch(ctx, "ri_Yvo", sizeof(double) * naux * o * v, (void**)&Y));
+    if (!M) CORR_TRY(scratch(ctx, "ri_M", sizeof(double) * naux * naux, (void**)&M));
+    if (!ctx->aux_stream) CORR_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+    if (!ctx->ev_fork) CORR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    if (!ctx->ev_join) CORR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    hipStream_t main = ctx->stream;
+    CORR_HIP(ctx, hipEventRecord(ctx->ev_fork, main));
+    CORR_TRY(ao2vo(ctx, naux, N, o, C, Pmn, X, p));
+    CORR_HIP(ctx, hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
+    ctx->stream = ctx->aux_stream;
+    int rc = ri_metric(ctx, naux, J, mode, M, nkept);
+    if (rc == CORR_OK && hipEventRecord(ctx->ev_join, ctx->aux_stream) != hipSuccess)
+        rc = set_error(ctx, CORR_EHIP, "ri_bia_from_j: event record failed");
+    ctx->stream = main;
+    if (rc != CORR_OK) {
+        (void)hipStreamSynchronize(ctx->aux_stream);
+        return rc;
+    }
+    CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_join, 0));
+    CORR_TRY(gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Y, o * v, 0, 1, p,
+                  mode == CORR_METRIC_CHOLESKY));
+    return transpose_vo(ctx, naux, v, o, Y, Bia);
+}
+
 }  // namespace corr
 
 extern "C" {
+
+int corr_ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc, const double* d_MOs,
+                       const double* d_Pmn, const double* d_J, int metric_mode, double* d_M, double* d_Bia,
+                       int prec, int64_t* h_nkept) {
+    CORR_CHECK_CTX(ctx);
+    return corr::ri_bia_from_j(ctx, naux, nroao, nocc, d_MOs, d_Pmn, d_J, metric_mode, d_M, d_Bia, prec, h_nkept);
+}
 
 int corr_build_fmo(corr_ctx* ctx, int64_t nroao, const double* d_MOs, const double* d_Fmat,
                    double* d_FMo) {

@@ -136,6 +136,20 @@ def ri_form_bia(MOs: torch.Tensor, Pmn: torch.Tensor, M: torch.Tensor, nocc: int
     return Bia
 
 
+def ri_bia_from_j(MOs: torch.Tensor, Pmn: torch.Tensor, J: torch.Tensor, nocc: int,
+                  mode: str = "cholesky", prec: str | int = "fp64") -> tuple[torch.Tensor, int]:
+    """(Bia, kept) from J and (P|mu nu): the metric factorisation runs on a
+    second stream, overlapped with the AO->MO GEMMs (calculate_ri +
+    transform_ri arithmetic, ri.cc:50-69, 102-103, 147-172)."""
+    _dev(MOs, Pmn, J)
+    naux, N = Pmn.shape[0], Pmn.shape[1]
+    Bia = torch.empty(naux, nocc, N - nocc, dtype=torch.float64, device=Pmn.device)
+    kept = C.c_int64(0)
+    _ctx(Pmn).call("corr_ri_bia_from_j", naux, N, nocc, ptr(MOs), ptr(Pmn), ptr(J), _abi.METRICS[mode], None,
+                   ptr(Bia), _prec(prec), C.byref(kept))
+    return Bia, kept.value
+
+
 def rimp2_energy(Bia: torch.Tensor, eps_occ: torch.Tensor, eps_vir: torch.Tensor,
                  prec: str | int = "fp64") -> dict:
     """RI-MP2 energies (run_canonical_mp2_ri, can_mp2.cc:53-132)."""

@@ -153,6 +153,16 @@ int corr_transform_ri(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,
 int corr_ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,
                      const double* d_MOs, const double* d_Pmn, const double* d_M,
                      int metric_mode, double* d_Bia, int prec);
+/* corr_ri_bia_from_j: the arithmetic of calculate_ri + transform_ri
+ * (ri.cc:50-69, 102-103, 147-172) from J and (P|mu nu) in one call: the
+ * metric factorisation (corr_ri_metric) runs on a second stream of the
+ * context, overlapped with the AO->MO GEMMs of corr_ri_form_bia, and the
+ * metric GEMM waits for it.  d_M receives the metric factor (NULL: context
+ * scratch); h_nkept as in corr_ri_metric (may be NULL). */
+int corr_ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,
+                       const double* d_MOs, const double* d_Pmn, const double* d_J,
+                       int metric_mode, double* d_M, double* d_Bia, int prec,
+                       int64_t* h_nkept);
 
 /* ---- a9: RI-MP2 energy, fused pair-block GEMM + denominator + reduction ---
  * (can_mp2.cc:53-132).  eps_occ[i] = FMo[i,i], eps_vir[a] = FMo[o+a,o+a]. */

@@ -72,6 +72,26 @@ def test_gemm_split_k(corr, ta, tb, prec):
     assert np.array_equal(out, again)   # deterministic
 
 
+@pytest.mark.parametrize("mode", ["cholesky", "eigen"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_ri_bia_from_j_equals_two_calls(corr, mode, prec):
+    """The overlapped entry (metric on the second stream) gives exactly the
+    bits of ri_metric followed by ri_form_bia."""
+    naux, N, o = 152, 48, 10
+    _say(f"ri_bia_from_j {mode} {prec}")
+    rng = np.random.default_rng(77)
+    J = S.metric(naux, seed=78, cond=1e5)
+    Pmn = rng.standard_normal((naux, N, N))
+    Pmn = Pmn + Pmn.transpose(0, 2, 1)
+    C = S.orthonormal(N, seed=79)
+    dJ, dP, dC = dev(J), dev(Pmn), dev(C)
+    M, kept = corr.ri_metric(dJ, mode)
+    ref = corr.ri_form_bia(dC, dP, M, o, mode, prec).cpu().numpy()
+    got, kept2 = corr.ri_bia_from_j(dC, dP, dJ, o, mode, prec)
+    assert kept2 == kept
+    assert np.array_equal(got.cpu().numpy(), ref)
+
+
 @pytest.mark.parametrize("naux,o,v", [(76, 5, 19), (50, 1, 1), (33, 9, 40), (140, 13, 70), (304, 20, 76)])
 def test_rimp2_fp64_matches_oracle(corr, naux, o, v):
     _say(f"rimp2 fp64 naux={naux} o={o} v={v}")
