@@ -6,7 +6,7 @@
 // FP32 path: operands rounded to FP32 when staged into LDS,
 //            v_mfma_f32_16x16x4_f32, FP32 accumulate, epilogue in FP64.
 //
-// A workgroup is 8 or 10 wave64s; each wave owns a 64x32 block of C built from
+// A workgroup is 8, 10 or 12 wave64s; each wave owns a 64x32 block of C built from
 // 4x2 MFMA 16x16 tiles.  A K-slab is BK deep (32 for FP32, 16 for FP64); the
 // four k-rows of one MFMA step are spread over the slab as k = g * BK/4 + t
 // (lane group g = lane / 16, step t), so that a lane's k values within a slab
@@ -134,7 +134,7 @@ struct Opnd {
     }
 };
 
-// WM x WN waves (8 or 10, two or more per SIMD); each wave owns 64 x 32 of C
+// WM x WN waves (8, 10 or 12; two or more per SIMD); each wave owns 64 x 32 of C
 // (4 x 2 MFMA tiles).  BM = 64 WM, BN = 32 WN.
 template <int TA, int TB, typename T, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
@@ -232,17 +232,18 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
 
 template <int TA, int TB, typename T>
 static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
-    // Tile shape with the least padded area of C: 128 x 128, 128 x 160 (o = 160
-    // of (H2O)32 in one tile) or 256 x 64 (narrow N); ties keep that order.
-    const int BMs[3] = {128, 256, 128}, BNs[3] = {128, 64, 160};
+    // Tile shape with the least padded area of C: 128 x 128, 128 x 192 (N = 192
+    // of the (H2O)8 4-index quarters in one tile, so A is read once), 128 x 160
+    // (o = 160 of (H2O)32) or 256 x 64 (narrow N); ties keep that order.
+    const int BMs[4] = {128, 256, 128, 128}, BNs[4] = {128, 64, 160, 192};
     int cfg = 0;
     int64_t best = -1;
-    for (int c : {0, 2, 1}) {
+    for (int c : {0, 3, 2, 1}) {
         const int64_t area = cdiv(a.M, BMs[c]) * BMs[c] * (cdiv(a.N, BNs[c]) * BNs[c]);
         if (best < 0 || area < best) { best = area; cfg = c; }
     }
     const int BM = BMs[cfg], BN = BNs[cfg];
-    const int nt = cfg == 2 ? 640 : 512;
+    const int nt = cfg == 3 ? 768 : (cfg == 2 ? 640 : 512);
     a.tilesM = cdiv(a.M, BM);
     const int64_t tiles = a.tilesM * cdiv(a.N, BN);
     CORR_REQUIRE(ctx, tiles < (int64_t)1 << 31, "gemm: too many tiles");
@@ -260,6 +261,8 @@ static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
         KernelTimer timer(ctx, "gemm");
         if (cfg == 1)
             hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2>), grid, dim3(nt), 0, ctx->stream, s);
+        else if (cfg == 3)
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 6>), grid, dim3(nt), 0, ctx->stream, s);
         else if (cfg == 2)
             hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 5>), grid, dim3(nt), 0, ctx->stream, s);
         else
