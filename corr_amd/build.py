@@ -83,7 +83,7 @@ def _run(cmd: list[str], what: str) -> None:
 
 def build_dropin() -> None:
     """Host-only C++ drop-in layer (reference signatures) over libcorr_gpu.so,
-    and the driver that runs the reference's post-HF sequence through it."""
+    and the check program that runs the reference's post-HF sequence through it."""
     cxx = os.environ.get("CXX", "g++")
     src = PKG / "dropin"
     tmp = DROPIN_LIB.with_suffix(".so.tmp")

@@ -72,6 +72,34 @@ def test_gemm_split_k(corr, ta, tb, prec):
     assert np.array_equal(out, again)   # deterministic
 
 
+@pytest.mark.parametrize("entry", ["ri_metric", "ri_bia_from_j"])
+def test_cholesky_metric_not_positive_definite(corr, entry):
+    """A metric with a negative eigenvalue is refused with CORR_ENOTPD (the
+    factorisation flags a non-positive pivot); the context keeps working, also
+    after the failure happened on the second stream of ri_bia_from_j."""
+    from corr_amd._abi import CORR_ENOTPD, CorrError
+    naux, N, o = 64, 16, 4
+    _say(f"cholesky not-PD via {entry}")
+    rng = np.random.default_rng(91)
+    q, _ = np.linalg.qr(rng.standard_normal((naux, naux)))
+    lam = np.geomspace(1.0, 1e3, naux)
+    lam[naux // 2] = -0.5
+    J = (q * lam) @ q.T
+    J = 0.5 * (J + J.T)
+    with pytest.raises(CorrError) as err:
+        if entry == "ri_metric":
+            corr.ri_metric(dev(J), "cholesky")
+        else:
+            Pmn = rng.standard_normal((naux, N, N))
+            corr.ri_bia_from_j(dev(S.orthonormal(N, seed=92)), dev(Pmn + Pmn.transpose(0, 2, 1)), dev(J), o,
+                               "cholesky", "fp64")
+    assert err.value.status == CORR_ENOTPD
+    B = S.bia(40, 3, 9, seed=93)
+    eo, ev = S.orbital_energies(3, 9, seed=93)
+    got = corr.rimp2_energy(dev(B), dev(eo), dev(ev), "fp64")
+    assert abs(got["EMP2"] - O.rimp2(B, eo, ev)["EMP2"]) < 1e-12
+
+
 @pytest.mark.parametrize("mode", ["cholesky", "eigen"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_ri_bia_from_j_equals_two_calls(corr, mode, prec):
