@@ -61,6 +61,18 @@ class CCSDResult(C.Structure):
                 ("converged", C.c_int)]
 
 
+class SCFOpts(C.Structure):
+    _fields_ = [("maxiter", C.c_int), ("diis_size", C.c_int), ("core_guess_check", C.c_int), ("reserved", C.c_int),
+                ("diis_thresh", C.c_double), ("damp", C.c_double), ("e_tol", C.c_double)]
+
+
+class SCFResult(C.Structure):
+    _fields_ = [("e_scf", C.c_double), ("e_one", C.c_double), ("e_two", C.c_double), ("e_total", C.c_double),
+                ("last_de", C.c_double), ("last_error", C.c_double), ("iterations", C.c_int),
+                ("converged", C.c_int), ("diis", C.c_int), ("diis_start", C.c_int), ("diis_singular", C.c_int),
+                ("core_guess", C.c_int)]
+
+
 P = C.c_void_p
 I64 = C.c_int64
 I = C.c_int
@@ -97,6 +109,10 @@ SIGNATURES = {
     "corr_ccsd_ints_so": [P, I64, P, P],
     "corr_ccsd_update": [P, I64, I64, P, P, P, P, P, P, I],
     "corr_ccsd_energy": [P, I64, I64, P, P, P, P, C.POINTER(D)],
+    "corr_build_fmat": [P, I64, P, P, P, P],
+    "corr_build_fmat_list": [P, I64, I64, C.POINTER(I64), P, P, P, P, P],
+    "corr_build_pmat": [P, I64, I64, P, D, P],
+    "corr_run_scf": [P, I64, I64, P, P, P, P, P, P, P, D, C.POINTER(SCFOpts), P, C.POINTER(SCFResult)],
 }
 
 _lib = None

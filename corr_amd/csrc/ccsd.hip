@@ -439,7 +439,7 @@ int ccsd_energy(corr_ctx* ctx, int64_t o, int64_t v, const double* V, const doub
 }
 
 // Solves the (m+1) x (m+1) DIIS system by Gaussian elimination with partial pivoting.
-static bool diis_solve(std::vector<double> B, int m, std::vector<double>& c) {
+bool diis_solve(std::vector<double> B, int m, std::vector<double>& c) {
     const int k = m + 1;
     std::vector<double> rhs(k, 0.0);
     rhs[m] = -1.0;

@@ -67,8 +67,18 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
 // J^-1/2 (EIGEN) or L^-1 of J = L L^T (CHOLESKY) into M (metric.hip).
 int ri_metric(corr_ctx* ctx, int64_t naux, const double* J, int mode, double* M, int64_t* nkept);
 
-// Deterministic sum of n doubles (device) into *h_out (host), n2 interleaved
-// series: data[i*nser + s].
+// Eigen-decomposition of the symmetric n x n matrix A (device, in place):
+// row c of A becomes eigenvector c, lam (device) the eigenvalues ascending
+// (metric.hip).  Synchronises the stream.
+int sym_eig(corr_ctx* ctx, int64_t n, double* A, double* lam);
+
+// Solves the (m+1) x (m+1) DIIS system [B -1; -1 0] c = [0; -1] (row-major B
+// of order m+1 with the border already set) by Gaussian elimination with
+// partial pivoting; false when it is singular (ccsd.hip).
+bool diis_solve(std::vector<double> B, int m, std::vector<double>& c);
+
+// Deterministic sum of n doubles (device) into *h_out (host), nser (<= 16)
+// interleaved series: data[i*nser + s].
 int reduce_sum(corr_ctx* ctx, const double* d_data, int64_t n, int nser, double* h_out);
 
 }  // namespace corr

@@ -367,8 +367,8 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double* __restrict__ 
 
 int reduce_sum(corr_ctx* ctx, const double* d_data, int64_t n, int nser, double* h_out) {
     double* d_out = nullptr;
-    CORR_TRY(scratch(ctx, "reduce_out", sizeof(double) * 8, (void**)&d_out));
-    CORR_REQUIRE(ctx, nser >= 1 && nser <= 8, "reduce_sum: 1..8 series");
+    CORR_TRY(scratch(ctx, "reduce_out", sizeof(double) * 16, (void**)&d_out));
+    CORR_REQUIRE(ctx, nser >= 1 && nser <= 16, "reduce_sum: 1..16 series");
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, ctx->stream, d_data, n, nser, d_out);
     CORR_LAUNCHED(ctx, "reduce_kernel");
     CORR_HIP(ctx, hipMemcpyAsync(h_out, d_out, sizeof(double) * nser, hipMemcpyDeviceToHost, ctx->stream));

@@ -232,7 +232,7 @@ static int cholesky_inverse(corr_ctx* ctx, int64_t n, const double* J, double* M
     return CORR_OK;
 }
 
-static int blas_handle(corr_ctx* ctx, rocblas_handle* h) {
+int blas_handle(corr_ctx* ctx, rocblas_handle* h) {
     if (!ctx->blas_handle) {
         rocblas_handle hh = nullptr;
         if (rocblas_create_handle(&hh) != rocblas_status_success)
@@ -245,30 +245,39 @@ static int blas_handle(corr_ctx* ctx, rocblas_handle* h) {
     return CORR_OK;
 }
 
-static int eigen_invsqrt(corr_ctx* ctx, int64_t n, const double* J, double* M, int64_t* nkept) {
-    CORR_REQUIRE(ctx, n < (1 << 30), "ri metric: naux too large for dsyevd");
-    double *W = nullptr, *lam = nullptr, *E = nullptr;
+int sym_eig(corr_ctx* ctx, int64_t n, double* A, double* lam) {
+    CORR_REQUIRE(ctx, n > 0 && n < (1 << 30), "sym_eig: bad order");
+    double* E = nullptr;
     int* info = nullptr;
-    int64_t* kept = nullptr;
-    CORR_TRY(scratch(ctx, "eig_W", sizeof(double) * n * n, (void**)&W));
-    CORR_TRY(scratch(ctx, "eig_lam", sizeof(double) * n, (void**)&lam));
     CORR_TRY(scratch(ctx, "eig_E", sizeof(double) * n, (void**)&E));
     CORR_TRY(scratch(ctx, "eig_info", 64, (void**)&info));
-    kept = (int64_t*)(info + 2);
-    CORR_HIP(ctx, hipMemcpyAsync(W, J, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
     CORR_HIP(ctx, hipMemsetAsync(info, 0, 64, ctx->stream));
     rocblas_handle h;
     CORR_TRY(blas_handle(ctx, &h));
-    // J is symmetric, so its row-major buffer is also its column-major form.
-    // On return the buffer holds the eigenvectors as columns (column-major),
-    // i.e. row c of the row-major view is eigenvector c.
-    if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, W,
+    // A symmetric row-major buffer is also its column-major form; rocSOLVER
+    // reads the column-major upper triangle (the reference's dsyev 'U').  On
+    // return the buffer holds the eigenvectors as columns (column-major),
+    // i.e. row c of the row-major view is eigenvector c, eigenvalues ascending.
+    if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, A,
                          (rocblas_int)n, lam, E, (rocblas_int*)info) != rocblas_status_success)
         return set_error(ctx, CORR_EHIP, "rocsolver_dsyevd failed");
     int h_info = 0;
     CORR_HIP(ctx, hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (h_info != 0) return set_error(ctx, CORR_ENOTCONV, "rocsolver_dsyevd did not converge");
+    return CORR_OK;
+}
+
+static int eigen_invsqrt(corr_ctx* ctx, int64_t n, const double* J, double* M, int64_t* nkept) {
+    CORR_REQUIRE(ctx, n < (1 << 30), "ri metric: naux too large for dsyevd");
+    double *W = nullptr, *lam = nullptr;
+    int64_t* kept = nullptr;
+    CORR_TRY(scratch(ctx, "eig_W", sizeof(double) * n * n, (void**)&W));
+    CORR_TRY(scratch(ctx, "eig_lam", sizeof(double) * n, (void**)&lam));
+    CORR_TRY(scratch(ctx, "eig_kept", 64, (void**)&kept));
+    CORR_HIP(ctx, hipMemcpyAsync(W, J, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    CORR_HIP(ctx, hipMemsetAsync(kept, 0, 64, ctx->stream));
+    CORR_TRY(sym_eig(ctx, n, W, lam));
     // Wt = rows scaled copy; M = W^T diag(s) W
     double* Ws = nullptr;
     CORR_TRY(scratch(ctx, "eig_Ws", sizeof(double) * n * n, (void**)&Ws));

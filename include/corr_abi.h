@@ -27,6 +27,10 @@
  *   run_canonical_mp2_ri  src/POSTHF/can_mp2.cc:53-132
  *   ccsd_ur and helpers   src/POSTHF/ccsd.cc:6-580,
  *                         src/POSTHF/ccsd_intermediates.cc:6-287
+ *   build_Fmat            src/SCF/ops_rhf.cc:466-585
+ *   build_Pmat            src/SCF/scf.cc:44-61
+ *   run_scf (+ calc_S12,  src/SCF/scf.cc:105-320,
+ *   diag_Fmat, check_initial_guess)  src/SCF/ops_rhf.cc:15-37, 342-373, 425-430
  */
 #ifndef CORR_ABI_H
 #define CORR_ABI_H
@@ -231,6 +235,68 @@ int corr_ccsd_update(corr_ctx* ctx, int64_t nocc, int64_t nvir,
 int corr_ccsd_energy(corr_ctx* ctx, int64_t nocc, int64_t nvir,
                      const double* d_ints_so, const double* d_f,
                      const double* d_T1, const double* d_T2, double* h_e);
+
+/* ---- RHF input side (the step before the post-HF path) --------------------
+ * Fock build F = H + J(P) - 1/2 K(P), replacing build_Fmat
+ * (src/SCF/ops_rhf.cc:549-585 with the perm_* image rules at :466-547).
+ * corr_build_fmat reads the dense AO tensor of corr_ao_unfold (nroao <= 512;
+ * it relies on the tensor's (pq)<->(qp) and (rs)<->(sr) symmetry, which the
+ * unfold guarantees); corr_build_fmat_list takes the reference's own
+ * arguments (sorted unique list + sortcount) and unfolds into context
+ * scratch (nroao^4 doubles) on every call.  F must not alias P or H. */
+int corr_build_fmat(corr_ctx* ctx, int64_t nroao, const double* d_dense_ao,
+                    const double* d_Pmat, const double* d_Hmat,
+                    double* d_Fmat);
+int corr_build_fmat_list(corr_ctx* ctx, int64_t nroao, int64_t nrofint,
+                         const int64_t* h_sortcount, const double* d_intval,
+                         const uint16_t* d_intnums, const double* d_Pmat,
+                         const double* d_Hmat, double* d_Fmat);
+/* P = 2(1-damp) C_occ^T C_occ + damp P (in place), nocc = nroe/2
+ * (build_Pmat, src/SCF/scf.cc:44-61). */
+int corr_build_pmat(corr_ctx* ctx, int64_t nroao, int64_t nroe,
+                    const double* d_MOs, double damp, double* d_Pmat);
+
+/* Damped RHF with DIIS (run_scf, src/SCF/scf.cc:145-320), preceded by
+ * calc_S12 (ops_rhf.cc:342-373) and, when core_guess_check is set,
+ * check_initial_guess (ops_rhf.cc:15-37).  d_MOs holds the starting orbitals
+ * (row = MO) and receives the final ones; d_MOens the orbital energies of the
+ * last diagonalisation; d_Pmat and d_Fmat the density and Fock matrix rebuilt
+ * after the loop, as the reference leaves them for build_FMo.
+ * h_energies (optional, maxiter doubles) receives each iteration's ESCF
+ * (electronic + ion_rep).  Fails with CORR_ENOTPD when S is not positive
+ * definite.  Defaults of the reference: maxiter 100 (main.cc:114),
+ * diis_size 8, diis_thresh 1e-3, damp 0.5, e_tol 1e-8 (scf.cc:170-190, 207). */
+typedef struct corr_scf_opts {
+    int    maxiter;
+    int    diis_size;         /* 1..15                                        */
+    int    core_guess_check;  /* 1: core guess when |trace C| < 0.1           */
+    int    reserved;
+    double diis_thresh;       /* mean |S P F - F P S| that enables DIIS       */
+    double damp;              /* density damping until DIIS starts            */
+    double e_tol;             /* stop when |E - E_old| <= e_tol               */
+} corr_scf_opts;
+
+typedef struct corr_scf_result {
+    double e_scf;         /* last iteration's ESCF (electronic + ion_rep)     */
+    double e_one;         /* "ONE E:" line, tr(P H) + ion_rep                 */
+    double e_two;         /* "TWO E:" line, (tr(P F) - tr(P H)) / 2           */
+    double e_total;       /* "TOTAL E:" line, (tr(P H) + tr(P F))/2 + ion_rep */
+    double last_de;       /* last energy change                               */
+    double last_error;    /* last mean |S P F - F P S|                        */
+    int    iterations;
+    int    converged;     /* |last_de| <= e_tol                               */
+    int    diis;          /* DIIS extrapolation was used                      */
+    int    diis_start;    /* iteration at which it started                    */
+    int    diis_singular; /* extrapolations skipped: singular DIIS system     */
+    int    core_guess;    /* the core guess replaced the input orbitals       */
+} corr_scf_result;
+
+int corr_run_scf(corr_ctx* ctx, int64_t nroao, int64_t nroe,
+                 const double* d_Hmat, const double* d_Smat,
+                 const double* d_dense_ao, double* d_MOs, double* d_MOens,
+                 double* d_Pmat, double* d_Fmat, double ion_rep,
+                 const corr_scf_opts* opts, double* h_energies,
+                 corr_scf_result* h_out);
 
 #ifdef __cplusplus
 }

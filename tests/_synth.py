@@ -65,3 +65,25 @@ def model_hamiltonian(N, seed=0, eri_scale=0.05):
     h = 0.5 * (h + h.T)
     eri = eri_scale * eri_like(N, seed=seed + 1) * N
     return h, eri
+
+
+def nonorthogonal_system(N, seed=0, spread=0.15, eri_scale=0.05, nocc=None):
+    """(H, S, eri, h, eri_orth, X): the model Hamiltonian of an orthonormal
+    basis phi re-expressed in the non-orthogonal basis chi_mu = sum_k phi_k X[k, mu]:
+    S = X^T X, H = X^T h X, (mu nu|la si) = X^4 applied to (kl|mn).
+    With nocc, h has a gap of 1.5 Eh after its nocc lowest levels, so that the
+    damped SCF converges in a dozen iterations."""
+    if nocc is None:
+        h, eri = model_hamiltonian(N, seed=seed, eri_scale=eri_scale)
+    else:
+        rng = np.random.default_rng(seed)
+        d = np.concatenate([np.linspace(-2.0, -1.0, nocc), np.linspace(0.5, 2.5, N - nocc)])
+        h = np.diag(d) + 0.05 * rng.standard_normal((N, N))
+        h = 0.5 * (h + h.T)
+        eri = eri_scale * eri_like(N, seed=seed + 1) * N
+    rng = np.random.default_rng(seed + 7)
+    X = np.eye(N) + spread * rng.standard_normal((N, N))
+    S = X.T @ X
+    H = X.T @ h @ X
+    eri_ao = np.einsum("km,ln,ar,bs,klab->mnrs", X, X, X, X, eri, optimize=True)
+    return H, S, eri_ao, h, eri, X
