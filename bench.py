@@ -20,8 +20,10 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--suite]
 
 --suite adds a "suite" object with the other rows of the hot path, each at its
 own shape: build_FMo (N=768), AO unfold + 4-index transform + canonical MP2
-((H2O)8, N=192), metric and B formation ((H2O)32), RI-MP2 gradient ((H2O)16)
-and CCSD per iteration ((H2O)4).
+((H2O)8, N=192), metric and B formation ((H2O)32), RI-MP2 gradient ((H2O)16),
+CCSD per iteration ((H2O)4) and the RHF input side (Fock build and one
+RHF+DIIS iteration at (H2O)8).  --rows NAME,... restricts the suite to the
+named rows.
 """
 from __future__ import annotations
 
@@ -104,6 +106,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (a known-size kernel for scaling HBM counters)")
+    ap.add_argument("--rows", default="", help="comma-separated suite rows to run (default: all)")
     ap.add_argument("--suite", action="store_true",
                     help="also time every other row of the hot path at its own shape (adds a 'suite' key)")
     args = ap.parse_args()
@@ -207,7 +210,7 @@ def main() -> int:
     if args.suite:
         del C, J, Pmn
         torch.cuda.empty_cache()
-        line["suite"] = run_suite(dev, args.seed)
+        line["suite"] = run_suite(dev, args.seed, [r for r in args.rows.split(",") if r])
     print(json.dumps(line), flush=True)
     return 0
 
@@ -286,12 +289,14 @@ def unique_list_gpu(N: int, device, seed: int):
     return val, nums.contiguous(), sortcount
 
 
-def run_suite(dev, seed: int) -> dict:
+def run_suite(dev, seed: int, rows=()) -> dict:
     import torch
     import corr_amd
     out = {"peaks_TFLOPs": {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}}
 
     def rec(name, fn):
+        if rows and name not in rows:
+            return
         try:
             out[name] = fn()
             log(f"suite {name}: {out[name]}")
@@ -406,6 +411,40 @@ def run_suite(dev, seed: int) -> dict:
         res["e_ccsd_iter20"] = r["e_ccsd"]
         return res
     rec("ccsd_h2o4", ccsd)
+
+    # RHF input side at (H2O)8, N = 192, 80 electrons: Fock build from the
+    # dense tensor (HBM-bound: slabs p >= q read once) and one RHF+DIIS iteration
+    def rhf():
+        from corr_amd._abi import Context
+        N, nroe = 192, 80
+        val, nums, sc = unique_list_gpu(N, dev, seed)
+        dense = corr_amd.ao_unfold(N, val, nums, sc)
+        del val, nums
+        torch.cuda.empty_cache()
+        h = torch.diag(torch.cat([torch.linspace(-2.0, -1.0, nroe // 2), torch.linspace(0.5, 2.5, N - nroe // 2)]))
+        H = h.to(dtype=torch.float64, device=dev).contiguous()
+        S = torch.eye(N, dtype=torch.float64, device=dev)
+        P = corr_amd.build_Pmat(random_orthogonal(N, rng, dev), nroe)
+        ctx = Context.get(dev.index if dev.index is not None else torch.cuda.current_device())
+        ctx.enable_timing(True)
+        ctx.kernel_time(None)
+        t = _timed(lambda: corr_amd.build_Fmat(dense, P, H), reps=5)
+        ms, n = ctx.kernel_time("fock_slab_kernel")
+        ctx.enable_timing(False)
+        slab_bytes = N * (N + 1) // 2 * N * N * 8
+        res = {"shape": "(H2O)8 N=192 nroe=80", "fock_ms": round(t * 1e3, 3),
+               "fock_slab_kernel_ms": round(ms / max(n, 1), 3),
+               "fock_slab_bytes_GB": round(slab_bytes / 1e9, 3),
+               "fock_slab_GBs": round(slab_bytes / (ms / max(n, 1) * 1e-3) / 1e9, 1)}
+        C0 = random_orthogonal(N, rng, dev)
+        t1 = _timed(lambda: corr_amd.run_scf(H, S, dense, C0, nroe, maxiter=1, e_tol=0.0,
+                                             core_guess_check=False), reps=2)
+        t9 = _timed(lambda: corr_amd.run_scf(H, S, dense, C0, nroe, maxiter=9, e_tol=0.0,
+                                             core_guess_check=False), reps=2)
+        res["rhf_iteration_ms"] = round((t9 - t1) / 8 * 1e3, 2)
+        res["rhf_setup_ms"] = round((t1 - (t9 - t1) / 8) * 1e3, 2)
+        return res
+    rec("rhf_h2o8", rhf)
     return out
 
 

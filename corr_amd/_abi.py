@@ -112,7 +112,7 @@ SIGNATURES = {
     "corr_build_fmat": [P, I64, P, P, P, P],
     "corr_build_fmat_list": [P, I64, I64, C.POINTER(I64), P, P, P, P, P],
     "corr_build_pmat": [P, I64, I64, P, D, P],
-    "corr_run_scf": [P, I64, I64, P, P, P, P, P, P, P, D, C.POINTER(SCFOpts), P, C.POINTER(SCFResult)],
+    "corr_run_scf": [P, I64, I64, P, P, P, P, P, P, P, D, C.POINTER(SCFOpts), P, P, C.POINTER(SCFResult)],
 }
 
 _lib = None

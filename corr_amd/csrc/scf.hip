@@ -52,17 +52,20 @@ __device__ inline double wave_sum(double v) {
 }
 
 // One workgroup per slab (p, q), p >= q.  Wave w streams rows r = w, w + 4, ...
-// of the slab (coalesced N-long rows); lane l owns columns t = l + 64 j and
-// keeps the two K column sums and its share of the J sum in registers, so no
-// cross-lane reduction happens inside the row loop.  RU rows are loaded before
-// any of them is used, to keep RU * NJ loads in flight per lane.
-template <int NJ, int RU>
+// of the slab (coalesced N-long rows); lane l owns the VEC adjacent columns
+// t = VEC l + 64 VEC j + v and keeps the two K column sums and its share of the
+// J sum in registers, so no cross-lane reduction happens inside the row loop.
+// VEC = 2 (16-byte loads, N even) halves the load instructions against VEC = 1.
+// RU rows are loaded before any of them is used.  Out-of-range rows and
+// columns load element 0 of the slab (in bounds) and are masked to zero.
+template <int NJ, int RU, int VEC>
 __global__ __launch_bounds__(FOCK_THREADS) void fock_slab_kernel(const double* __restrict__ D,
                                                                  const double* __restrict__ P, int64_t N,
                                                                  double* __restrict__ J,
                                                                  double* __restrict__ Kpart) {
-    __shared__ double colq[FOCK_WAVES][NJ * 64];
-    __shared__ double colp[FOCK_WAVES][NJ * 64];
+    typedef double vd __attribute__((ext_vector_type(VEC)));
+    __shared__ double colq[FOCK_WAVES][NJ * 64 * VEC];
+    __shared__ double colp[FOCK_WAVES][NJ * 64 * VEC];
     __shared__ double jw[FOCK_WAVES];
     const int64_t b = blockIdx.x;
     int64_t p = (int64_t)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
@@ -74,50 +77,56 @@ __global__ __launch_bounds__(FOCK_THREADS) void fock_slab_kernel(const double* _
     const double* Pp = P + p * N;
     const double* Pq = P + q * N;
 
-    double kq[NJ], kp[NJ];
+    double kq[NJ][VEC], kp[NJ][VEC];
     bool on[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
-        kq[j] = 0.0;
-        kp[j] = 0.0;
-        on[j] = lane + 64 * j < N;
+        on[j] = VEC * lane + 64 * VEC * j < N;
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) kq[j][v] = kp[j][v] = 0.0;
     }
     double jacc = 0.0;
     for (int64_t r0 = w; r0 < N; r0 += FOCK_WAVES * RU) {
-        double d[RU][NJ], pr[RU][NJ], wq[RU], wp[RU];
+        vd d[RU][NJ], pr[RU][NJ];
+        double wq[RU], wp[RU];
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
-            // out-of-range rows and columns read element 0 (in bounds) and are masked
             const int64_t r = r0 + FOCK_WAVES * u;
             const bool rv = r < N;
             const int64_t rc = rv ? r : 0;
-            const double* row = slab + rc * N;
-            const double* Prow = P + rc * N;
-            wq[u] = rv ? Pq[rc] : 0.0;
-            wp[u] = rv ? Pp[rc] : 0.0;
+            const double a = Pq[rc], c = Pp[rc];
+            wq[u] = rv ? a : 0.0;
+            wp[u] = rv ? c : 0.0;
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 const bool ok = rv && on[j];
-                const int64_t c = ok ? lane + 64 * j : 0;
-                const double dv = row[c], pv = Prow[c];
-                d[u][j] = ok ? dv : 0.0;
-                pr[u][j] = ok ? pv : 0.0;
+                const int64_t col = ok ? VEC * lane + 64 * VEC * j : 0;
+                const vd dv = *(const vd*)(slab + rc * N + col);
+                const vd pv = *(const vd*)(P + rc * N + col);
+                d[u][j] = ok ? dv : (vd)(0.0);
+                pr[u][j] = ok ? pv : (vd)(0.0);
             }
         }
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
-                jacc = fma(d[u][j], pr[u][j], jacc);
-                kq[j] = fma(d[u][j], wq[u], kq[j]);
-                kp[j] = fma(d[u][j], wp[u], kp[j]);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    jacc = fma(d[u][j][v], pr[u][j][v], jacc);
+                    kq[j][v] = fma(d[u][j][v], wq[u], kq[j][v]);
+                    kp[j][v] = fma(d[u][j][v], wp[u], kp[j][v]);
+                }
             }
         }
     }
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
-        colq[w][lane + 64 * j] = kq[j];
-        colp[w][lane + 64 * j] = kp[j];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            colq[w][VEC * lane + 64 * VEC * j + v] = kq[j][v];
+            colp[w][VEC * lane + 64 * VEC * j + v] = kp[j][v];
+        }
     }
     jacc = wave_sum(jacc);
     if (lane == 0) jw[w] = jacc;
@@ -234,13 +243,13 @@ __global__ __launch_bounds__(256) void trace_kernel(const double* __restrict__ C
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
-template <int NJ>
+template <int NJ, int VEC>
 int launch_slab(corr_ctx* ctx, const double* D, const double* P, int64_t N, double* J, double* Kpart) {
-    constexpr int RU = NJ <= 2 ? 8 : (NJ <= 4 ? 4 : 2);
+    constexpr int RU = NJ * VEC <= 2 ? 8 : (NJ * VEC <= 4 ? 4 : 2);
     const int64_t pairs = N * (N + 1) / 2;
     KernelTimer tm(ctx, "fock_slab_kernel");
-    hipLaunchKernelGGL((fock_slab_kernel<NJ, RU>), dim3((unsigned)pairs), dim3(FOCK_THREADS), 0, ctx->stream, D,
-                       P, N, J, Kpart);
+    hipLaunchKernelGGL((fock_slab_kernel<NJ, RU, VEC>), dim3((unsigned)pairs), dim3(FOCK_THREADS), 0, ctx->stream,
+                       D, P, N, J, Kpart);
     CORR_LAUNCHED(ctx, "fock_slab_kernel");
     return CORR_OK;
 }
@@ -264,15 +273,26 @@ int build_fmat(corr_ctx* ctx, int64_t N, const double* D, const double* P, const
     double *J = nullptr, *Kpart = nullptr;
     CORR_TRY(scratch(ctx, "fock_J", sizeof(double) * N * N, (void**)&J));
     CORR_TRY(scratch(ctx, "fock_Kpart", sizeof(double) * N * N * N, (void**)&Kpart));
-    switch ((int)cdiv(N, 64)) {
-        case 1: CORR_TRY(launch_slab<1>(ctx, D, P, N, J, Kpart)); break;
-        case 2: CORR_TRY(launch_slab<2>(ctx, D, P, N, J, Kpart)); break;
-        case 3: CORR_TRY(launch_slab<3>(ctx, D, P, N, J, Kpart)); break;
-        case 4: CORR_TRY(launch_slab<4>(ctx, D, P, N, J, Kpart)); break;
-        case 5: CORR_TRY(launch_slab<5>(ctx, D, P, N, J, Kpart)); break;
-        case 6: CORR_TRY(launch_slab<6>(ctx, D, P, N, J, Kpart)); break;
-        case 7: CORR_TRY(launch_slab<7>(ctx, D, P, N, J, Kpart)); break;
-        default: CORR_TRY(launch_slab<8>(ctx, D, P, N, J, Kpart)); break;
+    // 16-byte loads need every row start 16-byte aligned: N even (the slabs
+    // start at multiples of N^2 doubles and allocations are 256-byte aligned).
+    if (N % 2 == 0 && ((uintptr_t)D % 16) == 0 && ((uintptr_t)P % 16) == 0) {
+        switch ((int)cdiv(N, 128)) {
+            case 1: CORR_TRY((launch_slab<1, 2>(ctx, D, P, N, J, Kpart))); break;
+            case 2: CORR_TRY((launch_slab<2, 2>(ctx, D, P, N, J, Kpart))); break;
+            case 3: CORR_TRY((launch_slab<3, 2>(ctx, D, P, N, J, Kpart))); break;
+            default: CORR_TRY((launch_slab<4, 2>(ctx, D, P, N, J, Kpart))); break;
+        }
+    } else {
+        switch ((int)cdiv(N, 64)) {
+            case 1: CORR_TRY((launch_slab<1, 1>(ctx, D, P, N, J, Kpart))); break;
+            case 2: CORR_TRY((launch_slab<2, 1>(ctx, D, P, N, J, Kpart))); break;
+            case 3: CORR_TRY((launch_slab<3, 1>(ctx, D, P, N, J, Kpart))); break;
+            case 4: CORR_TRY((launch_slab<4, 1>(ctx, D, P, N, J, Kpart))); break;
+            case 5: CORR_TRY((launch_slab<5, 1>(ctx, D, P, N, J, Kpart))); break;
+            case 6: CORR_TRY((launch_slab<6, 1>(ctx, D, P, N, J, Kpart))); break;
+            case 7: CORR_TRY((launch_slab<7, 1>(ctx, D, P, N, J, Kpart))); break;
+            default: CORR_TRY((launch_slab<8, 1>(ctx, D, P, N, J, Kpart))); break;
+        }
     }
     hipLaunchKernelGGL(fock_combine_kernel, dim3((unsigned)cdiv(N, 256), (unsigned)N), dim3(256), 0, ctx->stream, H,
                        J, Kpart, N, F);
@@ -332,7 +352,7 @@ int diag_fmat(corr_ctx* ctx, const ScfWork& w, const double* F, double* C, doubl
 
 int run_scf(corr_ctx* ctx, int64_t N, int64_t nroe, const double* H, const double* S, const double* D, double* C,
             double* MOens, double* P, double* F, double ion_rep, const corr_scf_opts* opts, double* h_energies,
-            corr_scf_result* res) {
+            double* h_errors, corr_scf_result* res) {
     CORR_REQUIRE(ctx, H && S && D && C && MOens && P && F && opts && res, "run_scf: null pointer");
     CORR_REQUIRE(ctx, N > 0 && N <= 64 * FOCK_MAX_NJ, "run_scf: nroao must be in 1..512");
     CORR_REQUIRE(ctx, nroe > 0 && nroe % 2 == 0 && nroe / 2 <= N, "run_scf: nroe must be even and <= 2 nroao");
@@ -443,6 +463,7 @@ int run_scf(corr_ctx* ctx, int64_t N, int64_t nroe, const double* H, const doubl
         DE = Escf - Eold;
         Eold = Escf;
         if (h_energies) h_energies[it] = Escf + ion_rep;
+        if (h_errors) h_errors[it] = error;
         CORR_TRY(diag_fmat(ctx, w, F, C, MOens));
         ++it;
     }
@@ -492,10 +513,11 @@ int corr_build_pmat(corr_ctx* ctx, int64_t nroao, int64_t nroe, const double* d_
 
 int corr_run_scf(corr_ctx* ctx, int64_t nroao, int64_t nroe, const double* d_Hmat, const double* d_Smat,
                  const double* d_dense_ao, double* d_MOs, double* d_MOens, double* d_Pmat, double* d_Fmat,
-                 double ion_rep, const corr_scf_opts* opts, double* h_energies, corr_scf_result* res) {
+                 double ion_rep, const corr_scf_opts* opts, double* h_energies, double* h_errors,
+                 corr_scf_result* res) {
     CORR_CHECK_CTX(ctx);
     return corr::run_scf(ctx, nroao, nroe, d_Hmat, d_Smat, d_dense_ao, d_MOs, d_MOens, d_Pmat, d_Fmat, ion_rep,
-                         opts, h_energies, res);
+                         opts, h_energies, h_errors, res);
 }
 
 }  // extern "C"

@@ -1,5 +1,5 @@
-// Drop-in replacements for the reference's post-HF entry points (the C++
-// signatures of src/POSTHF/{motrans,can_mp2,ri,ccsd}.h).  Host code only: all
+// Drop-in replacements for the reference's post-HF and SCF entry points (the
+// C++ signatures of src/POSTHF/{motrans,can_mp2,ri,ccsd}.h and src/SCF).  Host code only: all
 // arithmetic is done by libcorr_gpu.so through the C-ABI.  Ownership follows
 // the reference (callee new[] for FMo, prec_ints, ints_so; caller-allocated
 // BPQ and Bia), and results are printed with the reference's labels and
@@ -12,6 +12,7 @@
 #include "../../include/corr_dropin.h"
 #include "../../include/corr_abi.h"
 
+#include <chrono>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -195,6 +196,77 @@ void read_transform_ri(std::string prefix, int nroe, int nroao, int naux_2, doub
     }
     DBuf dBPQ(BPQ.data(), sizeof(double) * count);
     transform_ri_device(naux_2, nroao, nroe, MOs, dBPQ.d(), Bia);
+}
+
+// ops_rhf.cc:549-585
+void build_Fmat(int nroao, double* Fmat, double* Pmat, double* Hmat, double* intval, unsigned short* intnums,
+                long long int* sortcount, long long int nrofint) {
+    const int64_t N = nroao, nint = nrofint;
+    DBuf dval(intval, sizeof(double) * (nint > 0 ? nint : 1));
+    DBuf dnum(intnums, sizeof(unsigned short) * 4 * (nint > 0 ? nint : 1));
+    DBuf dP(Pmat, sizeof(double) * N * N), dH(Hmat, sizeof(double) * N * N), dF(sizeof(double) * N * N);
+    const int64_t sc[4] = {sortcount[0], sortcount[1], sortcount[2], sortcount[3]};
+    check("corr_build_fmat_list", corr_build_fmat_list(ctx(), N, nint, sc, dval.d(), (const uint16_t*)dnum.p, dP.d(),
+                                                       dH.d(), dF.d()));
+    dF.to_host(Fmat, sizeof(double) * N * N);
+}
+
+// scf.cc:44-61
+void build_Pmat(double* C, int nroao, int nroe, double* Pmat, double damp) {
+    const int64_t N = nroao;
+    DBuf dC(C, sizeof(double) * N * N), dP(Pmat, sizeof(double) * N * N);
+    check("corr_build_pmat", corr_build_pmat(ctx(), N, nroe, dC.d(), damp, dP.d()));
+    dP.to_host(Pmat, sizeof(double) * N * N);
+}
+
+// scf.cc:145-320.  The iteration lines are printed after the GPU loop ends;
+// t[s] is the mean wall time of one iteration.
+void run_scf(systeminfo* sysinfo, OEints* onemats, TEints* twomats) {
+    std::cout << "\nSCF:\n----" << '\n';
+    const int64_t N = sysinfo->nroao, n2 = N * N, n4 = n2 * n2, nint = sysinfo->nrofint;
+    DBuf dval(twomats->intval, sizeof(double) * (nint > 0 ? nint : 1));
+    DBuf dnum(twomats->intnums, sizeof(unsigned short) * 4 * (nint > 0 ? nint : 1));
+    DBuf dense(sizeof(double) * n4);
+    const int64_t sc[4] = {twomats->sortcount[0], twomats->sortcount[1], twomats->sortcount[2],
+                           twomats->sortcount[3]};
+    check("corr_ao_unfold", corr_ao_unfold(ctx(), N, nint, sc, dval.d(), (const uint16_t*)dnum.p, dense.d()));
+    DBuf dH(onemats->Hmat, sizeof(double) * n2), dS(onemats->Smat, sizeof(double) * n2);
+    DBuf dC(onemats->MOs, sizeof(double) * n2), dE(sizeof(double) * N);
+    DBuf dP(sizeof(double) * n2), dF(sizeof(double) * n2);
+    corr_scf_opts opts;
+    opts.maxiter = sysinfo->scfiter > 0 ? sysinfo->scfiter : 100;
+    opts.diis_size = 8;
+    opts.core_guess_check = 0;
+    opts.reserved = 0;
+    opts.diis_thresh = 1e-3;
+    opts.damp = 0.5;
+    opts.e_tol = 1e-8;
+    std::vector<double> energies(opts.maxiter), errors(opts.maxiter);
+    corr_scf_result r;
+    const auto t0 = std::chrono::steady_clock::now();
+    check("corr_run_scf", corr_run_scf(ctx(), N, sysinfo->nroe, dH.d(), dS.d(), dense.d(), dC.d(), dE.d(), dP.d(),
+                                       dF.d(), sysinfo->ion_rep, &opts, energies.data(), errors.data(), &r));
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    dC.to_host(onemats->MOs, sizeof(double) * n2);
+    dE.to_host(onemats->MOens, sizeof(double) * N);
+    dP.to_host(onemats->Pmat, sizeof(double) * n2);
+    dF.to_host(onemats->Fmat, sizeof(double) * n2);
+    std::cout << std::fixed << std::setprecision(10);
+    std::cout << std::setw(-4) << "Iter" << std::setw(16) << "ESCF" << std::setw(16) << "DE" << std::setw(16)
+              << "ERROR" << std::setw(16) << "t[s]" << "\n";
+    double e_prev = 100.0 + sysinfo->ion_rep;
+    for (int it = 0; it < r.iterations; ++it) {
+        if (r.diis && it == r.diis_start) std::cout << "Starting DIIS \n";
+        std::cout << std::setw(4) << it << ":" << std::setw(16) << energies[it] << std::setw(16)
+                  << energies[it] - e_prev << std::setw(16) << errors[it] << std::setw(16)
+                  << dt / (r.iterations > 0 ? r.iterations : 1) << '\n';
+        e_prev = energies[it];
+    }
+    std::cout << "\nHF-Results:\n";
+    std::cout << "-----------\n";
+    std::cout << std::setw(PWIDTH_L) << "  ONE E:" << std::setw(PWIDTH_R) << r.e_one << '\n';
+    std::cout << std::setw(PWIDTH_L) << "  TWO E:" << std::setw(PWIDTH_R) << r.e_two << '\n';
+    std::cout << std::setw(PWIDTH_L) << "TOTAL E:" << std::setw(PWIDTH_R) << r.e_total << '\n';
 }
 
 // ccsd.cc:6-9 (index accessor over the spatial (ij|kl) array)

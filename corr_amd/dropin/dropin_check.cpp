@@ -3,13 +3,16 @@
 //   int64 N, nroe, naux, nrofint, sortcount[4]; double ion_rep;
 //   double MOs[N*N], Fmat[N*N], Hmat[N*N], intval[nrofint];
 //   uint16 intnums[4*nrofint]; double J[naux*naux], Pmn[naux*N*N]
-// and prints the reference's result lines.
+// and prints the reference's result lines.  With --scf the file also holds
+// Smat[N*N] after Pmn; run_scf then starts from MOs and produces the MOs and
+// Fmat the post-HF steps use (main.cc:111-121).
 #include "../../include/corr_dropin.h"
 
 #include <cstdint>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <string>
 #include <vector>
 
 template <typename T>
@@ -22,8 +25,9 @@ static void rd(std::ifstream& f, T* p, int64_t n) {
 }
 
 int main(int argc, char** argv) {
-    if (argc != 2) {
-        std::cerr << "usage: corr_dropin_check INPUT" << std::endl;
+    const bool scf = argc == 3 && std::string(argv[2]) == "--scf";
+    if (argc != 2 && !scf) {
+        std::cerr << "usage: corr_dropin_check INPUT [--scf]" << std::endl;
         return 1;
     }
     std::ifstream f(argv[1], std::ios::binary);
@@ -42,6 +46,8 @@ int main(int argc, char** argv) {
     rd(f, intnums.data(), 4 * nint);
     rd(f, J.data(), naux * naux);
     rd(f, Pmn.data(), naux * N * N);
+    std::vector<double> Smat(scf ? N * N : 0), MOens(N), Pmat(N * N);
+    if (scf) rd(f, Smat.data(), N * N);
 
     systeminfo* sys = new systeminfo;
     sys->nroe = (int)nroe;
@@ -53,13 +59,17 @@ int main(int argc, char** argv) {
     one->MOs = MOs.data();
     one->Fmat = Fmat.data();
     one->Hmat = Hmat.data();
+    one->Smat = Smat.data();
+    one->MOens = MOens.data();
+    one->Pmat = Pmat.data();
     TEints* two = new TEints;
     two->intval = intval.data();
     two->intnums = intnums.data();
     for (int k = 0; k < 4; ++k) two->sortcount[k] = hdr[4 + k];
     pHF* post = new pHF;
+    sys->scfiter = 100;
     std::cout << std::fixed << std::setprecision(10);   // the reference's stream state after SCF
-
+    if (scf) run_scf(sys, one, two);
     build_FMo(sys, one, post);
     MOtrans(sys, one, two, post);
     run_canonical_mp2(sys, post);

@@ -70,11 +70,14 @@ def run_scf(Hmat: torch.Tensor, Smat: torch.Tensor, dense_ao: torch.Tensor, MOs:
     F = torch.empty_like(Hmat)
     opts = SCFOpts(maxiter, diis_size, int(core_guess_check), 0, diis_thresh, damp, e_tol)
     energies = np.zeros(max(maxiter, 1))
+    errors = np.zeros(max(maxiter, 1))
     res = SCFResult()
     _ctx(Hmat).call("corr_run_scf", N, nroe, ptr(Hmat), ptr(Smat), ptr(dense_ao), ptr(C_out), ptr(eps), ptr(P),
-                    ptr(F), float(ion_rep), C.byref(opts), energies.ctypes.data_as(C.c_void_p), C.byref(res))
+                    ptr(F), float(ion_rep), C.byref(opts), energies.ctypes.data_as(C.c_void_p),
+                    errors.ctypes.data_as(C.c_void_p), C.byref(res))
     out = {k: getattr(res, k) for k, _ in SCFResult._fields_}
-    out.update(MOs=C_out, MOens=eps, Pmat=P, Fmat=F, energies=energies[:res.iterations].copy())
+    out.update(MOs=C_out, MOens=eps, Pmat=P, Fmat=F, energies=energies[:res.iterations].copy(),
+               errors=errors[:res.iterations].copy())
     return out
 
 

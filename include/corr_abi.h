@@ -262,8 +262,8 @@ int corr_build_pmat(corr_ctx* ctx, int64_t nroao, int64_t nroe,
  * (row = MO) and receives the final ones; d_MOens the orbital energies of the
  * last diagonalisation; d_Pmat and d_Fmat the density and Fock matrix rebuilt
  * after the loop, as the reference leaves them for build_FMo.
- * h_energies (optional, maxiter doubles) receives each iteration's ESCF
- * (electronic + ion_rep).  Fails with CORR_ENOTPD when S is not positive
+ * h_energies and h_errors (optional, maxiter doubles each) receive each
+ * iteration's ESCF (electronic + ion_rep) and mean |S P F - F P S|.  Fails with CORR_ENOTPD when S is not positive
  * definite.  Defaults of the reference: maxiter 100 (main.cc:114),
  * diis_size 8, diis_thresh 1e-3, damp 0.5, e_tol 1e-8 (scf.cc:170-190, 207). */
 typedef struct corr_scf_opts {
@@ -296,7 +296,7 @@ int corr_run_scf(corr_ctx* ctx, int64_t nroao, int64_t nroe,
                  const double* d_dense_ao, double* d_MOs, double* d_MOens,
                  double* d_Pmat, double* d_Fmat, double ion_rep,
                  const corr_scf_opts* opts, double* h_energies,
-                 corr_scf_result* h_out);
+                 double* h_errors, corr_scf_result* h_out);
 
 #ifdef __cplusplus
 }

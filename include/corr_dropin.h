@@ -92,6 +92,15 @@ void transform_ri(systeminfo* sysinfo, OEints* onemats, pHF* postHF);
  * libint2 integral evaluation): J (naux x naux) and (P|mu nu)
  * (naux x N x N) in, BPQ = J^-1/2 (P|mu nu) out (caller-allocated). */
 void calculate_ri_arith(int naux, int nroao, const double* J, const double* Pmn, double* BPQ);
+/* src/SCF/ops_rhf.h — F = H + J(P) - 1/2 K(P) from the sorted unique list. */
+void build_Fmat(int nroao, double* Fmat, double* Pmat, double* Hmat, double* intval, unsigned short* intnums,
+                long long int* sortcount, long long int nrofint);
+/* src/SCF/scf.cc:44 — P = 2(1-damp) C_occ^T C_occ + damp P. */
+void build_Pmat(double* C, int nroao, int nroe, double* Pmat, double damp);
+/* src/SCF/scf.h:5 — damped RHF + DIIS from onemats->MOs (the caller has run
+ * check_initial_guess, as main.cc:111-112 does); leaves MOs, MOens, Pmat and
+ * Fmat in onemats as the reference does.  S^-1/2 is recomputed on the GPU. */
+void run_scf(systeminfo* sysinfo, OEints* onemats, TEints* twomats);
 /* src/POSTHF/ccsd.h:6-7 */
 double get_integral(double* ints, long long int& istep, long long int& jstep, long long int& kstep, int& i,
                     int& j, int& k, int& l);

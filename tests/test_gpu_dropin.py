@@ -68,3 +68,48 @@ def test_dropin_sequence_matches_oracle(tmp_path):
     assert abs(el - ref["e_hf_check"]) < 2e-10
     emp2_so = float(re.search(r"EMP2: (-?\d+\.\d+)", out).group(1))
     assert abs(emp2_so - ref["emp2"]) < 2e-10
+
+
+def test_dropin_scf_then_post_hf_matches_oracle(tmp_path):
+    """main.cc:111-175 through the drop-ins: run_scf from a core guess, then
+    the post-HF sequence on the MOs and Fmat it leaves behind."""
+    from oracle import scf as SCF
+    N, nocc_sp, naux = 8, 2, 24
+    H, Sm, eri, *_ = S.nonorthogonal_system(N, seed=41, nocc=nocc_sp)
+    T = SCF.calc_s12(Sm)
+    C0, _ = SCF.check_initial_guess(np.zeros((N, N)), H, T)
+    val, idx, sc = O.sorted_unique_list(eri)
+    J = S.metric(naux, seed=7, cond=1e3)
+    X = np.random.default_rng(8).standard_normal((naux, N, N)) * 0.1
+    Pmn = X + X.transpose(0, 2, 1)
+    ion = 4.5
+    inp = tmp_path / "scf.bin"
+    write_input(inp, N, 2 * nocc_sp, naux, ion, C0, np.zeros((N, N)), H, val, idx, sc, J, Pmn)
+    with open(inp, "ab") as f:
+        np.ascontiguousarray(Sm, dtype=np.float64).tofile(f)
+    print("running corr_dropin_check --scf", flush=True)
+    res = subprocess.run([str(ROOT / "corr_amd" / "corr_dropin_check"), str(inp), "--scf"], capture_output=True,
+                         text=True, timeout=120)
+    sys.stdout.write(res.stdout[-3000:])
+    assert res.returncode == 0, res.stderr
+    out = res.stdout
+    ref = SCF.run_scf(H, Sm, lambda P: SCF.fock_dense(eri, P, H), C0, 2 * nocc_sp, ion_rep=ion)
+    its = [float(x) for x in re.findall(r"^ *\d+: +(-?\d+\.\d+)", out, re.M)]
+    assert len(its) == ref["iterations"]
+    assert np.allclose(its, ref["energies"], atol=2e-10)
+    assert ("Starting DIIS" in out) == ref["diis"]
+    tot = float(re.search(r"TOTAL E:\s+(-?\d+\.\d+)", out).group(1))
+    assert abs(tot - ref["total"]) < 2e-10
+    C, F = ref["C"], ref["F"]
+    prec = O.motrans(C, eri)
+    FMo = O.build_fmo(C, F)
+    can = O.canonical_mp2(prec, FMo, nocc_sp)
+    M, _ = O.ri_metric_eigen(J)
+    B = O.transform_ri(C, O.ri_bpq(M, Pmn), nocc_sp)
+    e = np.diag(FMo)
+    ri = O.rimp2(B, e[:nocc_sp], e[nocc_sp:])
+    vals = [float(x) for x in re.findall(r"^ +EMP2:\s+(-?\d+\.\d+)", out, re.M)]
+    assert abs(vals[0] - can["EMP2"]) < 2e-10 and abs(vals[1] - ri["EMP2"]) < 2e-10
+    cc = CC.ccsd_ur(prec, C, H, 2 * nocc_sp, ion, maxiter=20)
+    it = [float(x) for x in re.findall(r"E\(CCSD\)= (-?\d+\.\d+)", out)]
+    assert np.allclose(it, cc["energies"], atol=2e-10)
