@@ -73,6 +73,12 @@ class SCFResult(C.Structure):
                 ("core_guess", C.c_int)]
 
 
+class CorrBasis(C.Structure):
+    _fields_ = [("nshell", C.c_int64), ("l", C.POINTER(C.c_int32)), ("nprim", C.POINTER(C.c_int32)),
+                ("center", C.POINTER(C.c_double)), ("exps", C.POINTER(C.c_double)),
+                ("coefs", C.POINTER(C.c_double))]
+
+
 P = C.c_void_p
 I64 = C.c_int64
 I = C.c_int
@@ -112,6 +118,10 @@ SIGNATURES = {
     "corr_build_fmat": [P, I64, P, P, P, P],
     "corr_build_fmat_list": [P, I64, I64, C.POINTER(I64), P, P, P, P, P],
     "corr_build_pmat": [P, I64, I64, P, D, P],
+    "corr_int1e": [P, C.POINTER(CorrBasis), I64, P, P, P, P, P],
+    "corr_int2c": [P, C.POINTER(CorrBasis), P],
+    "corr_int3c": [P, C.POINTER(CorrBasis), C.POINTER(CorrBasis), P],
+    "corr_int4c": [P, C.POINTER(CorrBasis), P],
     "corr_run_scf": [P, I64, I64, P, P, P, P, P, P, P, D, C.POINTER(SCFOpts), P, P, C.POINTER(SCFResult)],
 }
 

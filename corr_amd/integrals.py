@@ -1,0 +1,57 @@
+"""Gaussian integrals on the GPU (libcorr_gpu.so), the counterpart of the
+reference's libint2 calls (calculate_libint_oei, ops_rhf.cc:77-103;
+calculate_libint_tei, :618-723; calculate_ri, ri.cc:25-100)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._abi import Context, ptr
+from .basis import Molecule, ShellArrays
+
+
+def _ctx(device) -> Context:
+    return Context.get(device if device is not None else torch.cuda.current_device())
+
+
+def _empty(shape, device):
+    return torch.empty(shape, dtype=torch.float64, device=device if device is not None else "cuda")
+
+
+def one_electron(mol: Molecule, obs: ShellArrays, device=None):
+    """(S, T, V) over the orbital basis."""
+    n = obs.nbf
+    S, T, V = _empty((n, n), device), _empty((n, n), device), _empty((n, n), device)
+    q = np.ascontiguousarray(mol.charges, dtype=np.float64)
+    X = np.ascontiguousarray(mol.coords, dtype=np.float64).reshape(-1)
+    b = obs.struct()
+    _ctx(device).call("corr_int1e", C.byref(b), len(q), q.ctypes.data_as(C.c_void_p), X.ctypes.data_as(C.c_void_p),
+                      ptr(S), ptr(T), ptr(V))
+    return S, T, V
+
+
+def metric(aux: ShellArrays, device=None) -> torch.Tensor:
+    """(P|Q), naux x naux."""
+    J = _empty((aux.nbf, aux.nbf), device)
+    b = aux.struct()
+    _ctx(device).call("corr_int2c", C.byref(b), ptr(J))
+    return J
+
+
+def three_center(aux: ShellArrays, obs: ShellArrays, device=None) -> torch.Tensor:
+    """(P|mu nu) as [naux, N, N]."""
+    out = _empty((aux.nbf, obs.nbf, obs.nbf), device)
+    ba, bo = aux.struct(), obs.struct()
+    _ctx(device).call("corr_int3c", C.byref(ba), C.byref(bo), ptr(out))
+    return out
+
+
+def four_center(obs: ShellArrays, device=None) -> torch.Tensor:
+    """Dense (mu nu|la si), N^4."""
+    n = obs.nbf
+    out = _empty((n, n, n, n), device)
+    b = obs.struct()
+    _ctx(device).call("corr_int4c", C.byref(b), ptr(out))
+    return out

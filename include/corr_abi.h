@@ -298,6 +298,36 @@ int corr_run_scf(corr_ctx* ctx, int64_t nroao, int64_t nroe,
                  const corr_scf_opts* opts, double* h_energies,
                  double* h_errors, corr_scf_result* h_out);
 
+/* ---- Gaussian integrals (replacing the reference's libint2 calls) ---------
+ * A basis is a list of contracted spherical shells in plain host arrays:
+ * shell s has angular momentum l[s] (0..4), nprim[s] primitives and centre
+ * center[3s..3s+2] (bohr); exps and coefs hold the primitives of all shells in
+ * order, coefs being the contraction coefficients of normalised primitives as
+ * basis-set files (.g94) list them.  Functions are numbered shell by shell,
+ * 2l+1 per shell, m = -l..l (real solid harmonics); every contracted function
+ * has unit self-overlap, as libint2 normalises them.  Orbital shells up to d.
+ *   corr_int1e  S, T, V (N x N)        calculate_libint_oei, ops_rhf.cc:77-103
+ *   corr_int2c  (P|Q) (naux x naux)    calculate_ri, ri.cc:25-60
+ *   corr_int3c  (P|mu nu) [naux][N][N] calculate_ri, ri.cc:62-100
+ *   corr_int4c  dense (mu nu|la si)     calculate_libint_tei, ops_rhf.cc:618-723
+ *               (the reference keeps the sorted unique list instead) */
+typedef struct corr_basis {
+    int64_t        nshell;
+    const int32_t* l;
+    const int32_t* nprim;
+    const double*  center;
+    const double*  exps;
+    const double*  coefs;
+} corr_basis;
+
+int corr_int1e(corr_ctx* ctx, const corr_basis* obs, int64_t natom,
+               const double* h_charges, const double* h_coords,
+               double* d_S, double* d_T, double* d_V);
+int corr_int2c(corr_ctx* ctx, const corr_basis* aux, double* d_J);
+int corr_int3c(corr_ctx* ctx, const corr_basis* aux, const corr_basis* obs,
+               double* d_Pmn);
+int corr_int4c(corr_ctx* ctx, const corr_basis* obs, double* d_dense);
+
 #ifdef __cplusplus
 }
 #endif
