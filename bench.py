@@ -14,7 +14,13 @@ naux = 2432 auxiliary functions.  Two paths are timed, K steps each:
 
 value = algorithmic TFLOP/s of the mixed path, (W_B + W_E) / t_step, with
 W_E = 2 [o(o+1)/2] v^2 naux and W_B = 2 naux o N^2 + 2 naux o v N + 2 naux^2 o v.
-Inputs are seeded synthetic tensors of the real shapes (no integral files).
+
+Inputs (--data real, the default) are real chemistry computed on the GPU
+before the timed region: a seeded (H2O)32 cluster (rigid waters, O-H 0.958 A,
+104.5 deg, random orientations, O atoms uniform in a cube of 40 A^3 per
+molecule, O-O >= 2.8 A), its def2-SVP / def2-SVP-RI integrals, and an RI-JK
+RHF (DIIS, |dE| <= 1e-10) whose orbitals and orbital energies feed the step.
+--data synthetic uses seeded tensors of the same shapes instead.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--suite]
 
@@ -106,6 +112,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (a known-size kernel for scaling HBM counters)")
+    ap.add_argument("--data", choices=("real", "synthetic"), default="real",
+                    help="real: GPU integrals + RI-JK RHF of a seeded water cluster; synthetic: seeded tensors")
     ap.add_argument("--rows", default="", help="comma-separated suite rows to run (default: all)")
     ap.add_argument("--suite", action="store_true",
                     help="also time every other row of the hot path at its own shape (adds a 'suite' key)")
@@ -131,15 +139,33 @@ def main() -> int:
     N, o, v, naux = sizes(args.nmol)
     W_E, W_B = work(N, o, v, naux)
     log(f"(H2O){args.nmol}: N={N} o={o} v={v} naux={naux}; W_E={W_E/1e12:.3f} TF W_B={W_B/1e12:.3f} TF")
-    C, J, Pmn, eo, ev = make_inputs(args.nmol, args.seed, dev)
-    # scale (P|mu nu) so that the FP64 correlation energy is ~ -0.2 Eh per water
-    M, _ = corr_amd.ri_metric(J, "cholesky")
-    B = corr_amd.ri_form_bia(C, Pmn, M, o, "cholesky", "fp64")
-    e0 = corr_amd.rimp2_energy(B, eo, ev, "fp64")["EMP2"]
-    Pmn.mul_((0.2 * args.nmol / abs(e0)) ** 0.25)
-    del B, M
-    torch.cuda.synchronize()
-    log(f"inputs ready (unscaled E={e0:.4f})")
+    inputs = {}
+    if args.data == "real":
+        from corr_amd import chem
+        t_in = time.perf_counter()
+        r = chem.water_cluster_rhf(args.nmol, args.seed, device=dev, log=log)
+        C, J, Pmn, eo, ev = r["C"], r["J"], r["Pmn"], r["eps_occ"], r["eps_vir"]
+        assert (r["nbf"], r["naux"], r["nocc"]) == (N, naux, o)
+        scf = r["scf"]
+        inputs = {"molecule": f"(H2O){args.nmol}, seed {args.seed}", "basis": "def2-SVP / def2-SVP-RI",
+                  "rhf": "RI-JK (def2-SVP-RI), DIIS from iteration 8, |dE| <= 1e-10, start: superposition of each water's own RHF density",
+                  "e_rhf": scf["e_total"], "rhf_iterations": scf["iterations"],
+                  "rhf_converged": bool(scf["converged"]), "homo_lumo_Eh": float(ev[0] - eo[-1]),
+                  "seconds_integrals": round(r["seconds_integrals"], 2), "seconds_rhf": round(r["seconds_scf"], 2),
+                  "seconds_setup_total": round(time.perf_counter() - t_in, 2)}
+        del r
+        torch.cuda.empty_cache()
+        log(f"real inputs ready: E(RHF) = {scf['e_total']:.10f} ({scf['iterations']} iterations)")
+    else:
+        C, J, Pmn, eo, ev = make_inputs(args.nmol, args.seed, dev)
+        # scale (P|mu nu) so that the FP64 correlation energy is ~ -0.2 Eh per water
+        M, _ = corr_amd.ri_metric(J, "cholesky")
+        B = corr_amd.ri_form_bia(C, Pmn, M, o, "cholesky", "fp64")
+        e0 = corr_amd.rimp2_energy(B, eo, ev, "fp64")["EMP2"]
+        Pmn.mul_((0.2 * args.nmol / abs(e0)) ** 0.25)
+        del B, M
+        torch.cuda.synchronize()
+        log(f"synthetic inputs ready (unscaled E={e0:.4f})")
     ctx = Context.get(0)
 
     def step(prec_gemm: str, prec_e: str):
@@ -186,7 +212,10 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16x3/fp32 (mixed; fp64 denominators and reduction)",
-        "data": "synthetic (seeded tensors of the (H2O)32 shapes; no integral files)",
+        "data": ("real chemistry: seeded water cluster, def2-SVP/def2-SVP-RI integrals and RI-JK RHF computed on the "
+                 "GPU by this library before the timed region" if args.data == "real" else
+                 "synthetic (seeded tensors of the (H2O)32 shapes; no integral files)"),
+        "inputs": inputs or None,
         "config": {"workload": f"RI-MP2 B formation + energy, (H2O){args.nmol} def2-SVP/def2-SVP-RI shape",
                    "N": N, "nocc": o, "nvir": v, "naux": naux, "parallelism": "single GPU"},
         "mixed_s": round(mixed["s_per_step"], 5),

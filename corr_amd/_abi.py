@@ -123,6 +123,8 @@ SIGNATURES = {
     "corr_int3c": [P, C.POINTER(CorrBasis), C.POINTER(CorrBasis), P],
     "corr_int4c": [P, C.POINTER(CorrBasis), P],
     "corr_run_scf": [P, I64, I64, P, P, P, P, P, P, P, D, C.POINTER(SCFOpts), P, P, C.POINTER(SCFResult)],
+    "corr_build_fmat_ri": [P, I64, I64, P, P, P, P],
+    "corr_run_scf_ri": [P, I64, I64, P, P, I64, P, P, P, P, P, D, C.POINTER(SCFOpts), P, P, C.POINTER(SCFResult)],
 }
 
 _lib = None

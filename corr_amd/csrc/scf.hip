@@ -243,6 +243,39 @@ __global__ __launch_bounds__(256) void trace_kernel(const double* __restrict__ C
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+// out[r] = sum_k B[r][k] x[k], one workgroup per row, fixed-order tree.
+__global__ __launch_bounds__(256) void rowdot_kernel(const double* __restrict__ B, const double* __restrict__ x,
+                                                     int64_t K, double* __restrict__ out) {
+    __shared__ double sh[256];
+    const double* row = B + (int64_t)blockIdx.x * K;
+    double acc = 0.0;
+    for (int64_t k = threadIdx.x; k < K; k += 256) acc = fma(row[k], x[k], acc);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+// out[k] = sum_r B[r][k] d[r], r ascending (coalesced over k).
+__global__ __launch_bounds__(256) void coldot_kernel(const double* __restrict__ B, const double* __restrict__ d,
+                                                     int64_t rows, int64_t K, double* __restrict__ out) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    double acc = 0.0;
+    for (int64_t r = 0; r < rows; ++r) acc = fma(B[r * K + k], d[r], acc);
+    out[k] = acc;
+}
+
+// F = H + J - 1/2 K
+__global__ void jk_combine_kernel(const double* __restrict__ H, const double* __restrict__ J,
+                                  const double* __restrict__ K, int64_t n, double* __restrict__ F) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        F[e] = H[e] + J[e] - 0.5 * K[e];
+}
+
 template <int NJ, int VEC>
 int launch_slab(corr_ctx* ctx, const double* D, const double* P, int64_t N, double* J, double* Kpart) {
     constexpr int RU = NJ * VEC <= 2 ? 8 : (NJ * VEC <= 4 ? 4 : 2);
@@ -309,6 +342,34 @@ int build_fmat_list(corr_ctx* ctx, int64_t N, int64_t nrofint, const int64_t* sc
     return build_fmat(ctx, N, D, P, H, F);
 }
 
+// RI-JK Fock matrix from B^Q_mu nu (naux x N x N, (mu nu|la si) ~ sum_Q B^Q_mu nu B^Q_la si):
+//   J = sum_Q B^Q (sum_la si B^Q_la si P_la si)           two fixed-order GEMV passes
+//   K = sum_Q B^Q P B^Q = Y B,  Y[mu][(Q,la)] = (B^Q P)[mu][la]
+// Y is written in [mu][Q][la] order by a strided-batched GEMM (ldc = naux N),
+// so K is one GEMM with inner dimension naux N (split-K, fixed-order reduce).
+// No counterpart in the reference, whose SCF needs the full integral list.
+int build_fmat_ri(corr_ctx* ctx, int64_t N, int64_t naux, const double* B, const double* P, const double* H,
+                  double* F) {
+    CORR_REQUIRE(ctx, N > 0 && naux > 0 && B && P && H && F, "build_fmat_ri: bad arguments");
+    CORR_REQUIRE(ctx, F != P && F != H, "build_fmat_ri: F must not alias P or H");
+    CORR_REQUIRE(ctx, naux < (1 << 30), "build_fmat_ri: naux too large");
+    const int64_t nn = N * N;
+    double *d = nullptr, *J = nullptr, *K = nullptr, *Y = nullptr;
+    CORR_TRY(scratch(ctx, "rijk_d", sizeof(double) * naux, (void**)&d));
+    CORR_TRY(scratch(ctx, "rijk_J", sizeof(double) * nn, (void**)&J));
+    CORR_TRY(scratch(ctx, "rijk_K", sizeof(double) * nn, (void**)&K));
+    CORR_TRY(scratch(ctx, "rijk_Y", sizeof(double) * naux * nn, (void**)&Y));
+    hipLaunchKernelGGL(rowdot_kernel, dim3((unsigned)naux), dim3(256), 0, ctx->stream, B, P, nn, d);
+    CORR_LAUNCHED(ctx, "rowdot_kernel");
+    hipLaunchKernelGGL(coldot_kernel, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, ctx->stream, B, d, naux, nn, J);
+    CORR_LAUNCHED(ctx, "coldot_kernel");
+    CORR_TRY(gemm(ctx, 0, 0, N, N, N, 1.0, B, N, nn, P, N, 0, 0.0, Y, naux * N, N, naux, CORR_PREC_FP64));
+    CORR_TRY(gemm(ctx, 0, 0, N, N, naux * N, 1.0, Y, naux * N, 0, B, N, 0, 0.0, K, N, 0, 1, CORR_PREC_FP64));
+    hipLaunchKernelGGL(jk_combine_kernel, dim3(grid_for(nn)), dim3(256), 0, ctx->stream, H, J, K, nn, F);
+    CORR_LAUNCHED(ctx, "jk_combine_kernel");
+    return CORR_OK;
+}
+
 int build_pmat(corr_ctx* ctx, int64_t N, int64_t nroe, const double* C, double damp, double* P) {
     CORR_REQUIRE(ctx, N > 0 && C && P && nroe >= 2 && nroe / 2 <= N, "build_pmat: need 2 <= nroe <= 2 nroao");
     CORR_REQUIRE(ctx, C != P, "build_pmat: P must not alias the MOs");
@@ -350,11 +411,11 @@ int diag_fmat(corr_ctx* ctx, const ScfWork& w, const double* F, double* C, doubl
 
 }  // namespace
 
-int run_scf(corr_ctx* ctx, int64_t N, int64_t nroe, const double* H, const double* S, const double* D, double* C,
-            double* MOens, double* P, double* F, double ion_rep, const corr_scf_opts* opts, double* h_energies,
-            double* h_errors, corr_scf_result* res) {
-    CORR_REQUIRE(ctx, H && S && D && C && MOens && P && F && opts && res, "run_scf: null pointer");
-    CORR_REQUIRE(ctx, N > 0 && N <= 64 * FOCK_MAX_NJ, "run_scf: nroao must be in 1..512");
+template <class Fock>
+int run_scf_impl(corr_ctx* ctx, int64_t N, int64_t nroe, const double* H, const double* S, const Fock& fock,
+                 double* C, double* MOens, double* P, double* F, double ion_rep, const corr_scf_opts* opts,
+                 double* h_energies, double* h_errors, corr_scf_result* res) {
+    CORR_REQUIRE(ctx, H && S && C && MOens && P && F && opts && res, "run_scf: null pointer");
     CORR_REQUIRE(ctx, nroe > 0 && nroe % 2 == 0 && nroe / 2 <= N, "run_scf: nroe must be even and <= 2 nroao");
     CORR_REQUIRE(ctx, opts->maxiter >= 0 && opts->diis_size >= 1 && opts->diis_size <= MAX_DIIS,
                  "run_scf: maxiter >= 0 and diis_size in 1..15");
@@ -400,7 +461,7 @@ int run_scf(corr_ctx* ctx, int64_t N, int64_t nroe, const double* H, const doubl
     int it = 0;
     while (it < opts->maxiter && std::fabs(DE) > opts->e_tol) {
         CORR_TRY(build_pmat(ctx, N, nroe, C, damp, P));
-        CORR_TRY(build_fmat(ctx, N, D, P, H, F));
+        CORR_TRY(fock(P, F));
         const int slot = ord[0];
         for (int i = 0; i + 1 < m; ++i) ord[i] = ord[i + 1];
         ord[m - 1] = slot;
@@ -468,7 +529,7 @@ int run_scf(corr_ctx* ctx, int64_t N, int64_t nroe, const double* H, const doubl
         ++it;
     }
     CORR_TRY(build_pmat(ctx, N, nroe, C, damp, P));
-    CORR_TRY(build_fmat(ctx, N, D, P, H, F));
+    CORR_TRY(fock(P, F));
     DotSet fin{};
     fin.nd = 2;
     fin.x[0] = P;
@@ -487,6 +548,23 @@ int run_scf(corr_ctx* ctx, int64_t N, int64_t nroe, const double* H, const doubl
     res->converged = std::fabs(DE) <= opts->e_tol ? 1 : 0;
     res->diis = diis ? 1 : 0;
     return CORR_OK;
+}
+
+int run_scf(corr_ctx* ctx, int64_t N, int64_t nroe, const double* H, const double* S, const double* D, double* C,
+            double* MOens, double* P, double* F, double ion_rep, const corr_scf_opts* opts, double* h_energies,
+            double* h_errors, corr_scf_result* res) {
+    CORR_REQUIRE(ctx, D, "run_scf: null integral tensor");
+    CORR_REQUIRE(ctx, N > 0 && N <= 64 * FOCK_MAX_NJ, "run_scf: nroao must be in 1..512");
+    auto fock = [&](const double* Pm, double* Fm) { return build_fmat(ctx, N, D, Pm, H, Fm); };
+    return run_scf_impl(ctx, N, nroe, H, S, fock, C, MOens, P, F, ion_rep, opts, h_energies, h_errors, res);
+}
+
+int run_scf_ri(corr_ctx* ctx, int64_t N, int64_t nroe, const double* H, const double* S, int64_t naux,
+               const double* B, double* C, double* MOens, double* P, double* F, double ion_rep,
+               const corr_scf_opts* opts, double* h_energies, double* h_errors, corr_scf_result* res) {
+    CORR_REQUIRE(ctx, B && naux > 0 && N > 0, "run_scf_ri: bad arguments");
+    auto fock = [&](const double* Pm, double* Fm) { return build_fmat_ri(ctx, N, naux, B, Pm, H, Fm); };
+    return run_scf_impl(ctx, N, nroe, H, S, fock, C, MOens, P, F, ion_rep, opts, h_energies, h_errors, res);
 }
 
 }  // namespace corr
@@ -509,6 +587,21 @@ int corr_build_fmat_list(corr_ctx* ctx, int64_t nroao, int64_t nrofint, const in
 int corr_build_pmat(corr_ctx* ctx, int64_t nroao, int64_t nroe, const double* d_MOs, double damp, double* d_Pmat) {
     CORR_CHECK_CTX(ctx);
     return corr::build_pmat(ctx, nroao, nroe, d_MOs, damp, d_Pmat);
+}
+
+int corr_build_fmat_ri(corr_ctx* ctx, int64_t nroao, int64_t naux, const double* d_BPQ, const double* d_Pmat,
+                       const double* d_Hmat, double* d_Fmat) {
+    CORR_CHECK_CTX(ctx);
+    return corr::build_fmat_ri(ctx, nroao, naux, d_BPQ, d_Pmat, d_Hmat, d_Fmat);
+}
+
+int corr_run_scf_ri(corr_ctx* ctx, int64_t nroao, int64_t nroe, const double* d_Hmat, const double* d_Smat,
+                    int64_t naux, const double* d_BPQ, double* d_MOs, double* d_MOens, double* d_Pmat,
+                    double* d_Fmat, double ion_rep, const corr_scf_opts* opts, double* h_energies, double* h_errors,
+                    corr_scf_result* res) {
+    CORR_CHECK_CTX(ctx);
+    return corr::run_scf_ri(ctx, nroao, nroe, d_Hmat, d_Smat, naux, d_BPQ, d_MOs, d_MOens, d_Pmat, d_Fmat, ion_rep,
+                            opts, h_energies, h_errors, res);
 }
 
 int corr_run_scf(corr_ctx* ctx, int64_t nroao, int64_t nroe, const double* d_Hmat, const double* d_Smat,

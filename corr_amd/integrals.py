@@ -12,12 +12,23 @@ from ._abi import Context, ptr
 from .basis import Molecule, ShellArrays
 
 
+def _index(device) -> int:
+    if device is None:
+        return torch.cuda.current_device()
+    if isinstance(device, int):
+        return device
+    d = torch.device(device)
+    if d.type != "cuda":
+        raise ValueError("the integral engine runs on the GPU only")
+    return d.index if d.index is not None else torch.cuda.current_device()
+
+
 def _ctx(device) -> Context:
-    return Context.get(device if device is not None else torch.cuda.current_device())
+    return Context.get(_index(device))
 
 
 def _empty(shape, device):
-    return torch.empty(shape, dtype=torch.float64, device=device if device is not None else "cuda")
+    return torch.empty(shape, dtype=torch.float64, device=torch.device("cuda", _index(device)))
 
 
 def one_electron(mol: Molecule, obs: ShellArrays, device=None):

@@ -53,6 +53,25 @@ def build_Pmat(MOs: torch.Tensor, nroe: int, damp: float = 0.0, Pmat: torch.Tens
     return Pmat
 
 
+def _run(fn, args, Hmat, Smat, MOs, nroe, ion_rep, maxiter, diis_size, diis_thresh, damp, e_tol, core_guess_check):
+    N = Hmat.shape[0]
+    C_out = MOs.clone()
+    eps = torch.empty(N, dtype=torch.float64, device=Hmat.device)
+    P = torch.empty_like(Hmat)
+    F = torch.empty_like(Hmat)
+    opts = SCFOpts(maxiter, diis_size, int(core_guess_check), 0, diis_thresh, damp, e_tol)
+    energies = np.zeros(max(maxiter, 1))
+    errors = np.zeros(max(maxiter, 1))
+    res = SCFResult()
+    _ctx(Hmat).call(fn, N, nroe, ptr(Hmat), ptr(Smat), *args, ptr(C_out), ptr(eps), ptr(P), ptr(F), float(ion_rep),
+                    C.byref(opts), energies.ctypes.data_as(C.c_void_p), errors.ctypes.data_as(C.c_void_p),
+                    C.byref(res))
+    out = {k: getattr(res, k) for k, _ in SCFResult._fields_}
+    out.update(MOs=C_out, MOens=eps, Pmat=P, Fmat=F, energies=energies[:res.iterations].copy(),
+               errors=errors[:res.iterations].copy())
+    return out
+
+
 def run_scf(Hmat: torch.Tensor, Smat: torch.Tensor, dense_ao: torch.Tensor, MOs: torch.Tensor, nroe: int,
             ion_rep: float = 0.0, maxiter: int = 100, diis_size: int = 8, diis_thresh: float = 1e-3,
             damp: float = 0.5, e_tol: float = 1e-8, core_guess_check: bool = True) -> dict:
@@ -64,21 +83,32 @@ def run_scf(Hmat: torch.Tensor, Smat: torch.Tensor, dense_ao: torch.Tensor, MOs:
     N = Hmat.shape[0]
     if dense_ao.numel() != N ** 4 or Smat.shape != (N, N) or MOs.shape != (N, N):
         raise ValueError("run_scf: shapes must be H, S, MOs (N,N) and dense (N,N,N,N)")
-    C_out = MOs.clone()
-    eps = torch.empty(N, dtype=torch.float64, device=Hmat.device)
-    P = torch.empty_like(Hmat)
+    return _run("corr_run_scf", (ptr(dense_ao),), Hmat, Smat, MOs, nroe, ion_rep, maxiter, diis_size, diis_thresh,
+                damp, e_tol, core_guess_check)
+
+
+def build_Fmat_ri(BPQ: torch.Tensor, Pmat: torch.Tensor, Hmat: torch.Tensor) -> torch.Tensor:
+    """RI-JK Fock matrix H + J - 1/2 K from B^Q_mu nu (no reference counterpart)."""
+    _dev(BPQ, Pmat, Hmat)
+    naux, N = BPQ.shape[0], Hmat.shape[0]
+    if BPQ.shape != (naux, N, N):
+        raise ValueError("BPQ must be [naux, N, N]")
     F = torch.empty_like(Hmat)
-    opts = SCFOpts(maxiter, diis_size, int(core_guess_check), 0, diis_thresh, damp, e_tol)
-    energies = np.zeros(max(maxiter, 1))
-    errors = np.zeros(max(maxiter, 1))
-    res = SCFResult()
-    _ctx(Hmat).call("corr_run_scf", N, nroe, ptr(Hmat), ptr(Smat), ptr(dense_ao), ptr(C_out), ptr(eps), ptr(P),
-                    ptr(F), float(ion_rep), C.byref(opts), energies.ctypes.data_as(C.c_void_p),
-                    errors.ctypes.data_as(C.c_void_p), C.byref(res))
-    out = {k: getattr(res, k) for k, _ in SCFResult._fields_}
-    out.update(MOs=C_out, MOens=eps, Pmat=P, Fmat=F, energies=energies[:res.iterations].copy(),
-               errors=errors[:res.iterations].copy())
-    return out
+    _ctx(Hmat).call("corr_build_fmat_ri", N, naux, ptr(BPQ), ptr(Pmat), ptr(Hmat), ptr(F))
+    return F
 
 
-__all__ = ["build_Fmat", "build_Fmat_list", "build_Pmat", "run_scf", "Context"]
+def run_scf_ri(Hmat: torch.Tensor, Smat: torch.Tensor, BPQ: torch.Tensor, MOs: torch.Tensor, nroe: int,
+               ion_rep: float = 0.0, maxiter: int = 100, diis_size: int = 8, diis_thresh: float = 1e-3,
+               damp: float = 0.5, e_tol: float = 1e-8, core_guess_check: bool = True) -> dict:
+    """run_scf with the RI-JK Fock build (for clusters whose full integral
+    tensor does not fit)."""
+    _dev(Hmat, Smat, BPQ, MOs)
+    naux, N = BPQ.shape[0], Hmat.shape[0]
+    if BPQ.shape != (naux, N, N) or Smat.shape != (N, N) or MOs.shape != (N, N):
+        raise ValueError("run_scf_ri: shapes must be H, S, MOs (N,N) and BPQ (naux,N,N)")
+    return _run("corr_run_scf_ri", (naux, ptr(BPQ)), Hmat, Smat, MOs, nroe, ion_rep, maxiter, diis_size,
+                diis_thresh, damp, e_tol, core_guess_check)
+
+
+__all__ = ["build_Fmat", "build_Fmat_list", "build_Fmat_ri", "build_Pmat", "run_scf", "run_scf_ri", "Context"]

@@ -298,6 +298,23 @@ int corr_run_scf(corr_ctx* ctx, int64_t nroao, int64_t nroe,
                  const corr_scf_opts* opts, double* h_energies,
                  double* h_errors, corr_scf_result* h_out);
 
+/* RI-JK variants: J and K from B^Q_mu nu (naux x N x N, e.g. corr_ri_bpq of
+ * the metric and (P|mu nu)) instead of the full integral list:
+ *   J = sum_Q B^Q tr(B^Q P),  K = sum_Q B^Q P B^Q.
+ * No counterpart in the reference, whose SCF needs the full integral list (at
+ * (H2O)32, N = 768, the dense tensor would take 2.8 TB); they make the RHF
+ * in front of the post-HF path possible at that size.  Scratch: naux N^2
+ * doubles. */
+int corr_build_fmat_ri(corr_ctx* ctx, int64_t nroao, int64_t naux,
+                       const double* d_BPQ, const double* d_Pmat,
+                       const double* d_Hmat, double* d_Fmat);
+int corr_run_scf_ri(corr_ctx* ctx, int64_t nroao, int64_t nroe,
+                    const double* d_Hmat, const double* d_Smat, int64_t naux,
+                    const double* d_BPQ, double* d_MOs, double* d_MOens,
+                    double* d_Pmat, double* d_Fmat, double ion_rep,
+                    const corr_scf_opts* opts, double* h_energies,
+                    double* h_errors, corr_scf_result* h_out);
+
 /* ---- Gaussian integrals (replacing the reference's libint2 calls) ---------
  * A basis is a list of contracted spherical shells in plain host arrays:
  * shell s has angular momentum l[s] (0..4), nprim[s] primitives and centre

@@ -87,3 +87,15 @@ def nonorthogonal_system(N, seed=0, spread=0.15, eri_scale=0.05, nocc=None):
     H = X.T @ h @ X
     eri_ao = np.einsum("km,ln,ar,bs,klab->mnrs", X, X, X, X, eri, optimize=True)
     return H, S, eri_ao, h, eri, X
+
+
+def factored_system(N, nocc, seed=0, spread=0.15, eri_scale=0.05):
+    """(H, S, B): the gapped model of nonorthogonal_system with its integrals
+    as exact factors, (mu nu|la si) = sum_Q B[Q, mu, nu] B[Q, la, si]."""
+    H, S, eri, h, eri_o, X = nonorthogonal_system(N, seed=seed, spread=spread, eri_scale=eri_scale, nocc=nocc)
+    rng = np.random.default_rng(seed + 1)
+    naux = 3 * N
+    Xq = rng.standard_normal((naux, N, N)) / N
+    Xq = 0.5 * (Xq + Xq.transpose(0, 2, 1))
+    B = np.sqrt(eri_scale * N) * np.einsum("km,Qkl,ln->Qmn", X, Xq, X)
+    return H, S, B

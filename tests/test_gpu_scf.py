@@ -160,3 +160,24 @@ def test_run_scf_maxiter_cap(corr):
     assert r["iterations"] == 3 and r["converged"] == 0 and not ref["converged"]
     assert np.abs(r["energies"] - ref["energies"]).max() < 1e-10
     assert abs(r["e_total"] - ref["total"]) < 1e-10
+
+
+def test_fock_ri_equals_dense_for_factorised_integrals(corr):
+    """With (mu nu|la si) = sum_Q B^Q_mu nu B^Q_la si exactly, the RI-JK Fock
+    matrix equals the dense Fock build, and the two SCF drivers agree."""
+    say("fock RI-JK vs dense")
+    N, nocc = 40, 6
+    rng = np.random.default_rng(11)
+    H, Sm, B = S.factored_system(N, nocc, seed=12)
+    eri = np.einsum("Qab,Qcd->abcd", B, B)
+    P, Hr = _sym(rng, N), _sym(rng, N)
+    Fd = corr.build_Fmat(dev(eri), dev(P), dev(Hr)).cpu().numpy()
+    Fr = corr.build_Fmat_ri(dev(B), dev(P), dev(Hr)).cpu().numpy()
+    assert np.abs(Fd - Fr).max() < 1e-12 * np.abs(Fd).max()
+    assert np.abs(Fr - SCF.fock_dense(eri, P, Hr)).max() < 1e-12 * np.abs(Fd).max()
+    C0 = np.zeros((N, N))
+    rd = corr.run_scf(dev(H), dev(Sm), dev(eri), dev(C0), 2 * nocc, ion_rep=0.5)
+    rr = corr.run_scf_ri(dev(H), dev(Sm), dev(B), dev(C0), 2 * nocc, ion_rep=0.5)
+    assert rd["converged"] == 1 and rr["iterations"] == rd["iterations"]
+    assert np.abs(rr["energies"] - rd["energies"]).max() < 1e-10
+    assert abs(rr["e_total"] - rd["e_total"]) < 1e-10
