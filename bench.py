@@ -25,8 +25,8 @@ RHF (DIIS, |dE| <= 1e-10) whose orbitals and orbital energies feed the step.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--suite]
 
 --suite adds a "suite" object with the other rows of the hot path, each at its
-own shape: build_FMo (N=768), AO unfold + 4-index transform + canonical MP2
-((H2O)8, N=192), metric and B formation ((H2O)32), RI-MP2 gradient ((H2O)16),
+own shape: build_FMo (N=768), AO unfold + 4-index transform + canonical and
+non-canonical MP2 ((H2O)8, N=192), metric and B formation ((H2O)32), RI-MP2 gradient ((H2O)16),
 CCSD per iteration ((H2O)4) and the RHF input side (Fock build and one
 RHF+DIIS iteration at (H2O)8).  --rows NAME,... restricts the suite to the
 named rows.
@@ -372,6 +372,16 @@ def run_suite(dev, seed: int, rows=()) -> dict:
         t = _timed(lambda: corr_amd.run_canonical_mp2(ints, FMo, o))
         res["canonical_mp2_ms"] = round(t * 1e3, 3)
         res["canonical_mp2_GBs"] = round(2 * o * o * (N - o) ** 2 * 8 / t / 1e9, 1)
+        # non-canonical MP2 (loc_mp2.cc): a rotated occupied block makes FMo non-diagonal
+        Fr = FMo.clone()
+        Fr[:o, :o] += 0.01 * torch.ones(o, o, dtype=torch.float64, device=dev)
+        t1 = _timed(lambda: corr_amd.run_non_canonical_mp2(ints, Fr, o, 1), reps=2)
+        t21 = _timed(lambda: corr_amd.run_non_canonical_mp2(ints, Fr, o, 21), reps=2)
+        it = (t21 - t1) / 20
+        ov2 = o * o * (N - o) ** 2
+        res["non_canonical_mp2_iteration_ms"] = round(it * 1e3, 3)
+        # per iteration: G GEMM reads T, writes G; the step reads K, T, G and both transposes, writes T
+        res["non_canonical_mp2_GBs"] = round(8 * ov2 * 8 / it / 1e9, 1)
         return res
     rec("fourindex_h2o8", fourindex)
 

@@ -118,6 +118,7 @@ SIGNATURES = {
     "corr_build_fmat": [P, I64, P, P, P, P],
     "corr_build_fmat_list": [P, I64, I64, C.POINTER(I64), P, P, P, P, P],
     "corr_build_pmat": [P, I64, I64, P, D, P],
+    "corr_non_canonical_mp2": [P, I64, I64, P, P, I, P, P, P],
     "corr_int1e": [P, C.POINTER(CorrBasis), I64, P, P, P, P, P],
     "corr_int2c": [P, C.POINTER(CorrBasis), P],
     "corr_int3c": [P, C.POINTER(CorrBasis), C.POINTER(CorrBasis), P],

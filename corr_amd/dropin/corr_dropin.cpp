@@ -198,6 +198,22 @@ void read_transform_ri(std::string prefix, int nroe, int nroao, int naux_2, doub
     transform_ri_device(naux_2, nroao, nroe, MOs, dBPQ.d(), Bia);
 }
 
+// loc_mp2.cc:7-92
+void run_non_canonical_mp2(int nroao, int nroe, double* prec_ints, double* FMo) {
+    const int64_t N = nroao, o = nroe / 2;
+    std::cout << "\nCalculating semi-canonical amplitudes ... ";
+    DBuf* owned = nullptr;
+    const double* dI = device_prec_ints(prec_ints, N, &owned);
+    DBuf dF(FMo, sizeof(double) * N * N);
+    std::vector<double> E(20), R(20);
+    check("corr_non_canonical_mp2", corr_non_canonical_mp2(ctx(), N, o, dI, dF.d(), 20, E.data(), R.data(), nullptr));
+    delete owned;
+    std::cout << "done" << '\n';
+    for (int it = 0; it < 20; ++it)
+        std::cout << std::fixed << std::setw(10) << std::setprecision(10) << "E(sem-loc):" << std::setw(16) << E[it]
+                  << '\t' << "Rsum:" << R[it] << '\n';
+}
+
 // ops_rhf.cc:549-585
 void build_Fmat(int nroao, double* Fmat, double* Pmat, double* Hmat, double* intval, unsigned short* intnums,
                 long long int* sortcount, long long int nrofint) {

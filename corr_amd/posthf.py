@@ -94,6 +94,26 @@ def run_canonical_mp2(prec_ints: torch.Tensor, FMo: torch.Tensor, nocc: int) -> 
     return e.as_dict()
 
 
+def run_non_canonical_mp2(prec_ints: torch.Tensor, FMo: torch.Tensor, nocc: int, maxiter: int = 20,
+                          return_T: bool = False) -> dict:
+    """Semi-canonical MP2 with the off-diagonal occupied Fock block
+    (run_non_canonical_mp2, loc_mp2.cc:7-92): per-iteration "E(sem-loc)" and
+    "Rsum"; optionally the final amplitudes T[i][j][a][b]."""
+    import numpy as np
+    _dev(prec_ints, FMo)
+    N = FMo.shape[0]
+    v = N - nocc
+    E = np.zeros(max(maxiter, 1))
+    R = np.zeros(max(maxiter, 1))
+    T = torch.empty(nocc, nocc, v, v, dtype=torch.float64, device=FMo.device) if return_T else None
+    _ctx(FMo).call("corr_non_canonical_mp2", N, nocc, ptr(prec_ints), ptr(FMo), maxiter,
+                   E.ctypes.data_as(C.c_void_p), R.ctypes.data_as(C.c_void_p), ptr(T))
+    out = {"energies": E[:maxiter].copy(), "rsum": R[:maxiter].copy()}
+    if return_T:
+        out["T"] = T
+    return out
+
+
 def ri_metric(J: torch.Tensor, mode: str = "eigen") -> tuple[torch.Tensor, int]:
     """(M, kept) with M^T M = J^-1 (ri.cc:50-69 for mode='eigen')."""
     _dev(J)

@@ -27,6 +27,7 @@
  *   run_canonical_mp2_ri  src/POSTHF/can_mp2.cc:53-132
  *   ccsd_ur and helpers   src/POSTHF/ccsd.cc:6-580,
  *                         src/POSTHF/ccsd_intermediates.cc:6-287
+ *   run_non_canonical_mp2 src/POSTHF/loc_mp2.cc:7-92
  *   build_Fmat            src/SCF/ops_rhf.cc:466-585
  *   build_Pmat            src/SCF/scf.cc:44-61
  *   run_scf (+ calc_S12,  src/SCF/scf.cc:105-320,
@@ -314,6 +315,18 @@ int corr_run_scf_ri(corr_ctx* ctx, int64_t nroao, int64_t nroe,
                     double* d_Pmat, double* d_Fmat, double ion_rep,
                     const corr_scf_opts* opts, double* h_energies,
                     double* h_errors, corr_scf_result* h_out);
+
+/* ---- non-canonical MP2 (run_non_canonical_mp2, loc_mp2.cc:7-92) ----------
+ * Semi-canonical MP2 amplitudes with the off-diagonal occupied block of FMo:
+ * maxiter Jacobi updates (20 in the reference).  h_energies[maxiter] and
+ * h_rsum[maxiter] receive each iteration's "E(sem-loc)" and "Rsum" (computed
+ * before that iteration's update, as printed by the reference).  d_T
+ * (optional, nocc^2 nvir^2 doubles, [i][j][a][b]) receives the final
+ * amplitudes.  Scratch: 4 nocc^2 nvir^2 doubles. */
+int corr_non_canonical_mp2(corr_ctx* ctx, int64_t nroao, int64_t nocc,
+                           const double* d_prec_ints, const double* d_FMo,
+                           int maxiter, double* h_energies, double* h_rsum,
+                           double* d_T);
 
 /* ---- Gaussian integrals (replacing the reference's libint2 calls) ---------
  * A basis is a list of contracted spherical shells in plain host arrays:

@@ -101,6 +101,8 @@ void build_Pmat(double* C, int nroao, int nroe, double* Pmat, double damp);
  * check_initial_guess, as main.cc:111-112 does); leaves MOs, MOens, Pmat and
  * Fmat in onemats as the reference does.  S^-1/2 is recomputed on the GPU. */
 void run_scf(systeminfo* sysinfo, OEints* onemats, TEints* twomats);
+/* src/POSTHF/loc_mp2.h — 20 semi-canonical iterations, "E(sem-loc)" lines. */
+void run_non_canonical_mp2(int nroao, int nroe, double* prec_ints, double* FMo);
 /* src/POSTHF/ccsd.h:6-7 */
 double get_integral(double* ints, long long int& istep, long long int& jstep, long long int& kstep, int& i,
                     int& j, int& k, int& l);

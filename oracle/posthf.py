@@ -264,3 +264,34 @@ def rimp2_gradient(Bia: np.ndarray, eo: np.ndarray, ev: np.ndarray, c_os: float 
     d_ev = 2.0 * np.einsum("ijab->a", W)
     d_eo = -2.0 * np.einsum("ijab->i", W)
     return float(E), dB, d_eo, d_ev
+
+
+# ------------------------------------------------------ non-canonical MP2 ----
+
+
+def non_canonical_mp2(prec: np.ndarray, FMo: np.ndarray, nocc: int, iters: int = 20):
+    """Semi-canonical MP2 amplitudes with off-diagonal occupied Fock coupling
+    (run_non_canonical_mp2, loc_mp2.cc:7-92).  Returns (E[iters], Rsum[iters],
+    T) where iteration n prints E and Rsum of the amplitudes before its
+    update:
+      T0 = -(ia|jb) / D,  D = f_aa + f_bb - f_ii - f_jj
+      G_ijab = -sum_{k != i} f_ik T_kjab
+      R = (ia|jb) + D T + G_ijab + G_jiba,  Rsum = sum |R|
+      E = sum ((ia|jb) + R)(2 T_ijab - T_ijba),  T -= R / D."""
+    N = FMo.shape[0]
+    o, v = nocc, N - nocc
+    K = prec[:o, o:, :o, o:].transpose(0, 2, 1, 3).copy()        # [i][j][a][b] = (ia|jb)
+    f = np.diag(FMo)
+    D = f[o:][None, None, :, None] + f[o:][None, None, None, :] - f[:o][:, None, None, None] \
+        - f[:o][None, :, None, None]
+    T = -K / D
+    Foff = FMo[:o, :o].copy()
+    np.fill_diagonal(Foff, 0.0)
+    E, Rs = [], []
+    for _ in range(iters):
+        G = -np.einsum("ik,kjab->ijab", Foff, T)
+        R = K + D * T + G + G.transpose(1, 0, 3, 2)
+        Rs.append(float(np.sum(np.abs(R))))
+        E.append(float(np.sum((K + R) * (2 * T - T.transpose(0, 1, 3, 2)))))
+        T = T - R / D
+    return np.array(E), np.array(Rs), T

@@ -232,3 +232,30 @@ def test_unfold_motrans_canonical_mp2(corr, N, o):
         assert abs(e[k] - e_ref[k]) < 1e-12, k
     mo32 = corr.MOtrans(dev(C), ao, prec="fp32").cpu().numpy()
     assert np.max(np.abs(mo32 - ref)) < 1e-5 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("N,o,rotate", [(10, 3, False), (10, 3, True), (37, 9, True)])
+def test_non_canonical_mp2_vs_oracle(N, o, rotate):
+    """run_non_canonical_mp2 (loc_mp2.cc:7-92): 20 iterations of E(sem-loc)
+    and Rsum against the oracle (1e-12 relative); with rotated occupied
+    orbitals (exp of a random antisymmetric generator, angle 0.3) the energy
+    converges to the canonical MP2 energy of the converged RHF."""
+    print(f"non-canonical MP2 N={N} o={o} rotate={rotate}", flush=True)
+    from scipy.linalg import expm
+    import corr_amd
+    *_, h, eri, _X = S.nonorthogonal_system(N, seed=N, nocc=o)
+    C, eps, F = S.rhf_orthonormal(h, eri, o)
+    e_can = O.canonical_mp2(O.motrans(C, eri), O.build_fmo(C, F), o)["EMP2"]
+    if rotate:
+        A = np.random.default_rng(N + 1).standard_normal((o, o))
+        C = C.copy()
+        C[:o] = expm(0.3 * (A - A.T)) @ C[:o]
+    prec = O.motrans(C, eri)
+    FMo = O.build_fmo(C, F)
+    E, Rs, T = O.non_canonical_mp2(prec, FMo, o, 20)
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to("cuda")   # noqa: E731
+    r = corr_amd.run_non_canonical_mp2(d(prec), d(FMo), o, 20, return_T=True)
+    assert np.allclose(r["energies"], E, rtol=1e-12, atol=1e-14)
+    assert np.allclose(r["rsum"], Rs, rtol=1e-9, atol=1e-13)
+    assert np.allclose(r["T"].cpu().numpy(), T, rtol=0, atol=1e-13)
+    assert abs(r["energies"][-1] - e_can) < 1e-10

@@ -171,3 +171,58 @@ def test_rimp2_gradient_finite_differences(c_os, c_ss):
         ep[a] += h
         em[a] -= h
         assert np.isclose(dev[a], (energy(B, eo, ep) - energy(B, eo, em)) / (2 * h), rtol=1e-6, atol=1e-10)
+
+
+def test_non_canonical_mp2_reference_loops_small():
+    """Vectorised restatement against the reference's explicit loops
+    (loc_mp2.cc:24-90) on a tiny non-canonical system, 3 iterations."""
+    N, o = 5, 2
+    v = N - o
+    rng = np.random.default_rng(3)
+    prec = O.motrans(S.orthonormal(N, seed=4), S.eri_like(N, seed=5))
+    F = np.diag(np.concatenate([[-1.2, -0.7], [0.3, 0.8, 1.4]]))
+    F[0, 1] = F[1, 0] = 0.05
+    E, Rs, _ = O.non_canonical_mp2(prec, F, o, iters=3)
+    K = lambda i, j, a, b: prec[i, a + o, j, b + o]   # noqa: E731
+    Dn = lambda i, j, a, b: F[o + a, o + a] + F[o + b, o + b] - F[i, i] - F[j, j]   # noqa: E731
+    T = np.zeros((o, o, v, v))
+    for i, j, a, b in np.ndindex(o, o, v, v):
+        T[i, j, a, b] = -K(i, j, a, b) / Dn(i, j, a, b)
+    for it in range(3):
+        G = np.zeros_like(T)
+        for i, j, a, b in np.ndindex(o, o, v, v):
+            for k in range(o):
+                if k != i:
+                    G[i, j, a, b] += -F[i, k] * T[k, j, a, b]
+        R = np.zeros_like(T)
+        rsum = 0.0
+        for i, j, a, b in np.ndindex(o, o, v, v):
+            R[i, j, a, b] = K(i, j, a, b) + Dn(i, j, a, b) * T[i, j, a, b] + G[i, j, a, b] + G[j, i, b, a]
+            rsum += abs(R[i, j, a, b])
+        e = sum((K(i, j, a, b) + R[i, j, a, b]) * (2 * T[i, j, a, b] - T[i, j, b, a])
+                for i, j, a, b in np.ndindex(o, o, v, v))
+        for i, j, a, b in np.ndindex(o, o, v, v):
+            T[i, j, a, b] += -R[i, j, a, b] / Dn(i, j, a, b)
+        assert abs(E[it] - e) < 1e-14 and abs(Rs[it] - rsum) < 1e-13
+
+
+def test_non_canonical_mp2_canonical_limit_and_occupied_invariance():
+    """Diagonal Fock: iteration 0 is canonical MP2 with Rsum = 0.  Rotating
+    the occupied orbitals makes the occupied Fock block non-diagonal; the
+    iterations converge back to the canonical energy."""
+    N, o = 10, 3
+    h, eri = S.model_hamiltonian(N, seed=21)
+    C, eps, F = S.rhf_orthonormal(h, eri, o)
+    prec = O.motrans(C, eri)
+    FMo = O.build_fmo(C, F)
+    e_can = O.canonical_mp2(prec, FMo, o)["EMP2"]
+    E, Rs, _ = O.non_canonical_mp2(prec, FMo, o, iters=2)
+    assert abs(E[0] - e_can) < 1e-13 and Rs[0] < 1e-12
+    U = S.orthonormal(o, seed=22)
+    C2 = C.copy()
+    C2[:o] = U @ C[:o]
+    prec2 = O.motrans(C2, eri)
+    FMo2 = O.build_fmo(C2, F)
+    assert np.abs(FMo2[:o, :o] - np.diag(np.diag(FMo2[:o, :o]))).max() > 1e-3
+    E2, Rs2, _ = O.non_canonical_mp2(prec2, FMo2, o, iters=20)
+    assert abs(E2[-1] - e_can) < 1e-10 and Rs2[-1] < Rs2[0] * 1e-6
