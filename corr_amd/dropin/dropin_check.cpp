@@ -5,7 +5,10 @@
 //   uint16 intnums[4*nrofint]; double J[naux*naux], Pmn[naux*N*N]
 // and prints the reference's result lines.  With --scf the file also holds
 // Smat[N*N] after Pmn; run_scf then starts from MOs and produces the MOs and
-// Fmat the post-HF steps use (main.cc:111-121).
+// Fmat the post-HF steps use (main.cc:111-121), and build_Pmat/build_Fmat
+// rebuild the Fock matrix from the final MOs ("Fcheck:" line: its trace and
+// sum of squares).  With --ncmp2, run_non_canonical_mp2 (loc_mp2.cc, not
+// called by main.cc) runs after MOtrans.
 #include "../../include/corr_dropin.h"
 
 #include <cstdint>
@@ -25,9 +28,18 @@ static void rd(std::ifstream& f, T* p, int64_t n) {
 }
 
 int main(int argc, char** argv) {
-    const bool scf = argc == 3 && std::string(argv[2]) == "--scf";
-    if (argc != 2 && !scf) {
-        std::cerr << "usage: corr_dropin_check INPUT [--scf]" << std::endl;
+    bool scf = false, ncmp2 = false;
+    for (int a = 2; a < argc; ++a) {
+        const std::string f(argv[a]);
+        if (f == "--scf") scf = true;
+        else if (f == "--ncmp2") ncmp2 = true;
+        else {
+            std::cerr << "usage: corr_dropin_check INPUT [--scf] [--ncmp2]" << std::endl;
+            return 1;
+        }
+    }
+    if (argc < 2) {
+        std::cerr << "usage: corr_dropin_check INPUT [--scf] [--ncmp2]" << std::endl;
         return 1;
     }
     std::ifstream f(argv[1], std::ios::binary);
@@ -69,10 +81,21 @@ int main(int argc, char** argv) {
     pHF* post = new pHF;
     sys->scfiter = 100;
     std::cout << std::fixed << std::setprecision(10);   // the reference's stream state after SCF
-    if (scf) run_scf(sys, one, two);
+    if (scf) {
+        run_scf(sys, one, two);
+        std::vector<double> P(N * N, 0.0), F2(N * N);
+        build_Pmat(MOs.data(), (int)N, (int)nroe, P.data(), 0.0);
+        long long sc[4] = {two->sortcount[0], two->sortcount[1], two->sortcount[2], two->sortcount[3]};
+        build_Fmat((int)N, F2.data(), P.data(), Hmat.data(), intval.data(), intnums.data(), sc, nint);
+        double tr = 0.0, ss = 0.0;
+        for (int64_t i = 0; i < N; ++i) tr += F2[i * N + i];
+        for (double x : F2) ss += x * x;
+        std::cout << "Fcheck: " << tr << " " << ss << "\n";
+    }
     build_FMo(sys, one, post);
     MOtrans(sys, one, two, post);
     run_canonical_mp2(sys, post);
+    if (ncmp2) run_non_canonical_mp2((int)N, (int)nroe, post->prec_ints, post->FMo);
     post->BPQ = new double[naux * N * N];
     calculate_ri_arith((int)naux, (int)N, J.data(), Pmn.data(), post->BPQ);
     post->Bia = new double[naux * (nroe / 2) * (N - nroe / 2)];

@@ -113,3 +113,35 @@ def test_dropin_scf_then_post_hf_matches_oracle(tmp_path):
     cc = CC.ccsd_ur(prec, C, H, 2 * nocc_sp, ion, maxiter=20)
     it = [float(x) for x in re.findall(r"E\(CCSD\)= (-?\d+\.\d+)", out)]
     assert np.allclose(it, cc["energies"], atol=2e-10)
+
+
+def test_dropin_scf_fock_and_non_canonical_mp2(tmp_path):
+    """build_Pmat/build_Fmat drop-ins rebuild F from the final MOs of run_scf
+    ("Fcheck:" trace and sum of squares vs the oracle's Fock matrix), and
+    run_non_canonical_mp2 prints the oracle's 20 E(sem-loc)/Rsum lines."""
+    from oracle import scf as SCF
+    N, nocc_sp, naux = 8, 2, 24
+    H, Sm, eri, *_ = S.nonorthogonal_system(N, seed=43, nocc=nocc_sp)
+    C0, _ = SCF.check_initial_guess(np.zeros((N, N)), H, SCF.calc_s12(Sm))
+    val, idx, sc = O.sorted_unique_list(eri)
+    J = S.metric(naux, seed=7, cond=1e3)
+    X = np.random.default_rng(8).standard_normal((naux, N, N)) * 0.1
+    inp = tmp_path / "scf2.bin"
+    write_input(inp, N, 2 * nocc_sp, naux, 1.0, C0, np.zeros((N, N)), H, val, idx, sc, J, X + X.transpose(0, 2, 1))
+    with open(inp, "ab") as f:
+        np.ascontiguousarray(Sm, dtype=np.float64).tofile(f)
+    print("running corr_dropin_check --scf --ncmp2", flush=True)
+    res = subprocess.run([str(ROOT / "corr_amd" / "corr_dropin_check"), str(inp), "--scf", "--ncmp2"],
+                         capture_output=True, text=True, timeout=120)
+    sys.stdout.write(res.stdout[-2000:])
+    assert res.returncode == 0, res.stderr
+    out = res.stdout
+    ref = SCF.run_scf(H, Sm, lambda P: SCF.fock_dense(eri, P, H), C0, 2 * nocc_sp, ion_rep=1.0)
+    P = SCF.build_pmat(ref["C"], 2 * nocc_sp, None, 0.0)
+    F = SCF.fock_dense(eri, P, H)
+    tr, ss = (float(x) for x in re.search(r"Fcheck: (\S+) (\S+)", out).groups())
+    assert abs(tr - np.trace(F)) < 2e-9 and abs(ss - np.sum(F * F)) < 2e-8
+    prec = O.motrans(ref["C"], eri)
+    E, Rs, _ = O.non_canonical_mp2(prec, O.build_fmo(ref["C"], ref["F"]), nocc_sp, 20)
+    got = [float(x) for x in re.findall(r"E\(sem-loc\):\s+(-?\d+\.\d+)", out)]
+    assert len(got) == 20 and np.allclose(got, E, atol=2e-10)

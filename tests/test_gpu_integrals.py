@@ -113,3 +113,19 @@ def test_h2o32_three_center_spot_checks(gi):
     assert torch.equal(blk, blk.transpose(1, 2))
     del P3
     torch.cuda.empty_cache()
+
+
+def test_g_aux_shells_against_oracle(gi):
+    """Aux shells up to g (l = 4, as in JK-fitting sets): (P|Q) and (P|mu nu)
+    for every (LX, LY) class with LX = 4, on a water with def2-SVP."""
+    say("integrals g aux shells")
+    mol = mono(5)
+    aux = {"O": [(4, np.array([1.9]), np.array([1.0])), (3, np.array([0.8, 2.6]), np.array([0.4, 0.7])),
+                 (0, np.array([3.1]), np.array([1.0]))],
+           "H": [(4, np.array([1.1]), np.array([1.0])), (2, np.array([0.9]), np.array([1.0]))]}
+    obs = B.ShellArrays(mol, B.read_g94("def2-svp"))
+    A = B.ShellArrays(mol, aux)
+    sh = oracle_shells(mol, "def2-svp")
+    ax = I.build_shells(list(zip(mol.symbols, mol.coords)), aux)
+    assert np.abs(gi.metric(A).cpu().numpy() - I.coulomb2(ax)).max() < 1e-12
+    assert np.abs(gi.three_center(A, obs).cpu().numpy() - I.coulomb3(ax, sh)).max() < 1e-12
