@@ -25,11 +25,13 @@ RHF (DIIS, |dE| <= 1e-10) whose orbitals and orbital energies feed the step.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--suite]
 
 --suite adds a "suite" object with the other rows of the hot path, each at its
-own shape: build_FMo (N=768), AO unfold + 4-index transform + canonical and
-non-canonical MP2 ((H2O)8, N=192), metric and B formation ((H2O)32), RI-MP2 gradient ((H2O)16),
-CCSD per iteration ((H2O)4) and the RHF input side (Fock build and one
-RHF+DIIS iteration at (H2O)8).  --rows NAME,... restricts the suite to the
-named rows.
+own shape and on real chemistry: build_FMo and metric + B formation on the
+(H2O)32 inputs above; AO unfold (of the unique list gathered from the exact
+(mu nu|la si) tensor), 4-index transform, canonical and non-canonical MP2, the
+Fock build and one RHF+DIIS iteration on the exact-integral RHF of (H2O)8
+(N=192); the RI-MP2 gradient on the RI-JK RHF of (H2O)16; CCSD per iteration
+and converged (FP64 and FP32) on the exact-integral RHF of (H2O)4.
+--rows NAME,... restricts the suite to the named rows.
 """
 from __future__ import annotations
 
@@ -139,7 +141,7 @@ def main() -> int:
     N, o, v, naux = sizes(args.nmol)
     W_E, W_B = work(N, o, v, naux)
     log(f"(H2O){args.nmol}: N={N} o={o} v={v} naux={naux}; W_E={W_E/1e12:.3f} TF W_B={W_B/1e12:.3f} TF")
-    inputs = {}
+    inputs, real = {}, None
     if args.data == "real":
         from corr_amd import chem
         t_in = time.perf_counter()
@@ -153,6 +155,7 @@ def main() -> int:
                   "rhf_converged": bool(scf["converged"]), "homo_lumo_Eh": float(ev[0] - eo[-1]),
                   "seconds_integrals": round(r["seconds_integrals"], 2), "seconds_rhf": round(r["seconds_scf"], 2),
                   "seconds_setup_total": round(time.perf_counter() - t_in, 2)}
+        real = r
         del r
         torch.cuda.empty_cache()
         log(f"real inputs ready: E(RHF) = {scf['e_total']:.10f} ({scf['iterations']} iterations)")
@@ -237,9 +240,10 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     if args.suite:
-        del C, J, Pmn
+        keep = {"C": C, "J": J, "Pmn": Pmn, "F": real["F"] if real is not None else None, "nocc": o}
+        del C, J, Pmn, real
         torch.cuda.empty_cache()
-        line["suite"] = run_suite(dev, args.seed, [r for r in args.rows.split(",") if r])
+        line["suite"] = run_suite(dev, args.seed, [r for r in args.rows.split(",") if r], keep)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -264,10 +268,11 @@ def _timed(fn, reps: int = 3):
     return float(np.median(ts))
 
 
-def unique_list_gpu(N: int, device, seed: int):
+def unique_list_gpu(N: int, device, seed: int, dense=None):
     """Canonical AO ERI quadruples (a>=b, c>=d, ab>=cd) in the reference's
     stored form: resort step 1 (ops_rhf.cc:766-818) and the five permutation
-    classes (ops_rhf.cc:820-896), with seeded random values.  Returns
+    classes (ops_rhf.cc:820-896).  Values are the entries of `dense` (the
+    molecule's (mu nu|la si)) or seeded random numbers.  Returns
     (intval, intnums int16 [n,4], sortcount)."""
     import torch
     P = N * (N + 1) // 2
@@ -312,15 +317,25 @@ def unique_list_gpu(N: int, device, seed: int):
     sortcount = [int((cls <= j).sum()) for j in range(4)]
     nums = torch.stack([a, b, c, d], dim=1)[order].to(torch.int16)
     del a, b, c, d, cls, order
-    g = torch.Generator(device=device)
-    g.manual_seed(seed)
-    val = torch.randn(T, dtype=torch.float64, device=device, generator=g) * 1e-2
+    if dense is not None:
+        q = nums.to(torch.int64)
+        val = dense[q[:, 0], q[:, 1], q[:, 2], q[:, 3]].contiguous()
+        del q
+    else:
+        g = torch.Generator(device=device)
+        g.manual_seed(seed)
+        val = torch.randn(T, dtype=torch.float64, device=device, generator=g) * 1e-2
     return val, nums.contiguous(), sortcount
 
 
-def run_suite(dev, seed: int, rows=()) -> dict:
+def run_suite(dev, seed: int, rows=(), main=None) -> dict:
+    """Every other row at its own shape, on real chemistry computed on the GPU
+    (seeded water clusters, def2-SVP / def2-SVP-RI, exact-integral RHF up to
+    (H2O)8, RI-JK RHF above).  `main` holds the headline (H2O)32 inputs."""
     import torch
     import corr_amd
+    from corr_amd import chem
+    from corr_amd._abi import Context
     out = {"peaks_TFLOPs": {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}}
 
     def rec(name, fn):
@@ -335,61 +350,108 @@ def run_suite(dev, seed: int, rows=()) -> dict:
         torch.cuda.empty_cache()
 
     rng = np.random.default_rng(seed)
+    ctx = Context.get(dev.index if dev.index is not None else torch.cuda.current_device())
 
-    # a1: FMo = C F C^T at N = 768
+    # a1: FMo = C F C^T at N = 768 (the headline inputs)
     def fmo():
-        N = 768
-        Cm = random_orthogonal(N, rng, dev)
-        F = torch.randn(N, N, dtype=torch.float64, device=dev)
+        if main is not None and main.get("F") is not None:
+            Cm, F, src = main["C"], main["F"], "real (H2O)32 RHF"
+        else:
+            Cm, F, src = random_orthogonal(768, rng, dev), torch.randn(768, 768, dtype=torch.float64, device=dev), \
+                "synthetic"
+        N = Cm.shape[0]
         t = _timed(lambda: corr_amd.build_FMo(Cm, F))
-        return {"shape": "N=768", "ms": round(t * 1e3, 3), "TFLOPs": round(4 * N ** 3 / t / 1e12, 2)}
+        return {"shape": f"N={N}", "inputs": src, "ms": round(t * 1e3, 3), "TFLOPs": round(4 * N ** 3 / t / 1e12, 2)}
     rec("build_fmo_h2o32", fmo)
 
-    # a2-a4, a6: unfold + 4-index transform + canonical MP2 at (H2O)8, N = 192, o = 40
+    # (H2O)8: exact integrals and RHF, then unfold + 4-index + canonical and non-canonical MP2 + Fock build
+    h2o8 = {}
+
     def fourindex():
-        N, o = 192, 40
-        val, nums, sc = unique_list_gpu(N, dev, seed)
+        r = chem.water_cluster_rhf(8, seed, jk="dense", log=log)
+        h2o8.update(r)
+        N, o = r["nbf"], r["nocc"]
+        G = r["G"]
+        val, nums, sc = unique_list_gpu(N, dev, seed, dense=G)
         n_unique = val.numel()
         t_unf = _timed(lambda: corr_amd.ao_unfold(N, val, nums, sc), reps=2)
         dense = corr_amd.ao_unfold(N, val, nums, sc)
+        # the engine computes (ab|cd) and (cd|ab) separately: equal to rounding, not bitwise
+        unf_diff = float((dense - G).abs().max().item())
         nbytes_unf = n_unique * 16 + dense.numel() * 8
-        del val, nums
+        del val, nums, dense
         torch.cuda.empty_cache()
-        Cm = random_orthogonal(N, rng, dev)
+        Cm = r["C"]
+        FMo = corr_amd.build_FMo(Cm, r["F"])
         W4 = 8.0 * N ** 5
-        res = {"shape": "N=192 (H2O)8", "unique_integrals": n_unique,
-               "unfold_ms": round(t_unf * 1e3, 2),
-               "unfold_GBs": round(nbytes_unf / t_unf / 1e9, 1), "W_4_TFLOP": round(W4 / 1e12, 3)}
+        res = {"shape": "(H2O)8 N=192 o=40, real integrals", "e_rhf": r["scf"]["e_total"],
+               "rhf_iterations": r["scf"]["iterations"], "unique_integrals": n_unique,
+               "unfold_ms": round(t_unf * 1e3, 2), "unfold_GBs": round(nbytes_unf / t_unf / 1e9, 1),
+               "unfold_vs_tensor_max_abs": unf_diff, "W_4_TFLOP": round(W4 / 1e12, 3)}
+        emp2 = {}
         for prec in ("fp64", "fp32"):
-            t = _timed(lambda: corr_amd.MOtrans(Cm, dense, prec), reps=2)
+            t = _timed(lambda: corr_amd.MOtrans(Cm, G, prec), reps=2)
             res[f"motrans_{prec}_ms"] = round(t * 1e3, 2)
             res[f"motrans_{prec}_TFLOPs"] = round(W4 / t / 1e12, 2)
-        ints = corr_amd.MOtrans(Cm, dense, "fp64")
-        del dense
-        torch.cuda.empty_cache()
-        eo, ev = water_spectrum(8, rng)
-        FMo = torch.diag(torch.from_numpy(np.concatenate([eo, ev])).to(dev)).contiguous()
-        t = _timed(lambda: corr_amd.run_canonical_mp2(ints, FMo, o))
-        res["canonical_mp2_ms"] = round(t * 1e3, 3)
-        res["canonical_mp2_GBs"] = round(2 * o * o * (N - o) ** 2 * 8 / t / 1e9, 1)
-        # non-canonical MP2 (loc_mp2.cc): a rotated occupied block makes FMo non-diagonal
-        Fr = FMo.clone()
-        Fr[:o, :o] += 0.01 * torch.ones(o, o, dtype=torch.float64, device=dev)
-        t1 = _timed(lambda: corr_amd.run_non_canonical_mp2(ints, Fr, o, 1), reps=2)
-        t21 = _timed(lambda: corr_amd.run_non_canonical_mp2(ints, Fr, o, 21), reps=2)
-        it = (t21 - t1) / 20
-        ov2 = o * o * (N - o) ** 2
-        res["non_canonical_mp2_iteration_ms"] = round(it * 1e3, 3)
-        # per iteration: G GEMM reads T, writes G; the step reads K, T, G and both transposes, writes T
-        res["non_canonical_mp2_GBs"] = round(8 * ov2 * 8 / it / 1e9, 1)
+            ints = corr_amd.MOtrans(Cm, G, prec)
+            emp2[prec] = corr_amd.run_canonical_mp2(ints, FMo, o)["EMP2"]
+            if prec == "fp64":
+                t = _timed(lambda: corr_amd.run_canonical_mp2(ints, FMo, o))
+                res["canonical_mp2_ms"] = round(t * 1e3, 3)
+                res["canonical_mp2_GBs"] = round(2 * o * o * (N - o) ** 2 * 8 / t / 1e9, 1)
+                t1 = _timed(lambda: corr_amd.run_non_canonical_mp2(ints, FMo, o, 1), reps=2)
+                t21 = _timed(lambda: corr_amd.run_non_canonical_mp2(ints, FMo, o, 21), reps=2)
+                it = (t21 - t1) / 20
+                ov2 = o * o * (N - o) ** 2
+                res["non_canonical_mp2_iteration_ms"] = round(it * 1e3, 3)
+                # per iteration: G GEMM reads T, writes G; the step reads K, T, G and both transposes, writes T
+                res["non_canonical_mp2_GBs"] = round(8 * ov2 * 8 / it / 1e9, 1)
+            del ints
+            torch.cuda.empty_cache()
+        res["emp2_fp64"] = emp2["fp64"]
+        res["emp2_fp32_error_uEh"] = round(abs(emp2["fp32"] - emp2["fp64"]) * 1e6, 4)
         return res
     rec("fourindex_h2o8", fourindex)
 
-    # a7-a8: metric and B formation at (H2O)32
+    # RHF input side at (H2O)8: Fock build from the dense tensor (HBM-bound:
+    # slabs p >= q read once) and one RHF + DIIS iteration
+    def rhf():
+        r = h2o8 if h2o8 else chem.water_cluster_rhf(8, seed, jk="dense", log=log)
+        N, nroe = r["nbf"], r["nroe"]
+        G, H, S = r["G"], r["H"], r["S"]
+        P = corr_amd.build_Pmat(r["C"], nroe)
+        ctx.enable_timing(True)
+        ctx.kernel_time(None)
+        t = _timed(lambda: corr_amd.build_Fmat(G, P, H), reps=5)
+        ms, n = ctx.kernel_time("fock_slab_kernel")
+        ctx.enable_timing(False)
+        slab_bytes = N * (N + 1) // 2 * N * N * 8
+        res = {"shape": "(H2O)8 N=192 nroe=80, real integrals", "fock_ms": round(t * 1e3, 3),
+               "fock_slab_kernel_ms": round(ms / max(n, 1), 3),
+               "fock_slab_bytes_GB": round(slab_bytes / 1e9, 3),
+               "fock_slab_GBs": round(slab_bytes / (ms / max(n, 1) * 1e-3) / 1e9, 1)}
+        C0 = r["C"]
+        t1 = _timed(lambda: corr_amd.run_scf(H, S, G, C0, nroe, maxiter=1, e_tol=0.0, core_guess_check=False),
+                    reps=2)
+        t9 = _timed(lambda: corr_amd.run_scf(H, S, G, C0, nroe, maxiter=9, e_tol=0.0, core_guess_check=False),
+                    reps=2)
+        res["rhf_iteration_ms"] = round((t9 - t1) / 8 * 1e3, 2)
+        res["rhf_setup_ms"] = round((t1 - (t9 - t1) / 8) * 1e3, 2)
+        return res
+    rec("rhf_h2o8", rhf)
+    h2o8.clear()
+    torch.cuda.empty_cache()
+
+    # a7-a8: metric and B formation at (H2O)32 (the headline inputs)
     def bform():
-        C, J, Pmn, eo, ev = make_inputs(32, seed, dev)
+        if main is not None:
+            C, J, Pmn = main["C"], main["J"], main["Pmn"]
+            src = "real (H2O)32"
+        else:
+            C, J, Pmn, _, _ = make_inputs(32, seed, dev)
+            src = "synthetic"
         N, o, v, naux = sizes(32)
-        res = {"shape": "(H2O)32 N=768 naux=2432"}
+        res = {"shape": "(H2O)32 N=768 naux=2432", "inputs": src}
         res["metric_eigen_ms"] = round(_timed(lambda: corr_amd.ri_metric(J, "eigen")) * 1e3, 2)
         res["metric_cholesky_ms"] = round(_timed(lambda: corr_amd.ri_metric(J, "cholesky")) * 1e3, 2)
         M, _ = corr_amd.ri_metric(J, "cholesky")
@@ -403,87 +465,59 @@ def run_suite(dev, seed: int, rows=()) -> dict:
         return res
     rec("b_formation_h2o32", bform)
 
-    # new row: RI-MP2 gradient at (H2O)16
+    # RI-MP2 gradient at (H2O)16 on its RI-JK RHF
     def grad():
-        N, o, v, naux = sizes(16)
-        g = torch.Generator(device=dev)
-        g.manual_seed(seed)
-        B = 0.02 * torch.randn(naux, o, v, dtype=torch.float64, device=dev, generator=g)
-        eo, ev = water_spectrum(16, rng)
-        eo, ev = torch.from_numpy(eo).to(dev), torch.from_numpy(ev).to(dev)
+        r = chem.water_cluster_rhf(16, seed, log=log)
+        N, o, v, naux = r["nbf"], r["nocc"], r["nbf"] - r["nocc"], r["naux"]
+        B, _ = corr_amd.ri_bia_from_j(r["C"], r["Pmn"], r["J"], o, "cholesky", "fp64")
+        eo, ev = r["eps_occ"], r["eps_vir"]
+        del r
+        torch.cuda.empty_cache()
         WE = work(N, o, v, naux)[0]
-        res = {"shape": "(H2O)16 o=80 v=304 naux=1216", "W_E_TFLOP": round(WE / 1e12, 4)}
+        res = {"shape": "(H2O)16 o=80 v=304 naux=1216, real B", "W_E_TFLOP": round(WE / 1e12, 4)}
+        grads = {}
         for prec in ("fp64", "fp32"):
             te = _timed(lambda: corr_amd.rimp2_energy(B, eo, ev, prec))
             tg = _timed(lambda: corr_amd.rimp2_gradient(B, eo, ev, 1.0, 1.0, prec))
             res[f"energy_{prec}_ms"] = round(te * 1e3, 2)
             res[f"gradient_{prec}_ms"] = round(tg * 1e3, 2)
             res[f"gradient_{prec}_TFLOPs"] = round(3 * WE / tg / 1e12, 2)
+            grads[prec] = corr_amd.rimp2_gradient(B, eo, ev, 1.0, 1.0, prec)
+        E64, dB64 = grads["fp64"][0], grads["fp64"][1]
+        E32, dB32 = grads["fp32"][0], grads["fp32"][1]
+        res["emp2_fp64"] = E64
+        res["fp32_energy_error_uEh"] = round(abs(E32 - E64) * 1e6, 4)
+        res["fp32_dE_dB_max_rel_error"] = float(((dB32 - dB64).abs().max() / dB64.abs().max()).item())
         return res
     rec("rimp2_gradient_h2o16", grad)
 
-    # a10-a19: CCSD at (H2O)4, N = 96, 40 electrons (spin orbitals: o = 40, v = 152)
+    # a10-a19: CCSD at (H2O)4 on its exact-integral RHF (spin orbitals: o = 40, v = 152)
     def ccsd():
-        N, nroe = 96, 40
+        r = chem.water_cluster_rhf(4, seed, jk="dense", log=log)
+        N, nroe = r["nbf"], r["nroe"]
         o, v = nroe, 2 * N - nroe
-        g = torch.Generator(device=dev)
-        g.manual_seed(seed)
-        h = torch.diag(torch.linspace(-2.0, 1.5, N, dtype=torch.float64, device=dev))
-        h = h + 0.02 * torch.randn(N, N, dtype=torch.float64, device=dev, generator=g)
-        h = 0.5 * (h + h.T)
-        X = torch.randn(2 * N, N, N, dtype=torch.float64, device=dev, generator=g) / N
-        X = 0.5 * (X + X.transpose(1, 2))
-        Xm = X.reshape(2 * N, N * N)
-        eri = (0.05 * (Xm.T @ Xm)).reshape(N, N, N, N).contiguous()
-        del X, Xm
-        Cm = torch.eye(N, dtype=torch.float64, device=dev)
+        prec_ints = corr_amd.MOtrans(r["C"], r["G"], "fp64")
+        Cm, h, ion = r["C"], r["H"], r["mol"].nuclear_repulsion()
         W = 4.0 * o * o * v ** 4 + 4.0 * o ** 3 * v ** 3 + 4.0 * o ** 4 * v * v
-        res = {"shape": "(H2O)4 N=96 spin-orbital o=40 v=152", "W_CC_TFLOP_per_iter": round(W / 1e12, 3)}
+        res = {"shape": "(H2O)4 N=96 spin-orbital o=40 v=152, real integrals",
+               "e_rhf": r["scf"]["e_total"], "W_CC_TFLOP_per_iter": round(W / 1e12, 3)}
+        del r
+        energies = {}
         for prec in ("fp64", "fp32"):
-            t1 = _timed(lambda: corr_amd.ccsd_ur(eri, Cm, h, nroe, 0.0, maxiter=1, prec=prec), reps=1)
-            t11 = _timed(lambda: corr_amd.ccsd_ur(eri, Cm, h, nroe, 0.0, maxiter=11, prec=prec), reps=1)
+            t1 = _timed(lambda: corr_amd.ccsd_ur(prec_ints, Cm, h, nroe, ion, maxiter=1, prec=prec), reps=1)
+            t11 = _timed(lambda: corr_amd.ccsd_ur(prec_ints, Cm, h, nroe, ion, maxiter=11, prec=prec), reps=1)
             it = (t11 - t1) / 10
             res[f"iteration_{prec}_ms"] = round(it * 1e3, 2)
             res[f"iteration_{prec}_TFLOPs"] = round(W / it / 1e12, 2)
             res[f"setup_{prec}_ms"] = round((t1 - it) * 1e3, 1)
-        r = corr_amd.ccsd_ur(eri, Cm, h, nroe, 0.0, maxiter=20)
-        res["e_ccsd_iter20"] = r["e_ccsd"]
+            c = corr_amd.ccsd_ur(prec_ints, Cm, h, nroe, ion, maxiter=100, mode="diis", prec=prec, e_tol=1e-9,
+                                 r_tol=1e-7)
+            energies[prec] = c["e_ccsd"]
+            res[f"converged_{prec}_iterations"] = c["iterations"]
+        res["e_ccsd_fp64"] = energies["fp64"]
+        res["fp32_error_uEh"] = round(abs(energies["fp32"] - energies["fp64"]) * 1e6, 4)
         return res
     rec("ccsd_h2o4", ccsd)
-
-    # RHF input side at (H2O)8, N = 192, 80 electrons: Fock build from the
-    # dense tensor (HBM-bound: slabs p >= q read once) and one RHF+DIIS iteration
-    def rhf():
-        from corr_amd._abi import Context
-        N, nroe = 192, 80
-        val, nums, sc = unique_list_gpu(N, dev, seed)
-        dense = corr_amd.ao_unfold(N, val, nums, sc)
-        del val, nums
-        torch.cuda.empty_cache()
-        h = torch.diag(torch.cat([torch.linspace(-2.0, -1.0, nroe // 2), torch.linspace(0.5, 2.5, N - nroe // 2)]))
-        H = h.to(dtype=torch.float64, device=dev).contiguous()
-        S = torch.eye(N, dtype=torch.float64, device=dev)
-        P = corr_amd.build_Pmat(random_orthogonal(N, rng, dev), nroe)
-        ctx = Context.get(dev.index if dev.index is not None else torch.cuda.current_device())
-        ctx.enable_timing(True)
-        ctx.kernel_time(None)
-        t = _timed(lambda: corr_amd.build_Fmat(dense, P, H), reps=5)
-        ms, n = ctx.kernel_time("fock_slab_kernel")
-        ctx.enable_timing(False)
-        slab_bytes = N * (N + 1) // 2 * N * N * 8
-        res = {"shape": "(H2O)8 N=192 nroe=80", "fock_ms": round(t * 1e3, 3),
-               "fock_slab_kernel_ms": round(ms / max(n, 1), 3),
-               "fock_slab_bytes_GB": round(slab_bytes / 1e9, 3),
-               "fock_slab_GBs": round(slab_bytes / (ms / max(n, 1) * 1e-3) / 1e9, 1)}
-        C0 = random_orthogonal(N, rng, dev)
-        t1 = _timed(lambda: corr_amd.run_scf(H, S, dense, C0, nroe, maxiter=1, e_tol=0.0,
-                                             core_guess_check=False), reps=2)
-        t9 = _timed(lambda: corr_amd.run_scf(H, S, dense, C0, nroe, maxiter=9, e_tol=0.0,
-                                             core_guess_check=False), reps=2)
-        res["rhf_iteration_ms"] = round((t9 - t1) / 8 * 1e3, 2)
-        res["rhf_setup_ms"] = round((t1 - (t9 - t1) / 8) * 1e3, 2)
-        return res
-    rec("rhf_h2o8", rhf)
     return out
 
 

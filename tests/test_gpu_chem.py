@@ -111,3 +111,60 @@ def test_h2o32_mixed_precision_error_real_chemistry():
     assert abs(emx - e64) <= 1e-6
     del r, B32
     torch.cuda.empty_cache()
+
+
+def test_single_water_plain_method():
+    """The plain method of the single-water configuration, all FP64 on the GPU:
+    H2O (O at the origin, O-H 0.958 A, 104.5 deg), def2-SVP, exact-integral
+    RHF + DIIS to 1e-10 Eh, canonical MP2 through the full 4-index transform,
+    RI-MP2 with def2-SVP-RI and the reference's eigen metric, spin-orbital
+    CCSD (48 spin orbitals, 10 occupied): the reference's 20 fixed iterations
+    and DIIS to 1e-10 Eh.  The oracle repeats every step from the oracle
+    integrals (SCF) and from the GPU's orbitals (post-HF, identical inputs)."""
+    say("chem single water plain method")
+    from corr_amd import chem
+    import corr_amd
+    from oracle import ccsd as CC
+    r = chem.single_water_rhf()
+    mol, o = r["mol"], r["nocc"]
+    assert (r["nbf"], o, r["naux"]) == (24, 5, 76)
+    # SCF against the oracle's own integrals and SCF
+    atoms = list(zip(mol.symbols, mol.coords))
+    sh = I.build_shells(atoms, I.parse_g94(f"{B.HERE}/def2-svp.g94"))
+    ax = I.build_shells(atoms, I.parse_g94(f"{B.HERE}/def2-svp-ri.g94"))
+    S, H = I.overlap(sh), I.kinetic(sh) + I.nuclear(sh, mol.charges, list(mol.coords))
+    G = I.coulomb4(sh)
+    C0 = SCF.check_initial_guess(np.zeros_like(S), H, SCF.calc_s12(S))[0]
+    ref = SCF.run_scf(H, S, lambda P: SCF.fock_dense(G, P, H), C0, 10, ion_rep=mol.nuclear_repulsion(),
+                      e_tol=1e-10, diis_thresh=1e9)
+    assert r["scf"]["converged"] == 1 and abs(r["scf"]["e_total"] - ref["total"]) < 1e-9
+    assert -75.97 < ref["total"] < -75.95
+    assert np.abs(r["G"].cpu().numpy() - G).max() < 1e-12
+    # post-HF on the GPU's orbitals, FP64
+    C, F = r["C"], r["F"]
+    FMo = corr_amd.build_FMo(C, F)
+    prec = corr_amd.MOtrans(C, r["G"], "fp64")
+    mp2 = corr_amd.run_canonical_mp2(prec, FMo, o)
+    BPQ = corr_amd.calculate_ri(r["J"], r["Pmn"], "eigen", "fp64")
+    ri = corr_amd.run_canonical_mp2_ri(corr_amd.transform_ri(C, BPQ, o, "fp64"), FMo, o, "fp64")
+    cc20 = corr_amd.ccsd_ur(prec, C, r["H"], 10, mol.nuclear_repulsion(), maxiter=20)
+    ccd = corr_amd.ccsd_ur(prec, C, r["H"], 10, mol.nuclear_repulsion(), maxiter=100, mode="diis", e_tol=1e-10,
+                           r_tol=1e-8)
+    Ch, Fh, Hh = C.cpu().numpy(), F.cpu().numpy(), r["H"].cpu().numpy()
+    FMo_h = O.build_fmo(Ch, Fh)
+    prec_h = O.motrans(Ch, G)
+    mp2_h = O.canonical_mp2(prec_h, FMo_h, o)
+    Bh = O.transform_ri(Ch, O.ri_bpq(O.ri_metric_eigen(I.coulomb2(ax))[0], I.coulomb3(ax, sh)), o)
+    e = np.diag(FMo_h)
+    ri_h = O.rimp2(Bh, e[:o], e[o:])
+    cc_h = CC.ccsd_ur(prec_h, Ch, Hh, 10, mol.nuclear_repulsion(), maxiter=20)
+    V = CC.ints_so(prec_h)
+    h, f = CC.form_fock(Ch, Hh, V, 10)
+    e_conv, *_ = CC.ccsd_converge(CC.Blocks(V, f, 10), tol=1e-12)
+    say(f"E(RHF) {r['scf']['e_total']:.10f} EMP2 {mp2['EMP2']:.10f} EMP2(RI) {ri['EMP2']:.10f} "
+        f"ECCSD(20) {cc20['energies'][-1]:.10f} ECCSD(conv) {ccd['e_ccsd']:.10f}")
+    assert abs(mp2["EMP2"] - mp2_h["EMP2"]) < 1e-10
+    assert abs(ri["EMP2"] - ri_h["EMP2"]) < 1e-10
+    assert np.allclose(cc20["energies"], cc_h["energies"], atol=1e-10, rtol=0)
+    assert ccd["converged"] and abs(ccd["e_ccsd"] - e_conv) < 1e-10
+    assert -0.22 < mp2["EMP2"] < -0.19 and abs(ri["EMP2"] - mp2["EMP2"]) < 2e-4
