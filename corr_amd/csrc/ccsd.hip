@@ -42,15 +42,29 @@ struct Perm4 {
 };
 
 // dst[i0][i1][i2][i3] = beta dst + alpha src[i0 s0 + i1 s1 + i2 s2 + i3 s3]
+// Grid (ceil(d2 d3 / 256), d1, d0): i0 and i1 come from the block, (i2, i3)
+// from one 32-bit division (d2 d3 < 2^31 is checked by the launcher).
 __global__ void permute4_kernel(double* __restrict__ dst, const double* __restrict__ src, Perm4 p,
                                 double alpha, double beta) {
-    const int64_t total = p.d[0] * p.d[1] * p.d[2] * p.d[3];
-    GRID_STRIDE(e, total) {
-        const int64_t i3 = e % p.d[3], i2 = (e / p.d[3]) % p.d[2];
-        const int64_t i1 = (e / (p.d[3] * p.d[2])) % p.d[1], i0 = e / (p.d[3] * p.d[2] * p.d[1]);
-        const double x = alpha * src[i0 * p.s[0] + i1 * p.s[1] + i2 * p.s[2] + i3 * p.s[3]];
-        dst[e] = (beta == 0.0) ? x : beta * dst[e] + x;
-    }
+    const uint32_t d3 = (uint32_t)p.d[3], n23 = (uint32_t)(p.d[2] * p.d[3]);
+    const uint32_t e23 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e23 >= n23) return;
+    const int64_t i0 = blockIdx.z, i1 = blockIdx.y;
+    const uint32_t i2 = e23 / d3, i3 = e23 - i2 * d3;
+    const double x = alpha * src[i0 * p.s[0] + i1 * p.s[1] + (int64_t)i2 * p.s[2] + (int64_t)i3 * p.s[3]];
+    const int64_t e = (i0 * p.d[1] + i1) * (int64_t)n23 + e23;
+    dst[e] = (beta == 0.0) ? x : beta * dst[e] + x;
+}
+
+// W[a][b][:] = V[a][b][:] - X[a][b][:] + X[b][a][:] over rows of length r:
+// <ab||ef> - P(ab) t_mb <am||ef> with X[a][b][ef] = sum_m t_mb <am||ef>.
+__global__ void wabef_kernel(double* __restrict__ W, const double* __restrict__ V, const double* __restrict__ X,
+                             int64_t v, int64_t r) {
+    const int64_t a = blockIdx.z, b = blockIdx.y;
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= r) return;
+    const int64_t e = (a * v + b) * r + c;
+    W[e] = V[e] - X[e] + X[(b * v + a) * r + c];
 }
 
 // Spin-orbital h from hs = C H C^T and f = h + sum_{j<nocc} <pj||qj>  (ccsd.cc:11-48)
@@ -185,7 +199,7 @@ struct CCSD {
     double *PeOOf, *Pmenf, *Pmine, *Pianf, *Pmbje, *Pabje;
     double *fo;   // f (n x n)
     // per-iteration work
-    double *tau_s, *tau, *tauS_a, *Fae, *Faep, *Fmi, *Fmip, *Fme, *Yk, *Wmnij, *Wabef, *Wmbej;
+    double *tau_s, *tau, *tauS_a, *Fae, *Faep, *Fmi, *Fmip, *Fme, *Yk, *Wmnij, *Wabef, *Xab, *Wmbej;
     double *Zp, *W2, *Wp, *T2p, *T2a, *X1, *X2, *Zr, *U, *Q, *Tmp1;
     double* partial;
 
@@ -196,9 +210,11 @@ struct CCSD {
         for (auto d : dims) p.d[k++] = d;
         k = 0;
         for (auto s : strides) p.s[k++] = s;
-        const int64_t total = p.d[0] * p.d[1] * p.d[2] * p.d[3];
-        hipLaunchKernelGGL(permute4_kernel, dim3(grid_for(total)), dim3(256), 0, ctx->stream, dst, src, p,
-                           alpha, beta);
+        const int64_t n23 = p.d[2] * p.d[3];
+        CORR_REQUIRE(ctx, n23 < ((int64_t)1 << 31) && p.d[1] <= 65535 && p.d[0] <= 65535, "ccsd: permute too large");
+        if (n23 == 0 || p.d[0] == 0 || p.d[1] == 0) return CORR_OK;
+        hipLaunchKernelGGL(permute4_kernel, dim3((unsigned)cdiv(n23, 256), (unsigned)p.d[1], (unsigned)p.d[0]),
+                           dim3(256), 0, ctx->stream, dst, src, p, alpha, beta);
         CORR_LAUNCHED(ctx, "permute4_kernel");
         return CORR_OK;
     }
@@ -272,6 +288,7 @@ struct CCSD {
         CORR_TRY(alloc(&Yk, "cc_Yk", o * v));
         CORR_TRY(alloc(&Wmnij, "cc_Wmnij", O4));
         CORR_TRY(alloc(&Wabef, "cc_Wabef", v * v * v * v));
+        CORR_TRY(alloc(&Xab, "cc_Xab", v * v * v * v));
         CORR_TRY(alloc(&Wmbej, "cc_Wmbej", O2V2));
         CORR_TRY(alloc(&Zp, "cc_Zp", O2V2));
         CORR_TRY(alloc(&W2, "cc_W2", O2V2));
@@ -334,9 +351,12 @@ struct CCSD {
         CORR_TRY(mm(0, 1, o, o, v, -1.0, T1, v, 0, Vooov, v, o * v, 1.0, Wmnij, o, o2, o2));
         CORR_TRY(mm(0, 1, o2, o2, v2, 0.25, Voovv, v2, 0, tau, v2, 0, 1.0, Wmnij, o2, 0));
         // ---- Wabef = <ab||ef> - P(ab) t_mb <am||ef> + 1/4 tau_mnab <mn||ef>
-        CORR_TRY(perm(Wabef, Vvvvv, {v, v, v, v}, {v * v2, v2, v, 1}));
-        CORR_TRY(mm(1, 0, v, v2, o, -1.0, T1, v, 0, Vvovv, v2, o * v2, 1.0, Wabef, v2, v * v2, v));
-        CORR_TRY(mm(1, 0, v, v2, o, 1.0, T1, v, 0, Vvovv, v2, o * v2, 1.0, Wabef, v * v2, v2, v));
+        // Xab[a][b][(ef)] = sum_m t_mb <am||ef> (one GEMM per a), then one pass
+        // Wabef = Vvvvv - Xab[a][b] + Xab[b][a], then the tau term
+        CORR_TRY(mm(1, 0, v, v2, o, 1.0, T1, v, 0, Vvovv, v2, o * v2, 0.0, Xab, v2, v * v2, v));
+        hipLaunchKernelGGL(wabef_kernel, dim3((unsigned)cdiv(v2, 256), (unsigned)v, (unsigned)v), dim3(256), 0,
+                           ctx->stream, Wabef, Vvvvv, Xab, v, v2);
+        CORR_LAUNCHED(ctx, "wabef_kernel");
         CORR_TRY(mm(1, 0, v2, v2, o2, 0.25, tau, v2, 0, Voovv, v2, 0, 1.0, Wabef, v2, 0));
         // ---- Wmbej = <mb||ej> + t_jf <mb||ef> - t_nb <mn||ej> - (1/2 t_jnfb + t_jf t_nb) <mn||ef>
         CORR_TRY(perm(Wmbej, Vovvo, {o, v, v, o}, {v2 * o, v * o, o, 1}));

@@ -318,13 +318,14 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
     CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
                  "gemm: unknown precision");
     const int f64 = (prec == CORR_PREC_FP64) ? 1 : 0;
-    // Split K when the output has too few tiles to fill the GPU and K is deep:
-    // chunks of depth kc write partial products to a workspace, then a
-    // fixed-order sum applies alpha and beta (deterministic).
+    // Split K when the output has fewer tiles than the 256 CUs and K is deep
+    // (matrix-vector products included): chunks of depth kc >= 512 write
+    // partial products to a workspace, then a fixed-order sum applies alpha
+    // and beta (deterministic).
     const int64_t tiles_est = cdiv(M, 128) * cdiv(N, 128) * batch;
-    if (batch == 1 && !a_lower && tiles_est < 128 && K >= 8192) {
+    if (batch == 1 && !a_lower && tiles_est < 256 && K >= 2048) {
         int64_t S = cdiv(512, tiles_est);
-        if (S > K / 1024) S = K / 1024;
+        if (S > K / 512) S = K / 512;
         const int64_t cap = (int64_t)(256 << 20) / (8 * M * N);   // workspace at most 256 MiB
         if (S > cap) S = cap;
         if (S >= 2) {
