@@ -232,14 +232,23 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
 
 template <int TA, int TB, typename T>
 static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
-    // Tile shape with the least padded area of C: 128 x 128, 128 x 192 (N = 192
+    // Tile shape with the least weighted padded area of C: 128 x 128, 128 x 192 (N = 192
     // of the (H2O)8 4-index quarters in one tile, so A is read once), 128 x 160
     // (o = 160 of (H2O)32) or 256 x 64 (narrow N); ties keep that order.
     const int BMs[4] = {128, 256, 128, 128}, BNs[4] = {128, 64, 160, 192};
     int cfg = 0;
     int64_t best = -1;
+    // Padded areas are weighted by the measured time per useful FLOP on
+    // MI355X at equal padding (FP64 and FP32, 6080^3 .. 23104^2 x 1600):
+    // 128 x 192 runs 12 waves, 3 per SIMD (1.00); 128 x 128 (1.06) and
+    // 256 x 64 (1.11) run 2 per SIMD; 128 x 160 runs 10 waves on 4 SIMDs
+    // (3, 3, 2, 2) and idles matrix cores (1.20).  Outputs with fewer than
+    // 512 tiles of 128 x 128 keep unweighted areas: there the tile count
+    // (filling the 256 CUs) matters more than per-tile speed.
+    const bool big = cdiv(a.M, 128) * cdiv(a.N, 128) * batch >= 512;
+    const int64_t weight[4] = {big ? 106 : 100, big ? 111 : 100, big ? 120 : 100, 100};
     for (int c : {0, 3, 2, 1}) {
-        const int64_t area = cdiv(a.M, BMs[c]) * BMs[c] * (cdiv(a.N, BNs[c]) * BNs[c]);
+        const int64_t area = cdiv(a.M, BMs[c]) * BMs[c] * (cdiv(a.N, BNs[c]) * BNs[c]) / 100 * weight[c];
         if (best < 0 || area < best) { best = area; cfg = c; }
     }
     const int BM = BMs[cfg], BN = BNs[cfg];

@@ -55,9 +55,11 @@ __global__ void unfold_kernel(int64_t N, int64_t nrofint, int64_t s0, int64_t s1
 // produces is a permutation of (a b | c d) that keeps the pairs {a,b}, {c,d}
 // and swaps or keeps them, so the dense tensor is D[p][q][r][s] =
 // V[pair(p,q)][pair(r,s)] with pair(x,y) = max(x,y)(max(x,y)+1)/2 + min(x,y) and
-// V the symmetric P x P pair matrix (P = N(N+1)/2).  Step 1 puts each list entry
-// at V[ab][cd] and V[cd][ab]; step 2 fills one dense row (p,q) per workgroup
-// with coalesced writes, reading row pair(p,q) of V (L2-resident).
+// V the symmetric P x P pair matrix (P = N(N+1)/2).  Step 1 puts each list
+// entry at V[max(ab,cd)][min(ab,cd)] (one write; consecutive for a sorted
+// list), step 2 mirrors the lower triangle into the upper one through LDS
+// tiles, step 3 fills one dense row (p,q) per workgroup with coalesced writes,
+// reading row pair(p,q) of V.
 __device__ inline int64_t sym_pair(int64_t x, int64_t y) {
     return x >= y ? x * (x + 1) / 2 + y : y * (y + 1) / 2 + x;
 }
@@ -70,40 +72,113 @@ __global__ void pair_scatter_kernel(int64_t N, int64_t nrofint, const double* __
         const int64_t a = idx[4 * x + 0], b = idx[4 * x + 1], c = idx[4 * x + 2], d = idx[4 * x + 3];
         if (a >= N || b >= N || c >= N || d >= N) continue;
         const int64_t u = sym_pair(a, b), w = sym_pair(c, d);
-        V[u * P + w] = val[x];
-        V[w * P + u] = val[x];
+        V[(u >= w ? u * P + w : w * P + u)] = val[x];
     }
 }
 
+// V[i][j] = V[j][i] for j > i: block (bi, bj), bj >= bi, reads the 64 x 64 tile
+// (bj, bi) of the lower triangle and writes its transpose into tile (bi, bj).
+__global__ __launch_bounds__(256) void pair_mirror_kernel(int64_t P, double* __restrict__ V) {
+    const int64_t bi = blockIdx.y, bj = blockIdx.x;
+    if (bj < bi) return;
+    __shared__ double t[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const int64_t row = bj * 64 + r, col = bi * 64 + tx;
+        t[r][tx] = (row < P && col < P) ? V[row * P + col] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int64_t row = bi * 64 + r, col = bj * 64 + tx;
+        if (row < P && col < P && col > row) V[row * P + col] = t[tx][r];
+    }
+}
+
+// Pair u = blockIdx.x = pair(p, q), p >= q: the N x N blocks D[p][q][.][.] and
+// D[q][p][.][.] are the same symmetric matrix, whose lower triangle is row u
+// of V packed (element (r, s), r >= s, at r(r+1)/2 + s); both are written from
+// one read of the row.  Lower and diagonal 32 x 32 tiles read the packed row
+// along s; an upper tile (r < s) is the transpose of a lower one, read along r
+// into LDS and written along s, so every global access is coalesced.
 __global__ __launch_bounds__(256) void pair_expand_kernel(int64_t N, const double* __restrict__ V,
                                                           double* __restrict__ dense) {
+    __shared__ double t[32][33];
     const int64_t P = N * (N + 1) / 2;
-    const int64_t row = blockIdx.x;   // p * N + q
-    const int64_t p = row / N, q = row % N;
-    const double* Vu = V + sym_pair(p, q) * P;
-    double* out = dense + row * N * N;
-    for (int64_t e = threadIdx.x; e < N * N; e += 256) out[e] = Vu[sym_pair(e / N, e % N)];
+    const int n = (int)N;
+    const int64_t u = blockIdx.x;
+    int64_t p = (int64_t)((sqrt(8.0 * (double)u + 1.0) - 1.0) * 0.5);
+    while (p * (p + 1) / 2 > u) --p;
+    while ((p + 1) * (p + 2) / 2 <= u) ++p;
+    const int64_t q = u - p * (p + 1) / 2;
+    const double* Vu = V + u * P;
+    double* out = dense + (p * N + q) * N * N;
+    double* out2 = dense + (q * N + p) * N * N;
+    const bool two = p != q;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    const int nt = (n + 31) / 32;
+    for (int R = 0; R < nt; ++R) {
+        for (int S = 0; S < nt; ++S) {
+            if (R >= S) {
+                for (int k = ty; k < 32; k += 8) {
+                    const int r = R * 32 + k, c = S * 32 + tx;
+                    if (r < n && c < n) {
+                        const double x = Vu[r >= c ? (int64_t)r * (r + 1) / 2 + c : (int64_t)c * (c + 1) / 2 + r];
+                        out[(int64_t)r * n + c] = x;
+                        if (two) out2[(int64_t)r * n + c] = x;
+                    }
+                }
+            } else {
+                for (int k = ty; k < 32; k += 8) {
+                    const int c = S * 32 + k, r = R * 32 + tx;
+                    t[k][tx] = (r < n && c < n) ? Vu[(int64_t)c * (c + 1) / 2 + r] : 0.0;
+                }
+                __syncthreads();
+                for (int k = ty; k < 32; k += 8) {
+                    const int r = R * 32 + k, c = S * 32 + tx;
+                    if (r < n && c < n) {
+                        out[(int64_t)r * n + c] = t[tx][k];
+                        if (two) out2[(int64_t)r * n + c] = t[tx][k];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
 }
 
-// (ia|jb) = prec_ints[((i*N + o+a)*N + j)*N + o+b]; one partial pair per block.
+// (ia|jb) = prec_ints[((i*N + o+a)*N + j)*N + o+b].  One workgroup per
+// (i, j, 32 x 32 tile of (a, b)): the tile of (ia|jb) is read along b and
+// the tile of (ib|ja) along a (its mirror, staged through LDS), so every read
+// is coalesced; one partial pair per workgroup.
 __global__ __launch_bounds__(256) void canonical_mp2_kernel(const double* __restrict__ I4,
                                                             const double* __restrict__ FMo,
-                                                            int64_t N, int64_t o,
+                                                            int64_t N, int64_t o, int nt,
                                                             double* __restrict__ partial) {
-    const int64_t v = N - o;
-    const int64_t total = o * o * v * v;
-    double sos = 0.0, sss = 0.0;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t b = e % v, a = (e / v) % v, j = (e / (v * v)) % o, i = e / (v * v * o);
-        const double iajb = I4[((i * N + o + a) * N + j) * N + o + b];
-        const double ibja = I4[((i * N + o + b) * N + j) * N + o + a];
-        const double D = FMo[(o + a) * N + o + a] + FMo[(o + b) * N + o + b] - FMo[i * N + i] -
-                         FMo[j * N + j];
-        sos -= iajb * iajb / D;
-        sss -= (iajb - ibja) * iajb / D;
-    }
+    __shared__ double tX[32][33];   // tX[b][a] = (ib|ja)
     __shared__ double sh[2][256];
+    const int64_t v = N - o;
+    const int64_t blk = blockIdx.x;
+    const int bt = (int)(blk % nt), at = (int)((blk / nt) % nt);
+    const int64_t j = (blk / ((int64_t)nt * nt)) % o, i = blk / ((int64_t)nt * nt * o);
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int64_t a0 = (int64_t)at * 32, b0 = (int64_t)bt * 32;
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t bb = b0 + r, aa = a0 + tx;
+        tX[r][tx] = (bb < v && aa < v) ? I4[((i * N + o + bb) * N + j) * N + o + aa] : 0.0;
+    }
+    __syncthreads();
+    const double fij = FMo[i * N + i] + FMo[j * N + j];
+    double sos = 0.0, sss = 0.0;
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t aa = a0 + r, bb = b0 + tx;
+        if (aa < v && bb < v) {
+            const double iajb = I4[((i * N + o + aa) * N + j) * N + o + bb];
+            const double ibja = tX[tx][r];
+            const double D = FMo[(o + aa) * N + o + aa] + FMo[(o + bb) * N + o + bb] - fij;
+            sos -= iajb * iajb / D;
+            sss -= (iajb - ibja) * iajb / D;
+        }
+    }
     sh[0][threadIdx.x] = sos;
     sh[1][threadIdx.x] = sss;
     __syncthreads();
@@ -115,8 +190,8 @@ __global__ __launch_bounds__(256) void canonical_mp2_kernel(const double* __rest
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x + 0] = sh[0][0];
-        partial[2 * blockIdx.x + 1] = sh[1][0];
+        partial[2 * blk + 0] = sh[0][0];
+        partial[2 * blk + 1] = sh[1][0];
     }
 }
 
@@ -153,8 +228,10 @@ int ao_unfold(corr_ctx* ctx, int64_t N, int64_t nrofint, const int64_t* sc, cons
                                val, idx, V);
             CORR_LAUNCHED(ctx, "pair_scatter_kernel");
         }
-        CORR_REQUIRE(ctx, N * N < ((int64_t)1 << 31), "ao_unfold: N too large");
-        hipLaunchKernelGGL(pair_expand_kernel, dim3((unsigned)(N * N)), dim3(256), 0, ctx->stream, N, V, dense);
+        const unsigned nb = (unsigned)cdiv(P, 64);   // P < 2^17 here
+        hipLaunchKernelGGL(pair_mirror_kernel, dim3(nb, nb), dim3(256), 0, ctx->stream, P, V);
+        CORR_LAUNCHED(ctx, "pair_mirror_kernel");
+        hipLaunchKernelGGL(pair_expand_kernel, dim3((unsigned)P), dim3(256), 0, ctx->stream, N, V, dense);
         CORR_LAUNCHED(ctx, "pair_expand_kernel");
         return CORR_OK;
     }
@@ -184,10 +261,14 @@ int motrans(corr_ctx* ctx, int64_t N, const double* C, const double* ao, double*
 int canonical_mp2(corr_ctx* ctx, int64_t N, int64_t o, const double* I4, const double* FMo,
                   corr_mp2_energy* out) {
     CORR_REQUIRE(ctx, N > 0 && o > 0 && o < N && I4 && FMo && out, "canonical_mp2: bad arguments");
-    const int nblk = 1024;
+    const int64_t v = N - o;
+    const int nt = (int)cdiv(v, 32);
+    const int64_t nblk = o * o * (int64_t)nt * nt;
+    CORR_REQUIRE(ctx, nblk < ((int64_t)1 << 31), "canonical_mp2: too many tiles");
     double* partial = nullptr;
     CORR_TRY(scratch(ctx, "cmp2_partial", sizeof(double) * 2 * nblk, (void**)&partial));
-    hipLaunchKernelGGL(canonical_mp2_kernel, dim3(nblk), dim3(256), 0, ctx->stream, I4, FMo, N, o, partial);
+    hipLaunchKernelGGL(canonical_mp2_kernel, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, I4, FMo, N, o, nt,
+                       partial);
     CORR_LAUNCHED(ctx, "canonical_mp2_kernel");
     double s[2];
     CORR_TRY(reduce_sum(ctx, partial, nblk, 2, s));

@@ -56,11 +56,12 @@ def test_gemm_fp32(corr, ta, tb, M, N, K):
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_gemm_split_k(corr, ta, tb, prec):
-    """Few output tiles and a deep K take the split-K path (chunks of K into a
-    workspace, fixed-order sum); the last chunk is ragged."""
-    M, N, K = 40, 152, 20000
-    _say(f"gemm split-K {prec} ta={ta} tb={tb}")
+@pytest.mark.parametrize("M,N,K", [(40, 152, 20000), (2000, 1, 3001), (1600, 1300, 2500)])
+def test_gemm_split_k(corr, ta, tb, prec, M, N, K):
+    """Fewer output tiles than CUs and K >= 2048 take the split-K path (chunks
+    of K into a workspace, fixed-order sum; the last chunk is ragged): a thin
+    deep product, a matrix-vector product and a medium output."""
+    _say(f"gemm split-K {prec} ta={ta} tb={tb} M={M} N={N} K={K}")
     rng = np.random.default_rng(11 + 2 * ta + tb)
     A = rng.standard_normal((K, M) if ta else (M, K))
     B = rng.standard_normal((N, K) if tb else (K, N))
@@ -70,6 +71,22 @@ def test_gemm_split_k(corr, ta, tb, prec):
     assert np.max(np.abs(out - ref)) < tol * np.sqrt(K) * 4
     again = corr.dgemm(dev(A), dev(B), bool(ta), bool(tb), alpha=-0.5, prec=prec).cpu().numpy()
     assert np.array_equal(out, again)   # deterministic
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gemm_large_output_tiles(corr, ta, tb, prec):
+    """An output of >= 512 tiles of 128 x 128 takes the weighted tile choice
+    (128 x 192, 12 waves); ragged in both M and N."""
+    M, N, K = 3001, 2999, 77
+    _say(f"gemm large output {prec} ta={ta} tb={tb}")
+    rng = np.random.default_rng(21 + ta)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    ref = (A.T if ta else A) @ (B.T if tb else B)
+    out = corr.dgemm(dev(A), dev(B), bool(ta), bool(tb), prec=prec).cpu().numpy()
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    assert np.max(np.abs(out - ref)) < tol * np.sqrt(K) * 4
 
 
 @pytest.mark.parametrize("entry", ["ri_metric", "ri_bia_from_j"])
@@ -232,6 +249,33 @@ def test_unfold_motrans_canonical_mp2(corr, N, o):
         assert abs(e[k] - e_ref[k]) < 1e-12, k
     mo32 = corr.MOtrans(dev(C), ao, prec="fp32").cpu().numpy()
     assert np.max(np.abs(mo32 - ref)) < 1e-5 * np.max(np.abs(ref))
+
+
+def test_unfold_shuffled_screened_list(corr):
+    """A list in random order, with a fifth of the entries screened out and a
+    quarter stored with the pairs swapped ((cd|ab) for (ab|cd)): every stored
+    value lands on its 8 images and the images of absent entries are zero
+    (P = 861 pairs, not a multiple of the 64-wide mirror tiles)."""
+    N = 41
+    _say(f"unfold shuffled/screened N={N}")
+    dense = S.eri_like(N, seed=5)
+    val, idx, sc = O.sorted_unique_list(dense)
+    rng = np.random.default_rng(6)
+    keep = rng.random(len(val)) > 0.2
+    val, idx = val[keep], idx[keep].astype(np.int64)
+    swap = rng.random(len(val)) < 0.25
+    idx[swap] = idx[swap][:, [2, 3, 0, 1]]
+    order = rng.permutation(len(val))
+    val, idx = val[order], idx[order]
+    ref = np.zeros_like(dense)
+    a, b, c, d = idx.T
+    for p_, q_, r_, s_ in [(a, b, c, d), (b, a, c, d), (a, b, d, c), (b, a, d, c),
+                           (c, d, a, b), (d, c, a, b), (c, d, b, a), (d, c, b, a)]:
+        ref[p_, q_, r_, s_] = val
+    d_idx = torch.from_numpy(idx.astype(np.uint16).view(np.int16).copy()).to("cuda")
+    sc_all = np.array([0, 0, 0, 0], dtype=np.int64)
+    ao = corr.ao_unfold(N, dev(val), d_idx, sc_all)
+    assert torch.equal(ao.cpu(), torch.from_numpy(ref))
 
 
 @pytest.mark.parametrize("N,o,rotate", [(10, 3, False), (10, 3, True), (37, 9, True)])
