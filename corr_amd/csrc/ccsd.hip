@@ -56,6 +56,43 @@ __global__ void permute4_kernel(double* __restrict__ dst, const double* __restri
     dst[e] = (beta == 0.0) ? x : beta * dst[e] + x;
 }
 
+// The same permutation when the source is contiguous along destination index
+// k < 3 (s_k = 1) and not along i3: 32 x 32 tiles of (i_k, i3) are read along
+// i_k into LDS and written along i3, so both sides are coalesced.  Grid
+// (tiles of (i_k, i3), d_b, d_a) with a < b the two remaining indices.
+__global__ __launch_bounds__(256) void permute4_tiled_kernel(double* __restrict__ dst, const double* __restrict__ src,
+                                                             Perm4 p, int k, double alpha, double beta) {
+    __shared__ double t[32][33];
+    const int64_t dk = p.d[k], n3 = (p.d[3] + 31) / 32;
+    const int64_t tk = blockIdx.x / n3, t3 = blockIdx.x % n3;
+    const int64_t ia = blockIdx.z, ib = blockIdx.y;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    auto index = [&](int64_t ik, int64_t& i0, int64_t& i1, int64_t& i2) {
+        i0 = k == 0 ? ik : ia;
+        i1 = k == 0 ? ia : (k == 1 ? ik : ib);
+        i2 = k == 2 ? ik : ib;
+    };
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t ik = tk * 32 + tx, i3 = t3 * 32 + j;
+        if (ik < dk && i3 < p.d[3]) {
+            int64_t i0, i1, i2;
+            index(ik, i0, i1, i2);
+            t[j][tx] = src[i0 * p.s[0] + i1 * p.s[1] + i2 * p.s[2] + i3 * p.s[3]];
+        }
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t ik = tk * 32 + j, i3 = t3 * 32 + tx;
+        if (ik < dk && i3 < p.d[3]) {
+            int64_t i0, i1, i2;
+            index(ik, i0, i1, i2);
+            const int64_t e = ((i0 * p.d[1] + i1) * p.d[2] + i2) * p.d[3] + i3;
+            const double x = alpha * t[tx][j];
+            dst[e] = (beta == 0.0) ? x : beta * dst[e] + x;
+        }
+    }
+}
+
 // W[a][b][:] = V[a][b][:] - X[a][b][:] + X[b][a][:] over rows of length r:
 // <ab||ef> - P(ab) t_mb <am||ef> with X[a][b][ef] = sum_m t_mb <am||ef>.
 __global__ void wabef_kernel(double* __restrict__ W, const double* __restrict__ V, const double* __restrict__ X,
@@ -213,6 +250,19 @@ struct CCSD {
         const int64_t n23 = p.d[2] * p.d[3];
         CORR_REQUIRE(ctx, n23 < ((int64_t)1 << 31) && p.d[1] <= 65535 && p.d[0] <= 65535, "ccsd: permute too large");
         if (n23 == 0 || p.d[0] == 0 || p.d[1] == 0) return CORR_OK;
+        // source contiguous along i_k (k < 3) but not along i3: tiled transpose
+        int kc = -1;
+        for (int c = 0; c < 3; ++c)
+            if (p.s[c] == 1 && p.d[c] >= 8) kc = c;
+        if (p.s[3] != 1 && kc >= 0 && p.d[3] >= 8) {
+            const int64_t da = kc == 0 ? p.d[1] : p.d[0], db = kc == 2 ? p.d[1] : p.d[2];
+            const int64_t tiles = cdiv(p.d[kc], 32) * cdiv(p.d[3], 32);
+            CORR_REQUIRE(ctx, da <= 65535 && db <= 65535 && tiles < ((int64_t)1 << 31), "ccsd: permute too large");
+            hipLaunchKernelGGL(permute4_tiled_kernel, dim3((unsigned)tiles, (unsigned)db, (unsigned)da), dim3(256), 0,
+                               ctx->stream, dst, src, p, kc, alpha, beta);
+            CORR_LAUNCHED(ctx, "permute4_tiled_kernel");
+            return CORR_OK;
+        }
         hipLaunchKernelGGL(permute4_kernel, dim3((unsigned)cdiv(n23, 256), (unsigned)p.d[1], (unsigned)p.d[0]),
                            dim3(256), 0, ctx->stream, dst, src, p, alpha, beta);
         CORR_LAUNCHED(ctx, "permute4_kernel");
