@@ -11,5 +11,6 @@ from .posthf import (MOtrans, ao_unfold, build_FMo, calculate_ri, dgemm, ri_bia_
 
 from .ccsd import ccsd_energy, ccsd_update, ccsd_ur, ints_so  # noqa: F401
 from .scf import build_Fmat, build_Fmat_list, build_Fmat_ri, build_Pmat, run_scf, run_scf_ri  # noqa: F401
+from .localize import localize, localize_PM, localize_boys  # noqa: F401
 
 __version__ = "0.1.0"

@@ -73,6 +73,15 @@ class SCFResult(C.Structure):
                 ("core_guess", C.c_int)]
 
 
+class LocOpts(C.Structure):
+    _fields_ = [("max_sweeps", C.c_int), ("tol", C.c_double)]
+
+
+class LocResult(C.Structure):
+    _fields_ = [("spread_initial", C.c_double), ("spread", C.c_double), ("sweeps", C.c_int),
+                ("converged", C.c_int)]
+
+
 class CorrBasis(C.Structure):
     _fields_ = [("nshell", C.c_int64), ("l", C.POINTER(C.c_int32)), ("nprim", C.POINTER(C.c_int32)),
                 ("center", C.POINTER(C.c_double)), ("exps", C.POINTER(C.c_double)),
@@ -123,6 +132,8 @@ SIGNATURES = {
     "corr_int2c": [P, C.POINTER(CorrBasis), P],
     "corr_int3c": [P, C.POINTER(CorrBasis), C.POINTER(CorrBasis), P],
     "corr_int4c": [P, C.POINTER(CorrBasis), P],
+    "corr_int1e_dipole": [P, C.POINTER(CorrBasis), P, P],
+    "corr_localize": [P, I64, I64, P, I, P, P, I64, C.POINTER(LocOpts), P, C.POINTER(LocResult)],
     "corr_run_scf": [P, I64, I64, P, P, P, P, P, P, P, D, C.POINTER(SCFOpts), P, P, C.POINTER(SCFResult)],
     "corr_build_fmat_ri": [P, I64, I64, P, P, P, P],
     "corr_run_scf_ri": [P, I64, I64, P, P, I64, P, P, P, P, P, D, C.POINTER(SCFOpts), P, P, C.POINTER(SCFResult)],

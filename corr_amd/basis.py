@@ -82,6 +82,11 @@ class ShellArrays:
                 cen.extend(np.asarray(xyz, float))
                 ex.extend(e)
                 co.extend(c)
+        atom = []
+        for a, sym in enumerate(mol.symbols):
+            for (ll, e, c) in basis[sym]:
+                atom.extend([a] * (2 * ll + 1))
+        self.bf_atom = np.array(atom, dtype=np.int32)   # atom of every basis function
         self.l = np.array(l, dtype=np.int32)
         self.nprim = np.array(npr, dtype=np.int32)
         self.center = np.array(cen, dtype=np.float64)

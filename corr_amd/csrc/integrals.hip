@@ -519,6 +519,73 @@ __device__ inline double nuclear_one(const Prim& q, const double* e, int64_t nce
     return s * 6.283185307179586477 / q.p;
 }
 
+// 1D overlap factors o1[d][i][j] = <x_A^i | x_B^j> along d, for i <= la and
+// j <= jmax, without the sqrt(pi/p) e^{-mu X_AB^2} factor (Obara-Saika).
+__device__ inline void overlap_1d(const Prim& q, int la, int jmax, double (&o1)[3][LMAX + 1][LMAX + 3]) {
+    for (int d = 0; d < 3; ++d) {
+        const double XPA = q.P[d] - q.A[d], XPB = q.P[d] - q.B[d], h = 0.5 / q.p;
+        for (int i = 0; i <= la; ++i)
+            for (int j = 0; j <= jmax; ++j) {
+                double v;
+                if (i == 0 && j == 0) v = 1.0;
+                else if (i > 0) v = XPA * o1[d][i - 1][j] + h * ((i - 1) * (i > 1 ? o1[d][i - 2][j] : 0.0) +
+                                                             j * (j > 0 ? o1[d][i - 1][j - 1] : 0.0));
+                else v = XPB * o1[d][i][j - 1] + h * (j - 1 > 0 ? (j - 1) * o1[d][i][j - 2] : 0.0);
+                o1[d][i][j] = v;
+            }
+    }
+}
+
+// <a| r_d - O_d |b> for d = x, y, z: along d, x - O = (x - B) + (B - O), so
+// the 1D factor is o1[i][j+1] + (B_d - O_d) o1[i][j].  D is [3][N][N].
+__global__ __launch_bounds__(128) void dipole_kernel(DevSet s, const int32_t* __restrict__ objs, int64_t nobj,
+                                                     const double* __restrict__ sph, double ox, double oy, double oz,
+                                                     int64_t N, double* __restrict__ D) {
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= nobj * 25) return;
+    const int m = (int)(tid % 25);
+    const Obj o = s.obj[objs[tid / 25]];
+    if (m >= o.ncomp) return;
+    const int ma = m / o.nb, mb = m % o.nb, la = o.la, lb = o.lb;
+    const double* Ta = sph + la * NSPH_MAX * NCART_MAX + ma * NCART_MAX;
+    const double* Tb = sph + lb * NSPH_MAX * NCART_MAX + mb * NCART_MAX;
+    const double org[3] = {ox, oy, oz};
+    double sD[3] = {0.0, 0.0, 0.0};
+    for (int pi = 0; pi < o.nprim; ++pi) {
+        const Prim q = s.prim[o.prim0 + pi];
+        double o1[3][LMAX + 1][LMAX + 3];
+        overlap_1d(q, la, lb + 1, o1);
+        double ab2 = 0.0;
+        for (int k = 0; k < 3; ++k) ab2 += (q.A[k] - q.B[k]) * (q.A[k] - q.B[k]);
+        const double pre = q.K * exp(-q.a * q.b / q.p * ab2) * pow(M_PI / q.p, 1.5);
+        int ca = 0;
+        for (int ia = la; ia >= 0; --ia)
+            for (int ja = la - ia; ja >= 0; --ja, ++ca) {
+                const double wa = Ta[ca];
+                if (wa == 0.0) continue;
+                const int ka = la - ia - ja, ai[3] = {ia, ja, ka};
+                int cb = 0;
+                for (int ib = lb; ib >= 0; --ib)
+                    for (int jb = lb - ib; jb >= 0; --jb, ++cb) {
+                        const double w = wa * Tb[cb];
+                        if (w == 0.0) continue;
+                        const int kb = lb - ib - jb, bi[3] = {ib, jb, kb};
+                        double sd[3], md[3];
+                        for (int d = 0; d < 3; ++d) {
+                            const int i = ai[d], j = bi[d];
+                            sd[d] = o1[d][i][j];
+                            md[d] = o1[d][i][j + 1] + (q.B[d] - org[d]) * o1[d][i][j];
+                        }
+                        sD[0] += pre * w * md[0] * sd[1] * sd[2];
+                        sD[1] += pre * w * sd[0] * md[1] * sd[2];
+                        sD[2] += pre * w * sd[0] * sd[1] * md[2];
+                    }
+            }
+    }
+    const int64_t i = o.fa + ma, j = o.fb + mb;
+    for (int d = 0; d < 3; ++d) D[(d * N + i) * N + j] = D[(d * N + j) * N + i] = sD[d];
+}
+
 __global__ __launch_bounds__(128) void oneel_kernel(DevSet s, const int32_t* __restrict__ objs, int64_t nobj,
                                                     const double* __restrict__ sph, int64_t ncen,
                                                     const double* __restrict__ Z, const double* __restrict__ Cxyz,
@@ -537,18 +604,7 @@ __global__ __launch_bounds__(128) void oneel_kernel(DevSet s, const int32_t* __r
         const Prim q = s.prim[o.prim0 + pi];
         // 1D overlaps o1[d][i][j], i <= la, j <= lb + 2 (without sqrt(pi/p) e^{-mu X^2})
         double o1[3][LMAX + 1][LMAX + 3];
-        for (int d = 0; d < 3; ++d) {
-            const double XPA = q.P[d] - q.A[d], XPB = q.P[d] - q.B[d], h = 0.5 / q.p;
-            for (int i = 0; i <= la; ++i)
-                for (int j = 0; j <= lb + 2; ++j) {
-                    double v;
-                    if (i == 0 && j == 0) v = 1.0;
-                    else if (i > 0) v = XPA * o1[d][i - 1][j] + h * ((i - 1) * (i > 1 ? o1[d][i - 2][j] : 0.0) +
-                                                                 j * (j > 0 ? o1[d][i - 1][j - 1] : 0.0));
-                    else v = XPB * o1[d][i][j - 1] + h * (j - 1 > 0 ? (j - 1) * o1[d][i][j - 2] : 0.0);
-                    o1[d][i][j] = v;
-                }
-        }
+        overlap_1d(q, la, lb + 2, o1);
         double ab2 = 0.0;
         for (int k = 0; k < 3; ++k) ab2 += (q.A[k] - q.B[k]) * (q.A[k] - q.B[k]);
         const double pre = q.K * exp(-q.a * q.b / q.p * ab2) * pow(M_PI / q.p, 1.5);
@@ -810,6 +866,31 @@ int int1e(corr_ctx* ctx, const corr_basis* obs, int64_t natom, const double* cha
     return CORR_OK;
 }
 
+int int1e_dipole(corr_ctx* ctx, const corr_basis* obs, const double* origin, double* D) {
+    CORR_REQUIRE(ctx, D && origin, "int1e_dipole: bad arguments");
+    std::vector<HShell> osh;
+    int64_t n = 0;
+    CORR_TRY(load_basis(ctx, obs, osh, &n));
+    CORR_TRY(check_orbital(ctx, osh));
+    double* sph = nullptr;
+    CORR_TRY(sph_table(ctx, &sph));
+    ObjSet X;
+    build_pairs(osh, X, 0.0);
+    DevObjSet dX;
+    CORR_TRY(upload_set(ctx, X, "x", sph, dX));
+    std::vector<int32_t> all(X.obj.size());
+    for (size_t i = 0; i < all.size(); ++i) all[i] = (int32_t)i;
+    int32_t* dl = nullptr;
+    CORR_TRY(scratch(ctx, "int_xl", sizeof(int32_t) * all.size(), (void**)&dl));
+    CORR_HIP(ctx, hipMemcpyAsync(dl, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice, ctx->stream));
+    const int64_t nt = (int64_t)all.size() * 25;
+    hipLaunchKernelGGL(dipole_kernel, dim3((unsigned)cdiv(nt, 128)), dim3(128), 0, ctx->stream, dX.view(), dl,
+                       (int64_t)all.size(), sph, origin[0], origin[1], origin[2], n, D);
+    CORR_LAUNCHED(ctx, "dipole_kernel");
+    CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CORR_OK;
+}
+
 }  // namespace corr
 
 extern "C" {
@@ -833,6 +914,11 @@ int corr_int3c(corr_ctx* ctx, const corr_basis* aux, const corr_basis* obs, doub
 int corr_int4c(corr_ctx* ctx, const corr_basis* obs, double* d_dense) {
     CORR_CHECK_CTX(ctx);
     return corr::int4c(ctx, obs, d_dense);
+}
+
+int corr_int1e_dipole(corr_ctx* ctx, const corr_basis* obs, const double* h_origin, double* d_D) {
+    CORR_CHECK_CTX(ctx);
+    return corr::int1e_dipole(ctx, obs, h_origin, d_D);
 }
 
 }  // extern "C"

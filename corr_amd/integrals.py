@@ -43,6 +43,17 @@ def one_electron(mol: Molecule, obs: ShellArrays, device=None):
     return S, T, V
 
 
+def dipole(obs: ShellArrays, origin=(0.0, 0.0, 0.0), device=None) -> torch.Tensor:
+    """AO dipole integrals [3, N, N]: <mu| r_d - origin_d |nu> (psi4's
+    MintsHelper.ao_dipole, used by the reference's Boys localisation,
+    syshfwplug/pymodule.py:156-160)."""
+    D = _empty((3, obs.nbf, obs.nbf), device)
+    o = np.ascontiguousarray(origin, dtype=np.float64).reshape(3)
+    b = obs.struct()
+    _ctx(device).call("corr_int1e_dipole", C.byref(b), o.ctypes.data_as(C.c_void_p), ptr(D))
+    return D
+
+
 def metric(aux: ShellArrays, device=None) -> torch.Tensor:
     """(P|Q), naux x naux."""
     J = _empty((aux.nbf, aux.nbf), device)

@@ -32,6 +32,8 @@
  *   build_Pmat            src/SCF/scf.cc:44-61
  *   run_scf (+ calc_S12,  src/SCF/scf.cc:105-320,
  *   diag_Fmat, check_initial_guess)  src/SCF/ops_rhf.cc:15-37, 342-373, 425-430
+ *   localize_PM /         syshfwplug/pymodule.py:75-167
+ *   localize_boys
  */
 #ifndef CORR_ABI_H
 #define CORR_ABI_H
@@ -340,7 +342,10 @@ int corr_non_canonical_mp2(corr_ctx* ctx, int64_t nroao, int64_t nocc,
  *   corr_int2c  (P|Q) (naux x naux)    calculate_ri, ri.cc:25-60
  *   corr_int3c  (P|mu nu) [naux][N][N] calculate_ri, ri.cc:62-100
  *   corr_int4c  dense (mu nu|la si)     calculate_libint_tei, ops_rhf.cc:618-723
- *               (the reference keeps the sorted unique list instead) */
+ *               (the reference keeps the sorted unique list instead)
+ *   corr_int1e_dipole  D_x, D_y, D_z [3][N][N] about h_origin: psi4's
+ *               MintsHelper.ao_dipole in the Boys localisation,
+ *               syshfwplug/pymodule.py:156-160 */
 typedef struct corr_basis {
     int64_t        nshell;
     const int32_t* l;
@@ -357,6 +362,41 @@ int corr_int2c(corr_ctx* ctx, const corr_basis* aux, double* d_J);
 int corr_int3c(corr_ctx* ctx, const corr_basis* aux, const corr_basis* obs,
                double* d_Pmn);
 int corr_int4c(corr_ctx* ctx, const corr_basis* obs, double* d_dense);
+int corr_int1e_dipole(corr_ctx* ctx, const corr_basis* obs, const double* h_origin,
+                      double* d_D);
+
+/* ---- orbital localisation (syshfwplug/pymodule.py:75-167) -----------------
+ * localize_PM / localize_boys of the reference: maximise
+ * f = sum_K sum_i (Q_K)_ii^2 over rotations of the nocc orbitals in d_C
+ * (nocc x nbf, one orbital per row, as run_scf's MOs), with
+ *   PIPEK-MEZEY (calculate_PM, :143-153): d_M = S (nbf x nbf) and
+ *     h_bf_atom[nbf] the atom of every function (natom atoms); Q_A holds the
+ *     Mulliken populations of atom A;
+ *   BOYS (calculate_boys, :156-167): d_M = D_x, D_y, D_z (3 x nbf x nbf),
+ *     h_bf_atom unused.
+ * The reference maximises with BFGS over U = expm(X); here parallel Jacobi
+ * sweeps (round-robin pairs) run until a sweep raises f by at most
+ * tol * max(f, 1) or max_sweeps is reached.  d_L receives the localised
+ * orbitals (same layout as d_C; order and signs are the algorithm's). */
+#define CORR_LOC_PM    0
+#define CORR_LOC_BOYS  1
+
+typedef struct corr_loc_opts {
+    int    max_sweeps;   /* 100 */
+    double tol;          /* 1e-14 */
+} corr_loc_opts;
+
+typedef struct corr_loc_result {
+    double spread_initial;
+    double spread;
+    int    sweeps;
+    int    converged;
+} corr_loc_result;
+
+int corr_localize(corr_ctx* ctx, int64_t nbf, int64_t nocc, const double* d_C,
+                  int kind, const double* d_M, const int32_t* h_bf_atom,
+                  int64_t natom, const corr_loc_opts* opts, double* d_L,
+                  corr_loc_result* h_res);
 
 #ifdef __cplusplus
 }

@@ -277,6 +277,32 @@ def kinetic_block(sa, sb):
     return _sph(out, (sa.l, sb.l))
 
 
+def dipole_block(sa, sb, origin):
+    """<a| r_d - C_d |b> for d = x, y, z: in Hermite form x_C = x_P + (P - C)
+    gives E^{ij}_1 + X_PC E^{ij}_0 per dimension (times (pi/p)^(3/2)).
+    Returns [3][cart a][cart b] before the spherical transform."""
+    C = np.asarray(origin, float)
+    out = np.zeros((3, len(cart_list(sa.l)), len(cart_list(sb.l))))
+    for ia in range(len(sa.exps)):
+        for ib in range(len(sb.exps)):
+            a, b = sa.exps[ia], sb.exps[ib]
+            p = a + b
+            P = (a * sa.center + b * sb.center) / p
+            Es = [hermite_E(sa.l, sb.l, a, b, sa.center[d], sb.center[d]) for d in range(3)]
+            f = sa.coefs[ia] * sb.coefs[ib] * (np.pi / p) ** 1.5
+            for x, ci in enumerate(cart_list(sa.l)):
+                for y, cj in enumerate(cart_list(sb.l)):
+                    S1 = [Es[d][ci[d], cj[d], 0] for d in range(3)]
+                    for d in range(3):
+                        m = Es[d][ci[d], cj[d], 1] + (P[d] - C[d]) * S1[d]
+                        prod = m
+                        for e in range(3):
+                            if e != d:
+                                prod *= S1[e]
+                        out[d, x, y] += f * prod
+    return np.stack([_sph(out[d], (sa.l, sb.l)) for d in range(3)])
+
+
 def nuclear_block(sa, sb, charges, coords):
     out = np.zeros((len(cart_list(sa.l)), len(cart_list(sb.l))))
     L = sa.l + sb.l
@@ -390,6 +416,24 @@ def kinetic(shells):
             T[off[i]:off[i] + 2 * a.l + 1, off[j]:off[j] + 2 * b.l + 1] = kinetic_block(a, b)
     d = _norms(shells)
     return T * d[:, None] * d[None, :]
+
+
+def dipole(shells, origin=(0.0, 0.0, 0.0)):
+    """AO dipole integrals D[d][mu][nu] = <mu| r_d - origin_d |nu>, normalised
+    like S (the matrices psi4's MintsHelper.ao_dipole gives the reference's
+    Boys localisation, syshfwplug/pymodule.py:156-160, with origin 0)."""
+    n, off = nbf(shells), offsets(shells)
+    D = np.zeros((3, n, n))
+    for i, a in enumerate(shells):
+        for j, b in enumerate(shells):
+            D[:, off[i]:off[i] + 2 * a.l + 1, off[j]:off[j] + 2 * b.l + 1] = dipole_block(a, b, origin)
+    d = _norms(shells)
+    return D * d[None, :, None] * d[None, None, :]
+
+
+def bf_atom(shells) -> np.ndarray:
+    """Atom index of every basis function (psi4's function_to_center)."""
+    return np.concatenate([np.full(2 * s.l + 1, s.atom, dtype=np.int64) for s in shells])
 
 
 def nuclear(shells, charges, coords):
