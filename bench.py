@@ -240,7 +240,8 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     if args.suite:
-        keep = {"C": C, "J": J, "Pmn": Pmn, "F": real["F"] if real is not None else None, "nocc": o}
+        keep = {"C": C, "J": J, "Pmn": Pmn, "F": real["F"] if real is not None else None, "nocc": o,
+                "S": real["S"] if real is not None else None, "mol": real["mol"] if real is not None else None}
         del C, J, Pmn, real
         torch.cuda.empty_cache()
         line["suite"] = run_suite(dev, args.seed, [r for r in args.rows.split(",") if r], keep)
@@ -363,6 +364,27 @@ def run_suite(dev, seed: int, rows=(), main=None) -> dict:
         t = _timed(lambda: corr_amd.build_FMo(Cm, F))
         return {"shape": f"N={N}", "inputs": src, "ms": round(t * 1e3, 3), "TFLOPs": round(4 * N ** 3 / t / 1e12, 2)}
     rec("build_fmo_h2o32", fmo)
+
+    # orbital localisation (pymodule.py:75-167) of the 160 occupied RHF orbitals of (H2O)32
+    def localize():
+        if main is None or main.get("S") is None:
+            return {"skipped": "needs the real (H2O)32 inputs"}
+        from corr_amd import integrals as GI
+        from corr_amd.basis import ShellArrays, read_g94
+        Cocc = main["C"][:main["nocc"]].contiguous()
+        O = ShellArrays(main["mol"], read_g94("def2-svp"))
+        D = GI.dipole(O)
+        res = {"shape": f"(H2O)32 nocc={Cocc.shape[0]} nbf={Cocc.shape[1]} atoms={len(main['mol'].symbols)}"}
+        for kind in ("pm", "boys"):
+            t0 = time.perf_counter()
+            info, _ = corr_amd.localize(Cocc, kind, S=main["S"], bf_atom=O.bf_atom, D=D)
+            torch.cuda.synchronize()
+            res[f"{kind}_s"] = round(time.perf_counter() - t0, 3)
+            res[f"{kind}_sweeps"] = info["sweeps"]
+            res[f"{kind}_converged"] = info["converged"]
+            res[f"{kind}_spread"] = [round(info["spread_initial"], 6), round(info["spread"], 6)]
+        return res
+    rec("localize_h2o32", localize)
 
     # (H2O)8: exact integrals and RHF, then unfold + 4-index + canonical and non-canonical MP2 + Fock build
     h2o8 = {}
