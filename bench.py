@@ -6,11 +6,13 @@ integrals (P|mu nu) already resident in HBM (Cholesky metric, overlapped on a
 second stream with the AO->MO GEMMs, then the metric GEMM), then the fused
 RI-MP2 pair-block energy.  Sizes are those of
 (H2O)32: N = 768 basis functions, o = 160 occupied, v = 608 virtual,
-naux = 2432 auxiliary functions.  Two paths are timed, K steps each:
+naux = 2432 auxiliary functions.  Three paths are timed, K steps each:
 
   mixed: metric FP64, B-formation GEMMs FP32 MFMA, energy BF16x3 MFMA with
          FP32 accumulation and FP64 denominators/reduction
   fp64:  everything FP64 (FP64 MFMA)
+  fp32:  as mixed with the energy in FP32 MFMA (reported beside, with its
+         error against fp64)
 
 value = algorithmic TFLOP/s of the mixed path, (W_B + W_E) / t_step, with
 W_E = 2 [o(o+1)/2] v^2 naux and W_B = 2 naux o N^2 + 2 naux o v N + 2 naux^2 o v.
@@ -177,7 +179,7 @@ def main() -> int:
         return corr_amd.rimp2_energy(B, eo, ev, prec_e)
 
     results = {}
-    for name, pg, pe in (("fp64", "fp64", "fp64"), ("mixed", "fp32", "bf16x3")):
+    for name, pg, pe in (("fp64", "fp64", "fp64"), ("mixed", "fp32", "bf16x3"), ("fp32", "fp32", "fp32")):
         for _ in range(args.warmup):
             step(pg, pe)
         torch.cuda.synchronize()
@@ -228,6 +230,10 @@ def main() -> int:
         "emp2_fp64": f64["energy"]["EMP2"],
         "emp2_mixed": mixed["energy"]["EMP2"],
         "fp64_tflops": round((W_E + W_B) / f64["s_per_step"] / 1e12, 3),
+        "fp32": {"s": round(results["fp32"]["s_per_step"], 5),
+                 "speedup_vs_fp64": round(f64["s_per_step"] / results["fp32"]["s_per_step"], 3),
+                 "error_uEh": round(abs(results["fp32"]["energy"]["EMP2"] - f64["energy"]["EMP2"]) * 1e6, 4),
+                 "note": "B formation and energy in FP32 MFMA (FP32 accumulation, FP64 denominators/reduction)"},
         "roofline": {"bound": "mfma", "kernel": "rimp2_energy_bf16x3", "achieved": round(achieved, 2),
                      "peak": 2500.0, "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4),
                      "traffic": recorded_traffic("rimp2_energy_bf16x3"),
