@@ -395,19 +395,24 @@ struct CCSD {
         CORR_TRY(mm(0, 0, ov, 1, ov, 1.0, Pmenf, ov, 0, T1, 1, 0, 0.0, Fme, 1, 0));
         hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(ov)), dim3(256), 0, ctx->stream, Fme, Tmp1, ov, 1.0, 1.0);
         CORR_LAUNCHED(ctx, "axpby_kernel");
-        // ---- Wmnij = <mn||ij> + P(ij) t_je <mn||ie> + 1/4 tau_ijef <mn||ef>
+        // ---- Wmnij = <mn||ij> + P(ij) t_je <mn||ie> + 1/2 tau_ijef <mn||ef>
+        // The reference's Wmnij has 1/4 here and its Wabef a term 1/4 tau_mnab
+        // <mn||ef> (ccsd_intermediates.cc:127-230).  That Wabef term adds
+        // 1/2 tau_ijef * 1/4 tau_mnab <mn||ef> = 1/8 tau_mnab sum_ef tau_ijef
+        // <mn||ef> to the pp ladder, which is what Wmnij's 1/4 term adds to the
+        // hh ladder; so Wmnij carries 1/2 and Wabef has no tau term: the
+        // amplitudes are the same and one o^2 v^4 GEMM is saved.
         CORR_TRY(perm(Wmnij, Voooo, {o, o, o, o}, {o * o2, o2, o, 1}));
         CORR_TRY(mm(0, 1, o * o2, o, v, 1.0, Vooov, v, 0, T1, v, 0, 1.0, Wmnij, o, 0));
         CORR_TRY(mm(0, 1, o, o, v, -1.0, T1, v, 0, Vooov, v, o * v, 1.0, Wmnij, o, o2, o2));
-        CORR_TRY(mm(0, 1, o2, o2, v2, 0.25, Voovv, v2, 0, tau, v2, 0, 1.0, Wmnij, o2, 0));
-        // ---- Wabef = <ab||ef> - P(ab) t_mb <am||ef> + 1/4 tau_mnab <mn||ef>
+        CORR_TRY(mm(0, 1, o2, o2, v2, 0.5, Voovv, v2, 0, tau, v2, 0, 1.0, Wmnij, o2, 0));
+        // ---- Wabef = <ab||ef> - P(ab) t_mb <am||ef>  (its tau term is in Wmnij)
         // Xab[a][b][(ef)] = sum_m t_mb <am||ef> (one GEMM per a), then one pass
-        // Wabef = Vvvvv - Xab[a][b] + Xab[b][a], then the tau term
+        // Wabef = Vvvvv - Xab[a][b] + Xab[b][a]
         CORR_TRY(mm(1, 0, v, v2, o, 1.0, T1, v, 0, Vvovv, v2, o * v2, 0.0, Xab, v2, v * v2, v));
         hipLaunchKernelGGL(wabef_kernel, dim3((unsigned)cdiv(v2, 256), (unsigned)v, (unsigned)v), dim3(256), 0,
                            ctx->stream, Wabef, Vvvvv, Xab, v, v2);
         CORR_LAUNCHED(ctx, "wabef_kernel");
-        CORR_TRY(mm(1, 0, v2, v2, o2, 0.25, tau, v2, 0, Voovv, v2, 0, 1.0, Wabef, v2, 0));
         // ---- Wmbej = <mb||ej> + t_jf <mb||ef> - t_nb <mn||ej> - (1/2 t_jnfb + t_jf t_nb) <mn||ef>
         CORR_TRY(perm(Wmbej, Vovvo, {o, v, v, o}, {v2 * o, v * o, o, 1}));
         CORR_TRY(mm(0, 1, ov * v, o, v, 1.0, Vovvv, v, 0, T1, v, 0, 1.0, Wmbej, o, 0));

@@ -72,3 +72,27 @@ def test_ccsd_ur_driver_shape_and_convergence():
     # HF check reproduces the SCF energy of the closed-shell determinant
     e_scf = 0.5 * np.sum(2 * C[:nocc_sp].T @ C[:nocc_sp] * (h_ao + F)) + 1.5
     assert np.isclose(r["e_hf_check"], e_scf, rtol=1e-12)
+
+
+def test_wabef_tau_term_equals_a_doubled_wmnij_tau_term():
+    """The T2 contribution of Wabef's 1/4 tau_mnab <mn||ef> term through the pp
+    ladder (1/2 tau_ijef W_abef) equals that of Wmnij's 1/4 tau_ijef <mn||ef>
+    term through the hh ladder (1/2 tau_mnab W_mnij), for arbitrary amplitudes;
+    so Wmnij with 1/2 and Wabef without its tau term give the same T2."""
+    N, nocc_sp = 6, 2
+    h_ao, eri, C, F, prec = _system(N, nocc_sp, seed=5)
+    V = CC.ints_so(prec)
+    h, f = CC.form_fock(C, h_ao, V, 2 * nocc_sp)
+    B = CC.Blocks(V, f, 2 * nocc_sp)
+    rng = np.random.default_rng(9)
+    T1 = 0.05 * rng.standard_normal((B.o, B.v))
+    T2 = 0.05 * rng.standard_normal((B.o, B.o, B.v, B.v))
+    T2 = T2 - T2.transpose(1, 0, 2, 3)
+    T2 = T2 - T2.transpose(0, 1, 3, 2)
+    _, tau = CC.taus(T1, T2)
+    wabef_tau = 0.25 * np.einsum("mnab,mnef->abef", tau, B.oovv)
+    pp = 0.5 * np.einsum("ijef,abef->ijab", tau, wabef_tau)
+    wmnij_tau = 0.25 * np.einsum("ijef,mnef->mnij", tau, B.oovv)
+    hh = 0.5 * np.einsum("mnab,mnij->ijab", tau, wmnij_tau)
+    assert np.abs(pp).max() > 1e-6
+    assert np.allclose(pp, hh, rtol=0, atol=1e-15)
