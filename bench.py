@@ -542,6 +542,7 @@ def run_suite(dev, seed: int, rows=(), main=None) -> dict:
             it = (t11 - t1) / 10
             res[f"iteration_{prec}_ms"] = round(it * 1e3, 2)
             res[f"iteration_{prec}_TFLOPs"] = round(W / it / 1e12, 2)
+            res[f"iteration_{prec}_TFLOPs_of_W_CC_reference"] = round(W_ref / it / 1e12, 2)
             res[f"setup_{prec}_ms"] = round((t1 - it) * 1e3, 1)
             c = corr_amd.ccsd_ur(prec_ints, Cm, h, nroe, ion, maxiter=100, mode="diis", prec=prec, e_tol=1e-9,
                                  r_tol=1e-7)
