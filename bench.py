@@ -526,11 +526,13 @@ def run_suite(dev, seed: int, rows=(), main=None) -> dict:
         o, v = nroe, 2 * N - nroe
         prec_ints = corr_amd.MOtrans(r["C"], r["G"], "fp64")
         Cm, h, ion = r["C"], r["H"], r["mol"].nuclear_repulsion()
-        # work of this implementation: pp ladder 2 o^2 v^4, ring 2 x 2 (ov)^3, Wmnij tau term and
-        # hh ladder 2 x 2 o^4 v^2 (the reference's intermediates add a second o^2 v^4 GEMM: Wabef's
-        # tau term, carried here by Wmnij)
-        W = 2.0 * o * o * v ** 4 + 4.0 * o ** 3 * v ** 3 + 4.0 * o ** 4 * v * v
-        W_ref = W + 2.0 * o * o * v ** 4
+        # work of this implementation: the pp ladder over packed antisymmetric pairs
+        # 2 Po Pv^2 (Po = o(o-1)/2, Pv = v(v-1)/2), ring 2 x 2 (ov)^3, Wmnij tau term and hh
+        # ladder 2 x 2 o^4 v^2; the reference's intermediates: 4 o^2 v^4 + 4 o^3 v^3 + 4 o^4 v^2
+        # (unpacked pp ladder, and Wabef's tau term, carried here by Wmnij)
+        Po, Pv = o * (o - 1) // 2, v * (v - 1) // 2
+        W = 2.0 * Po * Pv * Pv + 4.0 * o ** 3 * v ** 3 + 4.0 * o ** 4 * v * v
+        W_ref = 4.0 * o * o * v ** 4 + 4.0 * o ** 3 * v ** 3 + 4.0 * o ** 4 * v * v
         res = {"shape": "(H2O)4 N=96 spin-orbital o=40 v=152, real integrals",
                "e_rhf": r["scf"]["e_total"], "W_CC_TFLOP_per_iter": round(W / 1e12, 3),
                "W_CC_reference_intermediates_TFLOP": round(W_ref / 1e12, 3)}

@@ -93,16 +93,70 @@ __global__ __launch_bounds__(256) void permute4_tiled_kernel(double* __restrict_
     }
 }
 
-// W[a][b][:] = V[a][b][:] - X[a][b][:] + X[b][a][:] over rows of length r:
-// <ab||ef> - P(ab) t_mb <am||ef> with X[a][b][ef] = sum_m t_mb <am||ef>.
-__global__ void wabef_kernel(double* __restrict__ W, const double* __restrict__ V, const double* __restrict__ X,
-                             int64_t v, int64_t r) {
-    const int64_t a = blockIdx.z, b = blockIdx.y;
-    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= r) return;
-    const int64_t e = (a * v + b) * r + c;
-    W[e] = V[e] - X[e] + X[(b * v + a) * r + c];
+// Antisymmetric pair packing of the particle-particle ladder.  Pair p of n
+// orbitals is (x_p < y_p) (index arrays px, py); rows p of the 2-D grid
+// (blockIdx.y, looping past 65535) and columns q (blockIdx.x, threads).
+#define PAIR_LOOP(P, Q)                                                                   \
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;                     \
+    if (q >= (Q)) return;                                                                 \
+    for (int64_t p = blockIdx.y; p < (P); p += gridDim.y)
+
+// VP[p][q] = <ab||ef>, (a<b) = pair p and (e<f) = pair q of the virtuals, from
+// the spin-orbital tensor V (n^4, [p][q][r][s]).
+__global__ void pack_vvvv_kernel(const double* __restrict__ V, int64_t n, int64_t o, const int32_t* __restrict__ pa,
+                                 const int32_t* __restrict__ pb, int64_t Pv, double* __restrict__ out) {
+    PAIR_LOOP(Pv, Pv) {
+        const int64_t a = o + pa[p], b = o + pb[p], e = o + pa[q], f = o + pb[q];
+        out[p * Pv + q] = V[((a * n + b) * n + e) * n + f];
+    }
 }
+
+// out[a][m][q] = <am||ef>, (e<f) = pair q; row index a*o + m.
+__global__ void pack_vovv_kernel(const double* __restrict__ V, int64_t n, int64_t o, int64_t v,
+                                 const int32_t* __restrict__ pa, const int32_t* __restrict__ pb, int64_t Pv,
+                                 double* __restrict__ out) {
+    PAIR_LOOP(v * o, Pv) {
+        const int64_t a = o + p / o, m = p % o, e = o + pa[q], f = o + pb[q];
+        out[p * Pv + q] = V[((a * n + m) * n + e) * n + f];
+    }
+}
+
+// tauP[p][q] = tau[i][j][e][f], (i<j) = pair p of the occupied, (e<f) = pair q.
+__global__ void pack_tau_kernel(const double* __restrict__ tau, int64_t o, int64_t v, const int32_t* __restrict__ pi,
+                                const int32_t* __restrict__ pj, const int32_t* __restrict__ pa,
+                                const int32_t* __restrict__ pb, int64_t Po, int64_t Pv, double* __restrict__ out) {
+    PAIR_LOOP(Po, Pv) {
+        const int64_t i = pi[p], j = pj[p], e = pa[q], f = pb[q];
+        out[p * Pv + q] = tau[((i * o + j) * v + e) * v + f];
+    }
+}
+
+// WP[p][q] = VP[p][q] - XP[a][b][q] + XP[b][a][q], (a<b) = pair p:
+// <ab||ef> - P(ab) t_mb <am||ef> with XP[a][b][q] = sum_m t_mb <am||ef>.
+__global__ void wabef_packed_kernel(const double* __restrict__ VP, const double* __restrict__ XP, int64_t v,
+                                    const int32_t* __restrict__ pa, const int32_t* __restrict__ pb, int64_t Pv,
+                                    double* __restrict__ WP) {
+    PAIR_LOOP(Pv, Pv) {
+        const int64_t a = pa[p], b = pb[p];
+        WP[p * Pv + q] = VP[p * Pv + q] - XP[(a * v + b) * Pv + q] + XP[(b * v + a) * Pv + q];
+    }
+}
+
+// T2n[i][j][a][b] += Y[p][q] with the antisymmetry of T2 in (ij) and (ab):
+// (i<j) = pair p, (a<b) = pair q; each T2 element has one source.
+__global__ void scatter_pp_kernel(const double* __restrict__ Y, int64_t o, int64_t v, const int32_t* __restrict__ pi,
+                                  const int32_t* __restrict__ pj, const int32_t* __restrict__ pa,
+                                  const int32_t* __restrict__ pb, int64_t Po, int64_t Pv, double* __restrict__ T2n) {
+    PAIR_LOOP(Po, Pv) {
+        const int64_t i = pi[p], j = pj[p], a = pa[q], b = pb[q];
+        const double y = Y[p * Pv + q];
+        T2n[((i * o + j) * v + a) * v + b] += y;
+        T2n[((j * o + i) * v + a) * v + b] -= y;
+        T2n[((i * o + j) * v + b) * v + a] -= y;
+        T2n[((j * o + i) * v + b) * v + a] += y;
+    }
+}
+#undef PAIR_LOOP
 
 // Spin-orbital h from hs = C H C^T and f = h + sum_{j<nocc} <pj||qj>  (ccsd.cc:11-48)
 __global__ void form_fock_kernel(const double* __restrict__ hs, const double* __restrict__ V, int64_t N,
@@ -232,11 +286,15 @@ struct CCSD {
     int64_t o, v, n;
     int prec;
     // constant blocks of <pq||rs>
-    double *Voooo, *Vooov, *Voovo, *Voovv, *Vovoo, *Vovvo, *Vovvv, *Vvovv, *Vvvvv;
+    double *Voooo, *Vooov, *Voovo, *Voovv, *Vovoo, *Vovvo, *Vovvv, *Vvovv;
+    // packed over antisymmetric pairs: <ab||ef> [(a<b)][(e<f)], <am||ef> [a][m][(e<f)]
+    double *VvvvvP, *VvovvP;
+    int64_t Po = 0, Pv = 0;            // o(o-1)/2, v(v-1)/2
+    int32_t *po_i = nullptr, *po_j = nullptr, *pv_a = nullptr, *pv_b = nullptr;
     double *PeOOf, *Pmenf, *Pmine, *Pianf, *Pmbje, *Pabje;
     double *fo;   // f (n x n)
     // per-iteration work
-    double *tau_s, *tau, *tauS_a, *Fae, *Faep, *Fmi, *Fmip, *Fme, *Yk, *Wmnij, *Wabef, *Xab, *Wmbej;
+    double *tau_s, *tau, *tauS_a, *Fae, *Faep, *Fmi, *Fmip, *Fme, *Yk, *Wmnij, *WabefP, *XabP, *tauP, *Ypp, *Wmbej;
     double *Zp, *W2, *Wp, *T2p, *T2a, *X1, *X2, *Zr, *U, *Q, *Tmp1;
     double* partial;
 
@@ -296,7 +354,34 @@ struct CCSD {
         CORR_TRY(alloc(&Vovvo, "cc_Vovvo", O2V2));
         CORR_TRY(alloc(&Vovvv, "cc_Vovvv", OV3));
         CORR_TRY(alloc(&Vvovv, "cc_Vvovv", OV3));
-        CORR_TRY(alloc(&Vvvvv, "cc_Vvvvv", v * v * v * v));
+        // pair index arrays of the packed pp ladder
+        Po = o * (o - 1) / 2;
+        Pv = v * (v - 1) / 2;
+        {
+            std::vector<int32_t> h(2 * (Po + Pv) + 4);
+            int64_t k = 0;
+            for (int64_t x = 0; x < o; ++x)
+                for (int64_t y = x + 1; y < o; ++y, ++k) {
+                    h[k] = (int32_t)x;
+                    h[Po + k] = (int32_t)y;
+                }
+            k = 0;
+            for (int64_t x = 0; x < v; ++x)
+                for (int64_t y = x + 1; y < v; ++y, ++k) {
+                    h[2 * Po + k] = (int32_t)x;
+                    h[2 * Po + Pv + k] = (int32_t)y;
+                }
+            int32_t* d = nullptr;
+            CORR_TRY(scratch(ctx, "cc_pairs", sizeof(int32_t) * h.size(), (void**)&d));
+            CORR_HIP(ctx, hipMemcpyAsync(d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+            CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));   // h is read by the copy
+            po_i = d;
+            po_j = d + Po;
+            pv_a = d + 2 * Po;
+            pv_b = d + 2 * Po + Pv;
+        }
+        CORR_TRY(alloc(&VvvvvP, "cc_VvvvvP", std::max<int64_t>(Pv * Pv, 1)));
+        CORR_TRY(alloc(&VvovvP, "cc_VvovvP", std::max<int64_t>(v * o * Pv, 1)));
         CORR_TRY(alloc(&PeOOf, "cc_PeOOf", O2V2));
         CORR_TRY(alloc(&Pmenf, "cc_Pmenf", O2V2));
         CORR_TRY(alloc(&Pmine, "cc_Pmine", O3V));
@@ -313,7 +398,15 @@ struct CCSD {
         CORR_TRY(perm(Vovvo, at(0, ob, ob, 0), {o, v, v, o}, {s0, s1, s2, 1}));
         CORR_TRY(perm(Vovvv, at(0, ob, ob, ob), {o, v, v, v}, {s0, s1, s2, 1}));
         CORR_TRY(perm(Vvovv, at(ob, 0, ob, ob), {v, o, v, v}, {s0, s1, s2, 1}));
-        CORR_TRY(perm(Vvvvv, at(ob, ob, ob, ob), {v, v, v, v}, {s0, s1, s2, 1}));
+        if (Pv > 0) {
+            hipLaunchKernelGGL(pack_vvvv_kernel, dim3((unsigned)cdiv(Pv, 256), (unsigned)std::min<int64_t>(Pv, 65535)),
+                               dim3(256), 0, ctx->stream, V, n, o, pv_a, pv_b, Pv, VvvvvP);
+            CORR_LAUNCHED(ctx, "pack_vvvv_kernel");
+            hipLaunchKernelGGL(pack_vovv_kernel,
+                               dim3((unsigned)cdiv(Pv, 256), (unsigned)std::min<int64_t>(v * o, 65535)), dim3(256), 0,
+                               ctx->stream, V, n, o, v, pv_a, pv_b, Pv, VvovvP);
+            CORR_LAUNCHED(ctx, "pack_vovv_kernel");
+        }
         // PeOOf[e][m][n][f] = <mn||ef>
         CORR_TRY(perm(PeOOf, at(0, 0, ob, ob), {v, o, o, v}, {s2, s0, s1, 1}));
         // Pmenf[m][e][n][f] = <mn||ef>
@@ -337,8 +430,10 @@ struct CCSD {
         CORR_TRY(alloc(&Fme, "cc_Fme", o * v));
         CORR_TRY(alloc(&Yk, "cc_Yk", o * v));
         CORR_TRY(alloc(&Wmnij, "cc_Wmnij", O4));
-        CORR_TRY(alloc(&Wabef, "cc_Wabef", v * v * v * v));
-        CORR_TRY(alloc(&Xab, "cc_Xab", v * v * v * v));
+        CORR_TRY(alloc(&WabefP, "cc_WabefP", std::max<int64_t>(Pv * Pv, 1)));
+        CORR_TRY(alloc(&XabP, "cc_XabP", std::max<int64_t>(v * v * Pv, 1)));
+        CORR_TRY(alloc(&tauP, "cc_tauP", std::max<int64_t>(Po * Pv, 1)));
+        CORR_TRY(alloc(&Ypp, "cc_Ypp", std::max<int64_t>(Po * Pv, 1)));
         CORR_TRY(alloc(&Wmbej, "cc_Wmbej", O2V2));
         CORR_TRY(alloc(&Zp, "cc_Zp", O2V2));
         CORR_TRY(alloc(&W2, "cc_W2", O2V2));
@@ -406,13 +501,17 @@ struct CCSD {
         CORR_TRY(mm(0, 1, o * o2, o, v, 1.0, Vooov, v, 0, T1, v, 0, 1.0, Wmnij, o, 0));
         CORR_TRY(mm(0, 1, o, o, v, -1.0, T1, v, 0, Vooov, v, o * v, 1.0, Wmnij, o, o2, o2));
         CORR_TRY(mm(0, 1, o2, o2, v2, 0.5, Voovv, v2, 0, tau, v2, 0, 1.0, Wmnij, o2, 0));
-        // ---- Wabef = <ab||ef> - P(ab) t_mb <am||ef>  (its tau term is in Wmnij)
-        // Xab[a][b][(ef)] = sum_m t_mb <am||ef> (one GEMM per a), then one pass
-        // Wabef = Vvvvv - Xab[a][b] + Xab[b][a]
-        CORR_TRY(mm(1, 0, v, v2, o, 1.0, T1, v, 0, Vvovv, v2, o * v2, 0.0, Xab, v2, v * v2, v));
-        hipLaunchKernelGGL(wabef_kernel, dim3((unsigned)cdiv(v2, 256), (unsigned)v, (unsigned)v), dim3(256), 0,
-                           ctx->stream, Wabef, Vvvvv, Xab, v, v2);
-        CORR_LAUNCHED(ctx, "wabef_kernel");
+        // ---- Wabef = <ab||ef> - P(ab) t_mb <am||ef>  (its tau term is in Wmnij),
+        // antisymmetric in (ab) and (ef): kept only for a < b, e < f.
+        // XabP[a][b][(e<f)] = sum_m t_mb <am||ef> (one GEMM per a), then one pass
+        // WabefP = VvvvvP - XabP[a][b] + XabP[b][a]
+        if (Pv > 0) {
+            CORR_TRY(mm(1, 0, v, Pv, o, 1.0, T1, v, 0, VvovvP, Pv, o * Pv, 0.0, XabP, Pv, v * Pv, v));
+            hipLaunchKernelGGL(wabef_packed_kernel,
+                               dim3((unsigned)cdiv(Pv, 256), (unsigned)std::min<int64_t>(Pv, 65535)), dim3(256), 0,
+                               ctx->stream, VvvvvP, XabP, v, pv_a, pv_b, Pv, WabefP);
+            CORR_LAUNCHED(ctx, "wabef_packed_kernel");
+        }
         // ---- Wmbej = <mb||ej> + t_jf <mb||ef> - t_nb <mn||ej> - (1/2 t_jnfb + t_jf t_nb) <mn||ef>
         CORR_TRY(perm(Wmbej, Vovvo, {o, v, v, o}, {v2 * o, v * o, o, 1}));
         CORR_TRY(mm(0, 1, ov * v, o, v, 1.0, Vovvv, v, 0, T1, v, 0, 1.0, Wmbej, o, 0));
@@ -452,8 +551,20 @@ struct CCSD {
         CORR_TRY(perm(T2n, X2, {o, o, v, v}, {v2, o * v2, v, 1}, 1.0, 1.0));
         // hh ladder: 1/2 tau_mnab W_mnij
         CORR_TRY(mm(1, 0, o2, v2, o2, 0.5, Wmnij, o2, 0, tau, v2, 0, 1.0, T2n, v2, 0));
-        // pp ladder: 1/2 tau_ijef W_abef
-        CORR_TRY(mm(0, 1, o2, v2, v2, 0.5, tau, v2, 0, Wabef, v2, 0, 1.0, T2n, v2, 0));
+        // pp ladder: 1/2 sum_ef tau_ijef W_abef = sum_{e<f} tau_ijef W_abef, computed
+        // for i < j and a < b only (the product is symmetric under e <-> f, T2
+        // antisymmetric in (ij) and (ab)): one GEMM over the packed pairs, 1/8 of
+        // the unpacked work, then each result goes to its four T2 elements.
+        if (Po > 0 && Pv > 0) {
+            const dim3 gp((unsigned)cdiv(Pv, 256), (unsigned)std::min<int64_t>(Po, 65535));
+            hipLaunchKernelGGL(pack_tau_kernel, gp, dim3(256), 0, ctx->stream, tau, o, v, po_i, po_j, pv_a, pv_b, Po, Pv,
+                               tauP);
+            CORR_LAUNCHED(ctx, "pack_tau_kernel");
+            CORR_TRY(mm(0, 1, Po, Pv, Pv, 1.0, tauP, Pv, 0, WabefP, Pv, 0, 0.0, Ypp, Pv, 0));
+            hipLaunchKernelGGL(scatter_pp_kernel, gp, dim3(256), 0, ctx->stream, Ypp, o, v, po_i, po_j, pv_a, pv_b, Po,
+                               Pv, T2n);
+            CORR_LAUNCHED(ctx, "scatter_pp_kernel");
+        }
         // ring: Zr[(ia)][(jb)] = t_imae W_mbej - t_ie t_ma <mb||ej>
         CORR_TRY(perm(Wp, Wmbej, {o, v, o, v}, {v2 * o, o, 1, v * o}));   // Wp[m][e][j][b]
         CORR_TRY(mm(0, 0, ov, ov, ov, 1.0, T2p, ov, 0, Wp, ov, 0, 0.0, Zr, ov, 0));
