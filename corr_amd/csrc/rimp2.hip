@@ -391,13 +391,14 @@ __device__ inline void bf16_stage_compute(const Bf16Stage& S, const int (&ra)[2]
 // j = wc (rows a, columns b, all 32 x 32 of the tile's virtual blocks), and
 // T2_x = B[:, (J,A) row j]^T B[:, (I,B) row i] = M^{ji}_ab = M^{ij}_ba, so the
 // exchange partner of every element sits in the same lane and register.
-// The FP32 accumulators are added into FP64 totals and cleared every FLUSH
-// groups of four 16-deep stages (64 K values), which shortens the FP32
-// accumulation chains.  On the (H2O)32 / def2-SVP inputs the error of the
-// energy against FP64 drops from 0.73 to 0.43 uEh for 3% more time (flushing
-// every 8 groups: 0.47 uEh, +1%; every 2: 0.41 uEh, +7%).  The FP64 totals
-// take 128 VGPRs (256 in all, no spills, occupancy unchanged at 2 waves/SIMD,
-// which the 128 KiB of LDS sets).
+// Accumulation is FP32 on two levels: the MFMA accumulators are added into
+// FP32 totals and cleared every FLUSH groups of four 16-deep stages (64 K
+// values), which shortens each rounding chain from 456 MFMA steps (152
+// stages at naux = 2432) to 48 plus 10 total updates.  On the (H2O)32 / def2-SVP inputs the error of the energy
+// against FP64 drops from 0.77 to 0.48 uEh (0.47 with FP64 totals, which take
+// twice the registers and ran 2.5% slower; one level: 0.77 uEh).  The totals
+// take 64 VGPRs (206 in all, no spills, 2 waves/SIMD as the 128 KiB of LDS
+// sets).
 constexpr int BF16X3_FLUSH = 4;
 
 template <int FLUSH>
@@ -434,13 +435,13 @@ __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_bf16x3(
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int r = 0; r < 16; ++r) { acc1[x][r] = 0.0f; acc2[x][r] = 0.0f; }
-    // FP64 totals of the FP32 accumulators, flushed every FLUSH 4-stage groups
-    double d1[FLUSH > 0 ? 2 : 1][FLUSH > 0 ? 16 : 1], d2[FLUSH > 0 ? 2 : 1][FLUSH > 0 ? 16 : 1];
+    // FP32 totals of the accumulators, flushed every FLUSH 4-stage groups
+    float d1[FLUSH > 0 ? 2 : 1][FLUSH > 0 ? 16 : 1], d2[FLUSH > 0 ? 2 : 1][FLUSH > 0 ? 16 : 1];
     if constexpr (FLUSH > 0) {
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) { d1[x][r] = 0.0; d2[x][r] = 0.0; }
+            for (int r = 0; r < 16; ++r) { d1[x][r] = 0.0f; d2[x][r] = 0.0f; }
     }
     auto flush = [&]() {
         if constexpr (FLUSH > 0) {
@@ -448,8 +449,8 @@ __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_bf16x3(
             for (int x = 0; x < 2; ++x)
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    d1[x][r] += (double)acc1[x][r];
-                    d2[x][r] += (double)acc2[x][r];
+                    d1[x][r] += acc1[x][r];
+                    d2[x][r] += acc2[x][r];
                     acc1[x][r] = 0.0f;
                     acc2[x][r] = 0.0f;
                 }
@@ -498,8 +499,8 @@ __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_bf16x3(
         for (int r = 0; r < 16; ++r) {
             const int a = t.AC * PA + 8 * (r / 4) + 4 * kh + (r % 4);
             if (i < o && j < o && a < v && b < v) {
-                const double m1 = FLUSH > 0 ? d1[x][r] : (double)acc1[x][r];
-                const double m2 = FLUSH > 0 ? d2[x][r] : (double)acc2[x][r];
+                const double m1 = FLUSH > 0 ? (double)d1[x][r] : (double)acc1[x][r];
+                const double m2 = FLUSH > 0 ? (double)d2[x][r] : (double)acc2[x][r];
                 const double dinv = 1.0 / (ev[a] + ev[b] - eo[i] - eo[j]);
                 sos += (m1 * m1 + m2 * m2) * dinv;
                 sx += m1 * m2 * dinv;
