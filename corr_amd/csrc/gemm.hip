@@ -375,10 +375,41 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double* __restrict__ 
     }
 }
 
+// First pass of a long reduction: block b sums the fixed chunk
+// [b*chunk, (b+1)*chunk) of every series in a fixed order.
+__global__ __launch_bounds__(256) void reduce_chunks_kernel(const double* __restrict__ d, int64_t n, int nser,
+                                                            int64_t chunk, double* __restrict__ out) {
+    __shared__ double sh[256];
+    const int64_t lo = blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    for (int s = 0; s < nser; ++s) {
+        double acc = 0.0;
+        for (int64_t i = lo + threadIdx.x; i < hi; i += 256) acc += d[i * nser + s];
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[blockIdx.x * nser + s] = sh[0];
+        __syncthreads();
+    }
+}
+
 int reduce_sum(corr_ctx* ctx, const double* d_data, int64_t n, int nser, double* h_out) {
     double* d_out = nullptr;
     CORR_TRY(scratch(ctx, "reduce_out", sizeof(double) * 16, (void**)&d_out));
     CORR_REQUIRE(ctx, nser >= 1 && nser <= 16, "reduce_sum: 1..16 series");
+    // Long inputs: 256 fixed chunks first, so the single-workgroup pass is short.
+    constexpr int64_t NCHUNK = 256;
+    if (n > 64 * NCHUNK) {
+        double* d_mid = nullptr;
+        CORR_TRY(scratch(ctx, "reduce_mid", sizeof(double) * 16 * NCHUNK, (void**)&d_mid));
+        hipLaunchKernelGGL(reduce_chunks_kernel, dim3((unsigned)NCHUNK), dim3(256), 0, ctx->stream, d_data, n, nser,
+                           cdiv(n, NCHUNK), d_mid);
+        CORR_LAUNCHED(ctx, "reduce_chunks_kernel");
+        d_data = d_mid;
+        n = NCHUNK;
+    }
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, ctx->stream, d_data, n, nser, d_out);
     CORR_LAUNCHED(ctx, "reduce_kernel");
     CORR_HIP(ctx, hipMemcpyAsync(h_out, d_out, sizeof(double) * nser, hipMemcpyDeviceToHost, ctx->stream));

@@ -59,11 +59,26 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(double* __restrict__ A, 
         val[u] = act[u] ? A[(k0 + si[u]) * lda + (k0 + sl[u])] : 0.0;
         if (act[u] && sl[u] == 0) col[0][si[u]] = val[u];
     }
-    // factorisation
+    // LDS indices of the slots, clamped into the arrays (inactive slots read
+    // a valid element whose value they never use)
+    int ri[SLOTS], rl[SLOTS];
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u) {
+        ri[u] = si[u] < NB ? si[u] : NB - 1;
+        rl[u] = sl[u] < NB ? sl[u] : NB - 1;
+    }
+    // factorisation.  All LDS reads of a step are issued before any update,
+    // with no branch between them, so their latencies overlap.
     for (int j = 0; j < kb; ++j) {
         __syncthreads();
         const double* cj = col[j & 1];
         double dj = cj[j];
+        double a[SLOTS], b[SLOTS];
+#pragma unroll
+        for (int u = 0; u < SLOTS; ++u) {
+            a[u] = cj[ri[u]];
+            b[u] = cj[rl[u]];
+        }
         if (!(dj > 0.0)) {
             if (tid == 0) bad = 1;
             dj = 1.0;
@@ -71,10 +86,10 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(double* __restrict__ A, 
         const double inv = 1.0 / dj;
 #pragma unroll
         for (int u = 0; u < SLOTS; ++u) {
-            if (act[u] && sl[u] > j) {
-                val[u] -= cj[si[u]] * cj[sl[u]] * inv;
-                if (sl[u] == j + 1) col[(j + 1) & 1][si[u]] = val[u];
-            }
+            const bool upd = act[u] && sl[u] > j;
+            const double nv = val[u] - a[u] * b[u] * inv;
+            val[u] = upd ? nv : val[u];
+            if (upd && sl[u] == j + 1) col[(j + 1) & 1][si[u]] = val[u];
         }
     }
     __syncthreads();
@@ -105,9 +120,17 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(double* __restrict__ A, 
                 rk[sl[u]] = val[u];
             }
         __syncthreads();
+        double a[SLOTS], b[SLOTS];
 #pragma unroll
-        for (int u = 0; u < SLOTS; ++u)
-            if (act[u] && si[u] > k && sl[u] <= k) val[u] -= Ls[si[u]][k] * rk[sl[u]];
+        for (int u = 0; u < SLOTS; ++u) {
+            a[u] = Ls[ri[u]][k];
+            b[u] = rk[rl[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < SLOTS; ++u) {
+            const double nv = val[u] - a[u] * b[u];
+            val[u] = (act[u] && si[u] > k && sl[u] <= k) ? nv : val[u];
+        }
     }
     for (int e = tid; e < NB * NB; e += 256) {
         const int r = e / NB, c = e % NB;
