@@ -148,6 +148,23 @@ def test_rimp2_fp64_matches_oracle(corr, naux, o, v):
         assert abs(got[k] - ref[k]) < 1e-10 and abs(got[k] - ref[k]) <= 1e-12 * abs(ref[k]) + 1e-15, k
 
 
+def test_rimp2_many_tiles_two_pass_reduction(corr):
+    """o = 128, v = 256: 528 x 36 = 19008 workgroup partials, past the 16384
+    that one reduction workgroup takes, so the partials go through the fixed
+    256-chunk first pass.  FP64 against the oracle, and repeatable."""
+    naux, o, v = 64, 128, 256
+    nib, nac = (o + 3) // 4, (v + 31) // 32
+    assert nib * (nib + 1) // 2 * (nac * (nac + 1) // 2) > 16384
+    _say(f"rimp2 fp64 two-pass reduction naux={naux} o={o} v={v}")
+    B = S.bia(naux, o, v, seed=11)
+    eo, ev = S.orbital_energies(o, v, seed=12)
+    ref = O.rimp2(B, eo, ev)
+    got = corr.rimp2_energy(dev(B), dev(eo), dev(ev), prec="fp64")
+    for k in ref:
+        assert abs(got[k] - ref[k]) <= 1e-12 * abs(ref[k]) + 1e-15, k
+    assert corr.rimp2_energy(dev(B), dev(eo), dev(ev), prec="fp64") == got
+
+
 @pytest.mark.parametrize("prec,rtol", [("fp32", 2e-6), ("bf16x3", 2e-6)])
 def test_rimp2_reduced_precision(corr, prec, rtol):
     naux, o, v = 304, 20, 76
