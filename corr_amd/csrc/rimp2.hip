@@ -24,6 +24,8 @@
 // a direct global->LDS copy).
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace corr {
 
 namespace {
@@ -239,6 +241,32 @@ __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_real(
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) { acc1[x][y] = acc_t{0, 0, 0, 0}; acc2[x][y] = acc_t{0, 0, 0, 0}; }
+    // FP32: accumulation on two levels, as in the BF16x3 kernel.  The MFMA
+    // accumulators are added into FP32 totals and cleared every TWO_LEVEL
+    // slabs (64 K values), so each rounding chain is 16 MFMA steps plus one
+    // total update per 64 K values instead of naux / 4 steps.
+    constexpr bool TWO_LEVEL = std::is_same<T, float>::value;
+    constexpr int FLUSH_SLABS = 2;
+    acc_t tot1[TWO_LEVEL ? 4 : 1][TWO_LEVEL ? 2 : 1], tot2[TWO_LEVEL ? 4 : 1][TWO_LEVEL ? 2 : 1];
+    if constexpr (TWO_LEVEL) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) { tot1[x][y] = acc_t{0, 0, 0, 0}; tot2[x][y] = acc_t{0, 0, 0, 0}; }
+    }
+    auto flush = [&]() {
+        if constexpr (TWO_LEVEL) {
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    tot1[x][y] += acc1[x][y];
+                    tot2[x][y] += acc2[x][y];
+                    acc1[x][y] = acc_t{0, 0, 0, 0};
+                    acc2[x][y] = acc_t{0, 0, 0, 0};
+                }
+        }
+    };
 
     T reg[4][EL];
     const int64_t nK = Kp / KC;
@@ -297,9 +325,15 @@ __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_real(
                     acc2[rb][cb] = RealMfma<T>::mma(f3[rb % 2], f4[rb / 2][cb], acc2[rb][cb]);
                 }
         }
+        if ((kc + 1) % FLUSH_SLABS == 0) flush();
     }
     double sos = 0.0, sx = 0.0;
-    energy_epilogue(acc1, acc2, t, o, v, eo, ev, sos, sx);
+    if constexpr (TWO_LEVEL) {
+        flush();
+        energy_epilogue(tot1, tot2, t, o, v, eo, ev, sos, sx);
+    } else {
+        energy_epilogue(acc1, acc2, t, o, v, eo, ev, sos, sx);
+    }
     write_partial(sos, sx, t, id, sh, partial);
 }
 
