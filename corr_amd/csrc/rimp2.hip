@@ -198,144 +198,22 @@ __device__ inline void write_partial(double sos, double sx, const PairTile& t, i
 }
 
 // ---- FP64 / FP32 kernel --------------------------------------------------------
+// KS: K depth of one stage (32 KiB of the four panels in LDS).
 template <typename T> struct RealMfma;
 template <> struct RealMfma<double> {
     typedef corr_d4 acc_t;
-    static constexpr int KC = 16;   // 4 MFMA k-steps of 64 cycles per slab
+    static constexpr int KS = 8;    // 2 MFMA k-steps of 64 cycles per stage
     __device__ static inline acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
 };
 template <> struct RealMfma<float> {
     typedef corr_f4 acc_t;
-    static constexpr int KC = 32;   // 8 MFMA k-steps of 32 cycles per slab
+    static constexpr int KS = 16;   // 4 MFMA k-steps of 32 cycles per stage
     __device__ static inline acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
 };
-
-template <typename T>
-__global__ __launch_bounds__(WG_THREADS) void rimp2_energy_real(
-    const T* __restrict__ Bp, const double* __restrict__ eo, const double* __restrict__ ev,
-    int o, int v, int nIB, int nAC, int64_t Kp, int64_t nWG, int64_t per,
-    double* __restrict__ partial) {
-    constexpr int KC = RealMfma<T>::KC, LD = PW + 16, EL = KC * PW / WG_THREADS;
-    __shared__ T L[4][KC][LD];
-    __shared__ double sh[WG_THREADS / 64];
-    typedef typename RealMfma<T>::acc_t acc_t;
-
-    PairTile t;
-    int64_t id;
-    if (!decode_tile(nIB, nAC, nWG, per, t, id)) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave >> 2, wc = wave & 3, cl = lane & 15;
-
-    const T* P[4];
-    P[0] = Bp + ((int64_t)t.IB * nAC + t.AC) * Kp * PW;   // (I, A)
-    P[1] = Bp + ((int64_t)t.JB * nAC + t.BC) * Kp * PW;   // (J, B)
-    P[2] = Bp + ((int64_t)t.JB * nAC + t.AC) * Kp * PW;   // (J, A)
-    P[3] = Bp + ((int64_t)t.IB * nAC + t.BC) * Kp * PW;   // (I, B)
-
-    acc_t acc1[4][2], acc2[4][2];
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) { acc1[x][y] = acc_t{0, 0, 0, 0}; acc2[x][y] = acc_t{0, 0, 0, 0}; }
-    // FP32: accumulation on two levels, as in the BF16x3 kernel.  The MFMA
-    // accumulators are added into FP32 totals and cleared every TWO_LEVEL
-    // slabs (64 K values), so each rounding chain is 16 MFMA steps plus one
-    // total update per 64 K values instead of naux / 4 steps.
-    constexpr bool TWO_LEVEL = std::is_same<T, float>::value;
-    constexpr int FLUSH_SLABS = 2;
-    acc_t tot1[TWO_LEVEL ? 4 : 1][TWO_LEVEL ? 2 : 1], tot2[TWO_LEVEL ? 4 : 1][TWO_LEVEL ? 2 : 1];
-    if constexpr (TWO_LEVEL) {
-#pragma unroll
-        for (int x = 0; x < 4; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y) { tot1[x][y] = acc_t{0, 0, 0, 0}; tot2[x][y] = acc_t{0, 0, 0, 0}; }
-    }
-    auto flush = [&]() {
-        if constexpr (TWO_LEVEL) {
-#pragma unroll
-            for (int x = 0; x < 4; ++x)
-#pragma unroll
-                for (int y = 0; y < 2; ++y) {
-                    tot1[x][y] += acc1[x][y];
-                    tot2[x][y] += acc2[x][y];
-                    acc1[x][y] = acc_t{0, 0, 0, 0};
-                    acc2[x][y] = acc_t{0, 0, 0, 0};
-                }
-        }
-    };
-
-    T reg[4][EL];
-    const int64_t nK = Kp / KC;
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int u = 0; u < EL; ++u) reg[p][u] = P[p][u * WG_THREADS + threadIdx.x];
-
-    // fragment columns inside the 128-wide panels
-    int c1[4], c4[2][2];
-#pragma unroll
-    for (int rb = 0; rb < 4; ++rb) c1[rb] = (2 * wr + rb / 2) * PA + (rb % 2) * 16 + cl;
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) c4[x][y] = (2 * wr + x) * PA + y * 16 + cl;
-    const int c2 = wc * PA + cl;   // + cb * 16 (P2) and + a16 * 16 (P3)
-
-    for (int64_t kc = 0; kc < nK; ++kc) {
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-            for (int u = 0; u < EL; ++u) {
-                const int e = u * WG_THREADS + threadIdx.x;
-                L[p][e / PW][e % PW] = reg[p][u];
-            }
-        __syncthreads();
-        if (kc + 1 < nK) {
-            const int64_t off = (kc + 1) * KC * PW;
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-#pragma unroll
-                for (int u = 0; u < EL; ++u) reg[p][u] = P[p][off + u * WG_THREADS + threadIdx.x];
-        }
-#pragma unroll 2
-        for (int kk = 0; kk < KC; kk += 4) {
-            const int kr = kk + (lane >> 4);
-            T f1[4], f2[2], f3[2], f4[2][2];
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb) f1[rb] = L[0][kr][c1[rb]];
-#pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                f2[y] = L[1][kr][c2 + y * 16];
-                f3[y] = L[2][kr][c2 + y * 16];
-            }
-#pragma unroll
-            for (int x = 0; x < 2; ++x)
-#pragma unroll
-                for (int y = 0; y < 2; ++y) f4[x][y] = L[3][kr][c4[x][y]];
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-                for (int cb = 0; cb < 2; ++cb) {
-                    acc1[rb][cb] = RealMfma<T>::mma(f1[rb], f2[cb], acc1[rb][cb]);
-                    acc2[rb][cb] = RealMfma<T>::mma(f3[rb % 2], f4[rb / 2][cb], acc2[rb][cb]);
-                }
-        }
-        if ((kc + 1) % FLUSH_SLABS == 0) flush();
-    }
-    double sos = 0.0, sx = 0.0;
-    if constexpr (TWO_LEVEL) {
-        flush();
-        energy_epilogue(tot1, tot2, t, o, v, eo, ev, sos, sx);
-    } else {
-        energy_epilogue(acc1, acc2, t, o, v, eo, ev, sos, sx);
-    }
-    write_partial(sos, sx, t, id, sh, partial);
-}
 
 // ---- BF16x3 kernel -------------------------------------------------------------
 // M ~= hi.hi + hi.lo + lo.hi with hi = bf16(x), lo = bf16(x - hi), FP32 accumulate,
@@ -543,6 +421,170 @@ __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_bf16x3(
     }
     write_partial(sos, sx, t, id, sh, partial);
 }
+// ---- FP64 / FP32 kernel: stages -----------------------------------------------
+// A stage holds KS k-rows of the four 128-wide panels, [panel][k][128], moved
+// global -> LDS by global_load_lds_dwordx4 (32 x 1 KiB per stage, four per
+// wave), four stage buffers deep as in the BF16x3 kernel.  Row k is stored
+// rotated by 16 (k % 4) elements, which places the four k-rows that one MFMA
+// step reads in distinct bank groups (what a 16-element row pad did when the
+// stages went through registers).  The rotation is applied on the source
+// side: lane l of a copy writes LDS bytes [16 l, 16 l + 16) of its 1 KiB and
+// loads the global chunk that belongs there.
+template <typename T> using RealStage = T[4][RealMfma<T>::KS][PW];
+
+__device__ inline int real_rot(int k, int c) { return (c + 16 * (k & 3)) & (PW - 1); }
+
+template <typename T>
+__device__ inline void real_stage_copy(RealStage<T>& S, const T* const (&P)[4], int64_t st, int wave, int lane) {
+    constexpr int KS = RealMfma<T>::KS, EPC = 16 / sizeof(T);   // elements per 16-byte chunk
+    constexpr int RPC = 1024 / (PW * (int)sizeof(T));             // k-rows per 1 KiB copy
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int q = wave * 4 + s;                 // copy q of 32: panel q / 8, KiB q % 8
+        const int p = q >> 3, k0 = (q & 7) * RPC;
+        const int k = k0 + lane / (64 / RPC);
+        const int pos = (lane % (64 / RPC)) * EPC;  // stored column of this lane's chunk
+        const int c = (pos - 16 * (k & 3)) & (PW - 1);
+        const T* src = P[p] + (st * KS + k) * (int64_t)PW + c;
+        __builtin_amdgcn_global_load_lds((corr_gbl_ptr)src, (corr_lds_ptr)&S[p][k0][0], 16, 0, 0);
+    }
+}
+
+// Wave (wr, wc) as in the BF16x3 kernel, on 16 x 16 MFMA tiles: rows
+// i = 2 wr + {0,1} x 32 virtuals a (4 row tiles), columns j = wc x 32
+// virtuals b (2 column tiles), for T1 and its exchange partner T2.
+template <typename T>
+__device__ inline void real_stage_compute(const RealStage<T>& S, const int (&c1)[4], int c2, const int (&c4)[2][2],
+                                          int lane, typename RealMfma<T>::acc_t (&acc1)[4][2],
+                                          typename RealMfma<T>::acc_t (&acc2)[4][2]) {
+    constexpr int KS = RealMfma<T>::KS;
+#pragma unroll
+    for (int kk = 0; kk < KS; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        T f1[4], f2[2], f3[2], f4[2][2];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) f1[rb] = S[0][kr][real_rot(kr, c1[rb])];
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            f2[y] = S[1][kr][real_rot(kr, c2 + y * 16)];
+            f3[y] = S[2][kr][real_rot(kr, c2 + y * 16)];
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) f4[x][y] = S[3][kr][real_rot(kr, c4[x][y])];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                acc1[rb][cb] = RealMfma<T>::mma(f1[rb], f2[cb], acc1[rb][cb]);
+                acc2[rb][cb] = RealMfma<T>::mma(f3[rb % 2], f4[rb / 2][cb], acc2[rb][cb]);
+            }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(WG_THREADS) void rimp2_energy_real(
+    const T* __restrict__ Bp, const double* __restrict__ eo, const double* __restrict__ ev,
+    int o, int v, int nIB, int nAC, int64_t Kp, int64_t nWG, int64_t per,
+    double* __restrict__ partial) {
+    constexpr int KS = RealMfma<T>::KS;
+    __shared__ __attribute__((aligned(16))) RealStage<T> L0;
+    __shared__ __attribute__((aligned(16))) RealStage<T> L1;
+    __shared__ __attribute__((aligned(16))) RealStage<T> L2;
+    __shared__ __attribute__((aligned(16))) RealStage<T> L3;
+    __shared__ double sh[WG_THREADS / 64];
+    typedef typename RealMfma<T>::acc_t acc_t;
+
+    PairTile t;
+    int64_t id;
+    if (!decode_tile(nIB, nAC, nWG, per, t, id)) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 2, wc = wave & 3, cl = lane & 15;
+
+    const T* const P[4] = {Bp + ((int64_t)t.IB * nAC + t.AC) * Kp * PW,    // (I, A)
+                           Bp + ((int64_t)t.JB * nAC + t.BC) * Kp * PW,    // (J, B)
+                           Bp + ((int64_t)t.JB * nAC + t.AC) * Kp * PW,    // (J, A)
+                           Bp + ((int64_t)t.IB * nAC + t.BC) * Kp * PW};   // (I, B)
+
+    acc_t acc1[4][2], acc2[4][2];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) { acc1[x][y] = acc_t{0, 0, 0, 0}; acc2[x][y] = acc_t{0, 0, 0, 0}; }
+    // FP32: accumulation on two levels, as in the BF16x3 kernel.  The MFMA
+    // accumulators are added into FP32 totals and cleared every 64 K values,
+    // so each rounding chain is 16 MFMA steps plus one total update per 64 K
+    // values instead of naux / 4 steps.
+    constexpr bool TWO_LEVEL = std::is_same<T, float>::value;
+    acc_t tot1[TWO_LEVEL ? 4 : 1][TWO_LEVEL ? 2 : 1], tot2[TWO_LEVEL ? 4 : 1][TWO_LEVEL ? 2 : 1];
+    if constexpr (TWO_LEVEL) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) { tot1[x][y] = acc_t{0, 0, 0, 0}; tot2[x][y] = acc_t{0, 0, 0, 0}; }
+    }
+    auto flush = [&]() {
+        if constexpr (TWO_LEVEL) {
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    tot1[x][y] += acc1[x][y];
+                    tot2[x][y] += acc2[x][y];
+                    acc1[x][y] = acc_t{0, 0, 0, 0};
+                    acc2[x][y] = acc_t{0, 0, 0, 0};
+                }
+        }
+    };
+
+    // fragment columns inside the 128-wide panels
+    int c1[4], c4[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) c1[rb] = (2 * wr + rb / 2) * PA + (rb % 2) * 16 + cl;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) c4[x][y] = (2 * wr + x) * PA + y * 16 + cl;
+    const int c2 = wc * PA + cl;
+
+    // Stage s lives in buffer s % 4; the schedule of copies and barriers is
+    // the BF16x3 kernel's (copies three stages ahead, vmcnt 8 at each
+    // barrier, the last three steps re-copying the final stage into a buffer
+    // that is never read again).  nK is a multiple of 4 (Kp is one of 64).
+    const int64_t nK = Kp / KS;
+    real_stage_copy<T>(L0, P, 0, wave, lane);
+    real_stage_copy<T>(L1, P, nK > 1 ? 1 : nK - 1, wave, lane);
+    real_stage_copy<T>(L2, P, nK > 2 ? 2 : nK - 1, wave, lane);
+    CORR_STAGE_BARRIER(8)
+#define CORR_RSTAGE(CUR, NEXT3, s_)                                                          \
+    {                                                                                        \
+        real_stage_copy<T>(NEXT3, P, ((s_) + 3 < nK) ? (s_) + 3 : nK - 1, wave, lane);       \
+        real_stage_compute<T>(CUR, c1, c2, c4, lane, acc1, acc2);                            \
+        CORR_STAGE_BARRIER(8)                                                                \
+    }
+    constexpr int FLUSH_GROUPS = 64 / (4 * KS);   // groups of four stages per 64 K values (FP32: 1)
+    for (int64_t s = 0; s < nK; s += 4) {
+        CORR_RSTAGE(L0, L3, s)
+        CORR_RSTAGE(L1, L0, s + 1)
+        CORR_RSTAGE(L2, L1, s + 2)
+        CORR_RSTAGE(L3, L2, s + 3)
+        if constexpr (TWO_LEVEL) {
+            if ((s / 4 + 1) % FLUSH_GROUPS == 0) flush();
+        }
+    }
+#undef CORR_RSTAGE
+    CORR_WAIT_VMCNT(0);   // no copy outlives the loop
+    double sos = 0.0, sx = 0.0;
+    if constexpr (TWO_LEVEL) {
+        flush();
+        energy_epilogue(tot1, tot2, t, o, v, eo, ev, sos, sx);
+    } else {
+        energy_epilogue(acc1, acc2, t, o, v, eo, ev, sos, sx);
+    }
+    write_partial(sos, sx, t, id, sh, partial);
+}
+
 #undef CORR_WAIT_VMCNT
 #undef CORR_STAGE_BARRIER
 
