@@ -93,7 +93,9 @@ def test_molecular_density_guess_converges_to_same_state():
 
 def test_h2o32_mixed_precision_error_real_chemistry():
     """(H2O)32 / def2-SVP / def2-SVP-RI (N = 768, o = 160, naux = 2432): the
-    mixed path (FP32 B formation, BF16x3 energy) stays within 1 uEh of FP64."""
+    mixed path (FP32 B formation, BF16x3 energy) stays within 1 uEh of FP64,
+    and the all-FP32 path (two-level FP32 accumulation in the energy) within
+    0.2 uEh (0.03 measured; 1.38 with one level of accumulation)."""
     say("chem (H2O)32 mixed-precision error")
     from corr_amd import chem
     import corr_amd
@@ -109,6 +111,9 @@ def test_h2o32_mixed_precision_error_real_chemistry():
     say(f"EMP2 fp64 {e64:.10f} mixed {emx:.10f} error {abs(emx - e64) * 1e6:.4f} uEh")
     assert -0.22 * 32 < e64 < -0.19 * 32
     assert abs(emx - e64) <= 1e-6
+    e32 = corr_amd.rimp2_energy(B32, eo, ev, "fp32")["EMP2"]
+    say(f"EMP2 all-FP32 {e32:.10f} error {abs(e32 - e64) * 1e6:.4f} uEh")
+    assert abs(e32 - e64) <= 2e-7
     del r, B32
     torch.cuda.empty_cache()
 
