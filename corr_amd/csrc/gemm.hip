@@ -14,7 +14,14 @@
 // rows and read as two ds_read_b128 per fragment row; an operand contiguous
 // along x is staged as [k][x] and read as scalars.  Global loads move pairs of
 // doubles; LDS holds two slabs, so each slab costs one barrier.
+//
+// FP64 workgroups whose tile lies inside C, on 16-byte aligned operands,
+// stage with global_load_lds_dwordx4 instead (no VGPR round trip, no
+// ds_write), into the same LDS layouts, three slabs deep with one barrier per
+// slab; a K remainder below one slab goes through the register path.
 #include "common.hpp"
+
+#include <type_traits>
 
 namespace corr {
 
@@ -27,6 +34,7 @@ struct GemmArgs {
     int64_t tilesM;
     int a_lower;
     int64_t ksplit;   // > 0: blockIdx.y is a K chunk of this depth (C = partial slice), not a batch index
+    int dma;          // FP64 operands 16-byte aligned (rows, batch and K-chunk offsets): global_load_lds staging
 };
 
 template <typename T> struct Mfma;
@@ -134,17 +142,110 @@ struct Opnd {
     }
 };
 
+typedef __attribute__((address_space(3))) void* gemm_lds_ptr;
+typedef __attribute__((address_space(1))) const void* gemm_gbl_ptr;
+
+// global_load_lds staging of one FP64 operand slab into the register path's
+// LDS layout.  Instruction i of the operand's NI moves up to 1 KiB; lane l
+// writes bytes [16 l, 16 l + 16) from the instruction's LDS base.
+//  KC:  8 rows of [x][16] (128 B each) per instruction; lane l fills row
+//       8 i + l / 8, chunk slot l % 8, which holds logical chunk
+//       gemm_chunk(row, l % 8) (the swizzle is an involution).
+//  !KC: row k of [16][BDIM + XPAD] takes NPR instructions of up to 128
+//       doubles; lanes past the row's end are masked off.
+template <int BDIM, bool KC>
+struct DmaOpnd {
+    static constexpr int BK = 16, LDW = KC ? BK : BDIM + 4;
+    static constexpr int NPR = (BDIM + 127) / 128;
+    static constexpr int NI = KC ? BDIM / 8 : BK * NPR;
+    __device__ static inline void copy(double* S, const double* __restrict__ P, int64_t ld, int64_t x0, int64_t k0,
+                                       int i, int lane) {
+        if (KC) {
+            const int r = 8 * i + (lane >> 3);
+            const int ch = gemm_chunk(r, lane & 7);
+            const double* src = P + (x0 + r) * ld + k0 + 2 * ch;
+            __builtin_amdgcn_global_load_lds((gemm_gbl_ptr)src, (gemm_lds_ptr)(S + 8 * i * BK), 16, 0, 0);
+        } else {
+            const int k = i / NPR, j = i % NPR;
+            const int x = 128 * j + 2 * lane;
+            if (x < BDIM) {
+                const double* src = P + (k0 + k) * ld + x0 + x;
+                __builtin_amdgcn_global_load_lds((gemm_gbl_ptr)src, (gemm_lds_ptr)(S + k * LDW + 128 * j), 16, 0, 0);
+            }
+        }
+    }
+};
+
 // WM x WN waves (8, 10 or 12; two or more per SIMD); each wave owns 64 x 32 of C
-// (4 x 2 MFMA tiles).  BM = 64 WM, BN = 32 WN.
+// (4 x 2 MFMA 16x16 tiles).  BM = 64 WM, BN = 32 WN.
 template <int TA, int TB, typename T, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
-    constexpr int NT = 64 * WM * WN;
-    constexpr int BM = 64 * WM, BN = 32 * WN, BK = GemmSlab<T>::BK, KV = GemmSlab<T>::KV;
+struct GemmTile {
+    static constexpr int NT = 64 * WM * WN;
+    static constexpr int BM = 64 * WM, BN = 32 * WN, BK = GemmSlab<T>::BK, KV = GemmSlab<T>::KV;
     typedef Opnd<BM, TA == 0, NT, T> OA;   // op(A)(m,k): TA==0 -> A[m*lda+k]; TA==1 -> A[k*lda+m]
     typedef Opnd<BN, TB == 1, NT, T> OB;   // op(B)(k,n): TB==0 -> B[k*ldb+n]; TB==1 -> B[n*ldb+k]
-    __shared__ __attribute__((aligned(16))) T As[2][OA::SIZE];
-    __shared__ __attribute__((aligned(16))) T Bs[2][OB::SIZE];
     typedef typename Mfma<T>::acc_t acc_t;
+
+    // MFMAs of one staged slab.
+    __device__ static inline void compute(const T* As, const T* Bs, int wm, int wn, int lg, int lc,
+                                          acc_t (&acc)[4][2]) {
+        constexpr int PER = OA::PER;
+#pragma unroll
+        for (int h = 0; h < KV / PER; ++h) {
+            T fa[4][PER], fb[2][PER];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) OA::frag(As, lg, h, wm * 64 + i * 16 + lc, fa[i]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) OB::frag(Bs, lg, h, wn * 32 + j * 16 + lc, fb[j]);
+#pragma unroll
+            for (int t = 0; t < PER; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(fa[i][t], fb[j][t], acc[i][j]);
+        }
+    }
+
+    // Register-path slabs [kt0, kTiles) through the double buffer (As0/Bs0,
+    // As1/Bs1): global -> registers -> ds_write, loads two slabs ahead.
+    __device__ static inline void regs_loop(const GemmArgs& g, const double* A, const double* B, int64_t m0,
+                                            int64_t n0, int64_t kEnd, int64_t kt0, int64_t kTiles, T* As0, T* As1,
+                                            T* Bs0, T* Bs1, int wm, int wn, int lg, int lc, acc_t (&acc)[4][2]) {
+        corr_d2 ra[OA::PPT], rb[OB::PPT];
+        if (kt0 < kTiles) {
+            OA::load(A, g.lda, m0, g.M, kt0 * BK, kEnd, ra);
+            OB::load(B, g.ldb, n0, g.N, kt0 * BK, kEnd, rb);
+            OA::store(As0, ra);
+            OB::store(Bs0, rb);
+            if (kt0 + 1 < kTiles) {
+                OA::load(A, g.lda, m0, g.M, (kt0 + 1) * BK, kEnd, ra);
+                OB::load(B, g.ldb, n0, g.N, (kt0 + 1) * BK, kEnd, rb);
+            }
+        }
+        __syncthreads();
+        for (int64_t kt = kt0; kt < kTiles; ++kt) {
+            const bool odd = ((kt - kt0) & 1) != 0;
+            if (kt + 1 < kTiles) {
+                OA::store(odd ? As0 : As1, ra);
+                OB::store(odd ? Bs0 : Bs1, rb);
+                if (kt + 2 < kTiles) {
+                    OA::load(A, g.lda, m0, g.M, (kt + 2) * BK, kEnd, ra);
+                    OB::load(B, g.ldb, n0, g.N, (kt + 2) * BK, kEnd, rb);
+                }
+            }
+            compute(odd ? As1 : As0, odd ? Bs1 : Bs0, wm, wn, lg, lc, acc);
+            __syncthreads();
+        }
+    }
+};
+
+template <int TA, int TB, typename T, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
+    typedef GemmTile<TA, TB, T, WM, WN> GT;
+    constexpr int BM = GT::BM, BN = GT::BN, BK = GT::BK;
+    typedef typename GT::OA OA;
+    typedef typename GT::OB OB;
+    typedef typename GT::acc_t acc_t;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / WN, wn = wave % WN, lg = lane >> 4, lc = lane & 15;
@@ -173,44 +274,60 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    corr_d2 ra[OA::PPT], rb[OB::PPT];
-    if (kTiles > 0) {
-        OA::load(A, g.lda, m0, g.M, 0, kEnd, ra);
-        OB::load(B, g.ldb, n0, g.N, 0, kEnd, rb);
-        OA::store(As[0], ra);
-        OB::store(Bs[0], rb);
-        if (kTiles > 1) {
-            OA::load(A, g.lda, m0, g.M, BK, kEnd, ra);
-            OB::load(B, g.ldb, n0, g.N, BK, kEnd, rb);
-        }
-    }
-    __syncthreads();
-    for (int64_t kt = 0; kt < kTiles; ++kt) {
-        const int buf = (int)(kt & 1);
-        constexpr int PER = OA::PER;
-        if (kt + 1 < kTiles) {
-            OA::store(As[buf ^ 1], ra);
-            OB::store(Bs[buf ^ 1], rb);
-            if (kt + 2 < kTiles) {
-                OA::load(A, g.lda, m0, g.M, (kt + 2) * BK, kEnd, ra);
-                OB::load(B, g.ldb, n0, g.N, (kt + 2) * BK, kEnd, rb);
+    if constexpr (std::is_same<T, double>::value) {
+        // Three slab buffers [A slab | B slab] (the register path uses the first two).
+        typedef DmaOpnd<BM, TA == 0> DA;
+        typedef DmaOpnd<BN, TB == 1> DB;
+        constexpr int SLAB = OA::SIZE + OB::SIZE;
+        constexpr int NW = WM * WN, NI = DA::NI + DB::NI, CPW = (NI + NW - 1) / NW;
+        static_assert(CPW <= 15, "gemm: copies per wave exceed the vmcnt field used");
+        __shared__ __attribute__((aligned(16))) double S[3][SLAB];
+        const int64_t kFull = kEnd / BK;
+        if (g.dma && m0 + BM <= g.M && n0 + BN <= g.N && kFull > 0) {
+            // Wave w issues copies w*CPW .. w*CPW+CPW-1 of each slab (the last
+            // ones repeat copy NI-1 so every wave issues CPW).  Slab s lives in
+            // buffer s % 3; copies run two slabs ahead, and the barrier before
+            // slab s+1 waits until at most CPW copies (slab s+2) are in flight.
+            auto copy_slab = [&](int64_t st, int buf) {
+                double* As = S[buf];
+                double* Bs = S[buf] + OA::SIZE;
+#pragma unroll
+                for (int c = 0; c < CPW; ++c) {
+                    int q = wave * CPW + c;
+                    q = q < NI ? q : NI - 1;
+                    if (q < DA::NI) DA::copy(As, A, g.lda, m0, st * BK, q, lane);
+                    else DB::copy(Bs, B, g.ldb, n0, st * BK, q - DA::NI, lane);
+                }
+            };
+            copy_slab(0, 0);
+            copy_slab(kFull > 1 ? 1 : 0, 1);
+            __builtin_amdgcn_s_waitcnt(CPW | (0x7 << 4));   // slab 0 landed (vmcnt CPW, lgkmcnt 0)
+            __builtin_amdgcn_s_barrier();
+            __asm__ volatile("" ::: "memory");
+            int buf = 0;
+            for (int64_t kt = 0; kt < kFull; ++kt) {
+                const int nb2 = buf >= 1 ? buf - 1 : 2;                 // buffer of slab kt+2 (= kt-1's)
+                copy_slab(kt + 2 < kFull ? kt + 2 : kFull - 1, nb2);
+                GT::compute(S[buf], S[buf] + OA::SIZE, wm, wn, lg, lc, acc);
+                __builtin_amdgcn_s_waitcnt(CPW | (0x7 << 4));
+                __asm__ volatile("" ::: "memory");
+                __builtin_amdgcn_s_barrier();
+                __asm__ volatile("" ::: "memory");
+                buf = buf == 2 ? 0 : buf + 1;
             }
+            __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));  // vmcnt 0: no copy outlives the loop
+            __syncthreads();
+            if (kFull < kTiles)   // the K remainder, through registers
+                GT::regs_loop(g, A, B, m0, n0, kEnd, kFull, kTiles, S[0], S[1], S[0] + OA::SIZE, S[1] + OA::SIZE,
+                              wm, wn, lg, lc, acc);
+        } else {
+            GT::regs_loop(g, A, B, m0, n0, kEnd, 0, kTiles, S[0], S[1], S[0] + OA::SIZE, S[1] + OA::SIZE, wm, wn,
+                          lg, lc, acc);
         }
-#pragma unroll
-        for (int h = 0; h < KV / PER; ++h) {
-            T fa[4][PER], fb[2][PER];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) OA::frag(As[buf], lg, h, wm * 64 + i * 16 + lc, fa[i]);
-#pragma unroll
-            for (int j = 0; j < 2; ++j) OB::frag(Bs[buf], lg, h, wn * 32 + j * 16 + lc, fb[j]);
-#pragma unroll
-            for (int t = 0; t < PER; ++t)
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(fa[i][t], fb[j][t], acc[i][j]);
-        }
-        __syncthreads();
+    } else {
+        __shared__ __attribute__((aligned(16))) T As[2][OA::SIZE];
+        __shared__ __attribute__((aligned(16))) T Bs[2][OB::SIZE];
+        GT::regs_loop(g, A, B, m0, n0, kEnd, 0, kTiles, As[0], As[1], Bs[0], Bs[1], wm, wn, lg, lc, acc);
     }
 
 #pragma unroll
@@ -313,6 +430,13 @@ static int dispatch(corr_ctx* ctx, int transa, int transb, int f64, GemmArgs& a,
     }
 }
 
+// FP64 operands whose slab rows, batch strides and K-chunk offsets all start
+// on 16-byte boundaries can be staged by global_load_lds_dwordx4.
+static int dma_ok(int f64, const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB) {
+    const auto al16 = [](const double* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    return f64 && al16(A) && al16(B) && lda % 2 == 0 && ldb % 2 == 0 && sA % 2 == 0 && sB % 2 == 0 ? 1 : 0;
+}
+
 int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
          double alpha, const double* A, int64_t lda, int64_t sA,
          const double* B, int64_t ldb, int64_t sB, double beta, double* C,
@@ -345,6 +469,7 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
             const char* ws = (ctx->aux_stream && ctx->stream == ctx->aux_stream) ? "gemm_splitk_aux" : "gemm_splitk";
             CORR_TRY(scratch(ctx, ws, sizeof(double) * S * M * N, (void**)&P));
             GemmArgs a{M, N, K, A, lda, 0, B, ldb, 0, P, N, M * N, 1.0, 0.0, 0, 0, kc};
+            a.dma = dma_ok(f64, A, lda, 0, B, ldb, 0) && kc % 2 == 0;
             CORR_TRY(dispatch(ctx, transa, transb, f64, a, S));
             const int64_t grid = std::min<int64_t>(cdiv(M * N, 256), 4096);
             hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, P, S, M, N,
@@ -354,6 +479,7 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
         }
     }
     GemmArgs a{M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, alpha, beta, 0, a_lower ? 1 : 0, 0};
+    a.dma = dma_ok(f64, A, lda, sA, B, ldb, sB);
     return dispatch(ctx, transa, transb, f64, a, batch);
 }
 
