@@ -89,6 +89,35 @@ def test_gemm_large_output_tiles(corr, ta, tb, prec):
     assert np.max(np.abs(out - ref)) < tol * np.sqrt(K) * 4
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gemm_fp64_staged_interior_tiles(corr, ta, tb):
+    """Even leading dimensions and 16-byte aligned operands: the interior
+    tiles stage by global_load_lds, the K remainder (1014 = 63 x 16 + 6) and
+    the edge tiles through registers; batched with even strides.  The same
+    product from operands one double off 16-byte alignment takes the register
+    path only; both match numpy."""
+    from corr_amd import _abi
+    from corr_amd._abi import Context, ptr
+    M, N, K, nb = 520, 400, 1014, 3
+    _say(f"gemm fp64 staged ta={ta} tb={tb} M={M} N={N} K={K} batch={nb}")
+    rng = np.random.default_rng(31 + 2 * ta + tb)
+    ash, bsh = ((K, M) if ta else (M, K)), ((N, K) if tb else (K, N))
+    A = rng.standard_normal((nb,) + ash)
+    B = rng.standard_normal((nb,) + bsh)
+    ref = np.stack([(A[b].T if ta else A[b]) @ (B[b].T if tb else B[b]) for b in range(nb)])
+    ctx = Context.get(0)
+    for off in (0, 1):
+        dA = torch.zeros(A.size + 2, dtype=torch.float64, device="cuda")
+        dB = torch.zeros(B.size + 2, dtype=torch.float64, device="cuda")
+        dA[off:off + A.size] = dev(A.ravel())
+        dB[off:off + B.size] = dev(B.ravel())
+        C = torch.zeros(nb, M, N, dtype=torch.float64, device="cuda")
+        ctx.call("corr_dgemm", ta, tb, M, N, K, 1.0, ptr(dA[off:]), ash[1], A[0].size, ptr(dB[off:]), bsh[1],
+                 B[0].size, 0.0, ptr(C), N, M * N, nb, _abi.PRECISIONS["fp64"])
+        got = C.cpu().numpy()
+        assert np.max(np.abs(got - ref)) < 1e-12 * np.sqrt(K) * 4, off
+
+
 @pytest.mark.parametrize("entry", ["ri_metric", "ri_bia_from_j"])
 def test_cholesky_metric_not_positive_definite(corr, entry):
     """A metric with a negative eigenvalue is refused with CORR_ENOTPD (the
