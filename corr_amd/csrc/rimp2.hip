@@ -20,8 +20,8 @@
 // B is repacked once into panels: panel (I, A) is a contiguous
 // [Kp][128] slab (FP64/FP32), or a [Kp/16][hi/lo][128][2][8] slab of BF16
 // halves (BF16x3) already in the energy kernel's LDS order, so every K-slab a
-// workgroup stages into LDS is one contiguous, fully coalesced read (for BF16x3
-// a direct global->LDS copy).
+// workgroup stages into LDS is one contiguous, fully coalesced read, moved by
+// direct global->LDS copies in all three kernels.
 #include "common.hpp"
 
 #include <type_traits>
