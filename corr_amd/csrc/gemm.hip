@@ -281,6 +281,8 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
         constexpr int SLAB = OA::SIZE + OB::SIZE;
         constexpr int NW = WM * WN, NI = DA::NI + DB::NI, CPW = (NI + NW - 1) / NW;
         static_assert(CPW <= 15, "gemm: copies per wave exceed the vmcnt field used");
+        static_assert(DA::LDW == OA::LDW && DB::LDW == OB::LDW && DA::BK == BK,
+                      "gemm: staged and register layouts must agree");
         __shared__ __attribute__((aligned(16))) double S[3][SLAB];
         const int64_t kFull = kEnd / BK;
         if (g.dma && m0 + BM <= g.M && n0 + BN <= g.N && kFull > 0) {
