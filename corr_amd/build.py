@@ -18,9 +18,6 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libcorr_gpu.so"
 DROPIN_LIB = PKG / "libcorr_dropin.so"
-DROPIN_CHECK = PKG / "corr_dropin_check"
-MFMA_PEAK = PKG / "corr_mfma_peak"
-MEM_RATES = PKG / "corr_mem_rates"
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [
@@ -68,10 +65,8 @@ def build(verbose: bool = True) -> Path:
         raise RuntimeError(f"link failed:\n{res.stderr[-8000:]}")
     os.replace(tmp, LIB)
     build_dropin()
-    build_probes()
     if verbose:
-        print(f"built {LIB} from {len(sources)} sources, {DROPIN_LIB.name}, {DROPIN_CHECK.name}, "
-              f"{MFMA_PEAK.name}, {MEM_RATES.name}")
+        print(f"built {LIB} from {len(sources)} sources and {DROPIN_LIB.name}")
     return LIB
 
 
@@ -82,27 +77,13 @@ def _run(cmd: list[str], what: str) -> None:
 
 
 def build_dropin() -> None:
-    """Host-only C++ drop-in layer (reference signatures) over libcorr_gpu.so,
-    and the check program that runs the reference's post-HF sequence through it."""
+    """Host-only C++ drop-in layer (reference signatures) over libcorr_gpu.so."""
     cxx = os.environ.get("CXX", "g++")
     src = PKG / "dropin"
     tmp = DROPIN_LIB.with_suffix(".so.tmp")
     _run([cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", str(src / "corr_dropin.cpp"), "-o", str(tmp),
           f"-L{PKG}", "-lcorr_gpu", "-Wl,-rpath,$ORIGIN"], "g++ (libcorr_dropin.so)")
     os.replace(tmp, DROPIN_LIB)
-    tmp = DROPIN_CHECK.with_suffix(".tmp")
-    _run([cxx, "-O2", "-std=c++17", "-Wall", str(src / "dropin_check.cpp"), "-o", str(tmp), f"-L{PKG}",
-          "-lcorr_dropin", "-lcorr_gpu", "-Wl,-rpath,$ORIGIN"], "g++ (corr_dropin_check)")
-    os.replace(tmp, DROPIN_CHECK)
-
-
-def build_probes() -> None:
-    """Measurement probes (corr_amd/probes/): the MFMA issue rates and the
-    LDS / L2 rates that the kernel rooflines and designs are compared with."""
-    for src, dst in (("mfma_peak.cpp", MFMA_PEAK), ("mem_rates.cpp", MEM_RATES)):
-        tmp = dst.with_suffix(".tmp")
-        _run([hipcc(), *HIPCC_FLAGS, "-x", "hip", str(PKG / "probes" / src), "-o", str(tmp)], f"hipcc ({dst.name})")
-        os.replace(tmp, dst)
 
 
 if __name__ == "__main__":

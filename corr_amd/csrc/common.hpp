@@ -19,6 +19,7 @@ struct corr_ctx {
     void* blas_handle = nullptr;   // rocBLAS handle used by the eigen metric
     hipStream_t aux_stream = nullptr;   // second stream (default priority) for overlapped work
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_switch = nullptr;     // orders a new stream after the previous one
     bool timing = false;
     struct Timed { std::string name; hipEvent_t start, stop; };
     std::vector<Timed> pending;                  // recorded, not yet read

@@ -90,6 +90,7 @@ int corr_ctx_destroy(corr_ctx* ctx) {
     }
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_switch) (void)hipEventDestroy(ctx->ev_switch);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return CORR_OK;
@@ -97,7 +98,15 @@ int corr_ctx_destroy(corr_ctx* ctx) {
 
 int corr_ctx_set_stream(corr_ctx* ctx, void* hip_stream) {
     CORR_CHECK_CTX(ctx);
-    ctx->stream = (hipStream_t)hip_stream;
+    const hipStream_t next = (hipStream_t)hip_stream;
+    if (next == ctx->stream) return CORR_OK;
+    // Work still queued on the old stream may use the context's scratch:
+    // the new stream waits for it before anything enqueued from now on.
+    if (!ctx->ev_switch)
+        CORR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_switch, hipEventDisableTiming));
+    CORR_HIP(ctx, hipEventRecord(ctx->ev_switch, ctx->stream));
+    CORR_HIP(ctx, hipStreamWaitEvent(next, ctx->ev_switch, 0));
+    ctx->stream = next;
     return CORR_OK;
 }
 

@@ -19,7 +19,6 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
-#include <map>
 #include <vector>
 
 #define PWIDTH_L 12
@@ -68,22 +67,11 @@ struct DBuf {
     void to_host(void* host, int64_t bytes) const { check("corr_d2h", corr_d2h(ctx(), host, p, bytes)); }
 };
 
-// Device copies of prec_ints made by MOtrans, keyed by the host array they
-// were downloaded to (the reference never modifies prec_ints afterwards), so
-// that run_canonical_mp2 and ccsd_ur do not upload N^4 values again.
-std::map<const double*, DBuf*>& prec_cache() {
-    static std::map<const double*, DBuf*> m;
-    return m;
-}
-
-const double* device_prec_ints(const double* host, int64_t N, DBuf** owned) {
-    auto it = prec_cache().find(host);
-    if (it != prec_cache().end()) {
-        *owned = nullptr;
-        return it->second->d();
-    }
-    *owned = new DBuf(host, sizeof(double) * N * N * N * N);
-    return (*owned)->d();
+// Device copy of the caller's prec_ints.  It is uploaded on every call: the
+// host array may have been edited, or freed and its address reused, since
+// MOtrans wrote it, and the N^4 upload is small next to the N^4 transform.
+DBuf* device_prec_ints(const double* host, int64_t N) {
+    return new DBuf(host, sizeof(double) * N * N * N * N);
 }
 
 void print_mp2(const char* title, const char* rule, const char* scs_title, const char* scs_rule,
@@ -132,24 +120,23 @@ void MOtrans(systeminfo* sysinfo, OEints* onemats, TEints* twomats, pHF* postHF)
     DBuf dval(twomats->intval, sizeof(double) * (nint > 0 ? nint : 1));
     DBuf dnum(twomats->intnums, sizeof(unsigned short) * 4 * (nint > 0 ? nint : 1));
     DBuf dense(sizeof(double) * n4), work(sizeof(double) * n4);
-    DBuf* out = new DBuf(sizeof(double) * n4);
+    DBuf out(sizeof(double) * n4);
     const int64_t sc[4] = {twomats->sortcount[0], twomats->sortcount[1], twomats->sortcount[2],
                            twomats->sortcount[3]};
     check("corr_ao_unfold", corr_ao_unfold(ctx(), N, nint, sc, dval.d(), (const uint16_t*)dnum.p, dense.d()));
-    check("corr_motrans", corr_motrans(ctx(), N, dC.d(), dense.d(), work.d(), out->d(), prec_from_env() == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_PREC_FP32));
-    out->to_host(postHF->prec_ints, sizeof(double) * n4);
-    prec_cache()[postHF->prec_ints] = out;
+    check("corr_motrans", corr_motrans(ctx(), N, dC.d(), dense.d(), work.d(), out.d(),
+                                       prec_from_env() == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_PREC_FP32));
+    out.to_host(postHF->prec_ints, sizeof(double) * n4);
 }
 
 // can_mp2.cc:11-49
 void run_canonical_mp2(systeminfo* sysinfo, pHF* postHF) {
     const int N = sysinfo->nroao, o = sysinfo->nroe / 2;
-    DBuf* owned = nullptr;
-    const double* dI = device_prec_ints(postHF->prec_ints, N, &owned);
+    DBuf* dI = device_prec_ints(postHF->prec_ints, N);
     DBuf dF(postHF->FMo, sizeof(double) * N * N);
     corr_mp2_energy e;
-    check("corr_canonical_mp2", corr_canonical_mp2(ctx(), N, o, dI, dF.d(), &e));
-    delete owned;
+    check("corr_canonical_mp2", corr_canonical_mp2(ctx(), N, o, dI->d(), dF.d(), &e));
+    delete dI;
     print_mp2("\nMP2-Results:\n", "------------\n", "\nSCS-MP2-Results:\n", "------------\n", e);
 }
 
@@ -202,12 +189,11 @@ void read_transform_ri(std::string prefix, int nroe, int nroao, int naux_2, doub
 void run_non_canonical_mp2(int nroao, int nroe, double* prec_ints, double* FMo) {
     const int64_t N = nroao, o = nroe / 2;
     std::cout << "\nCalculating semi-canonical amplitudes ... ";
-    DBuf* owned = nullptr;
-    const double* dI = device_prec_ints(prec_ints, N, &owned);
+    DBuf* dI = device_prec_ints(prec_ints, N);
     DBuf dF(FMo, sizeof(double) * N * N);
     std::vector<double> E(20), R(20);
-    check("corr_non_canonical_mp2", corr_non_canonical_mp2(ctx(), N, o, dI, dF.d(), 20, E.data(), R.data(), nullptr));
-    delete owned;
+    check("corr_non_canonical_mp2", corr_non_canonical_mp2(ctx(), N, o, dI->d(), dF.d(), 20, E.data(), R.data(), nullptr));
+    delete dI;
     std::cout << "done" << '\n';
     for (int it = 0; it < 20; ++it)
         std::cout << std::fixed << std::setw(10) << std::setprecision(10) << "E(sem-loc):" << std::setw(16) << E[it]
@@ -306,8 +292,7 @@ void ccsd_ur(systeminfo* sysinfo, OEints* onemats, pHF* postHF) {
     std::cout << "Allocating " << mem_int / 1024 / 1024 << "MB(" << mem_int / 1024 / 1024 / 1024 << "Gb) for intermediates\n";
     std::cout.flush();
 
-    DBuf* owned = nullptr;
-    const double* dI = device_prec_ints(postHF->prec_ints, N, &owned);
+    DBuf* dI = device_prec_ints(postHF->prec_ints, N);
     DBuf dC(onemats->MOs, sizeof(double) * N * N), dH(onemats->Hmat, sizeof(double) * N * N);
     const int64_t L1 = (int64_t)nocc * nvir, L2 = L1 * L1;
     DBuf dT1(sizeof(double) * L1), dT2(sizeof(double) * L2);
@@ -321,7 +306,7 @@ void ccsd_ur(systeminfo* sysinfo, OEints* onemats, pHF* postHF) {
     opts.r_tol = 1e-6;
     std::vector<double> energies(opts.maxiter);
     corr_ccsd_result r;
-    const int st = corr_ccsd(ctx(), N, nocc, dI, dC.d(), dH.d(), sysinfo->ion_rep, &opts, energies.data(), dT1.d(),
+    const int st = corr_ccsd(ctx(), N, nocc, dI->d(), dC.d(), dH.d(), sysinfo->ion_rep, &opts, energies.data(), dT1.d(),
                              dT2.d(), &r);
     if (st != CORR_OK && st != CORR_ENOTCONV) die("corr_ccsd", st);
     // <pq||rs> as the reference leaves it in postHF->ints_so (ccsd.cc:514)
@@ -329,10 +314,10 @@ void ccsd_ur(systeminfo* sysinfo, OEints* onemats, pHF* postHF) {
         const int64_t n4 = (int64_t)norb * norb * norb * norb;
         postHF->ints_so = new double[(size_t)n4];
         DBuf dV(sizeof(double) * n4);
-        check("corr_ccsd_ints_so", corr_ccsd_ints_so(ctx(), N, dI, dV.d()));
+        check("corr_ccsd_ints_so", corr_ccsd_ints_so(ctx(), N, dI->d(), dV.d()));
         dV.to_host(postHF->ints_so, sizeof(double) * n4);
     }
-    delete owned;
+    delete dI;
     std::cout << "El:" << r.e_hf_check << std::endl;
     std::cout << "one:" << r.e_one_check << std::endl;
     std::cout << "fsum-0.5twoe:" << r.e_fsum_check;

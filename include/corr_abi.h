@@ -80,7 +80,12 @@ int         corr_version(void);
 int         corr_ctx_create(int device, corr_ctx** out);
 int         corr_ctx_destroy(corr_ctx* ctx);
 /* Enqueue on an existing hipStream_t (passed as void*); NULL is the HIP
- * default (null) stream.  A new context uses a stream of its own. */
+ * default (null) stream.  A new context uses a stream of its own.  The
+ * context's device scratch is shared by every call on it, and several entry
+ * points return before their kernels finish, so a change of stream orders
+ * the new stream after all work already enqueued on the old one (an event
+ * recorded on the old stream, waited for by the new one): calls on one
+ * context stay one ordered queue whichever streams they name. */
 int         corr_ctx_set_stream(corr_ctx* ctx, void* hip_stream);
 int         corr_ctx_synchronize(corr_ctx* ctx);
 const char* corr_ctx_last_error(const corr_ctx* ctx);

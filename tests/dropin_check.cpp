@@ -9,7 +9,7 @@
 // rebuild the Fock matrix from the final MOs ("Fcheck:" line: its trace and
 // sum of squares).  With --ncmp2, run_non_canonical_mp2 (loc_mp2.cc, not
 // called by main.cc) runs after MOtrans.
-#include "../../include/corr_dropin.h"
+#include "../include/corr_dropin.h"
 
 #include <cstdint>
 #include <fstream>

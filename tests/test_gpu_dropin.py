@@ -11,6 +11,7 @@ import pytest
 
 from oracle import ccsd as CC
 from oracle import posthf as O
+from tests import _harness
 from tests import _synth as S
 
 pytestmark = pytest.mark.gpu
@@ -28,7 +29,12 @@ def write_input(path, N, nroe, naux, ion_rep, C, F, H, val, idx, sc, J, Pmn):
             np.ascontiguousarray(a, dtype=np.float64).tofile(f)
 
 
-def test_dropin_sequence_matches_oracle(tmp_path):
+@pytest.mark.parametrize("sanitize", [False, True], ids=["plain", "asan_ubsan"])
+def test_dropin_sequence_matches_oracle(tmp_path, sanitize):
+    """With sanitize=True the check program and the drop-in layer are built with
+    AddressSanitizer and UndefinedBehaviorSanitizer (host code only; the
+    kernels in libcorr_gpu.so are not instrumented) and must run clean."""
+    import os
     N, nocc_sp, naux = 8, 2, 24
     h_ao, eri = S.model_hamiltonian(N, seed=31)
     C, eps, F = S.rhf_orthonormal(h_ao, eri, nocc_sp)
@@ -39,11 +45,16 @@ def test_dropin_sequence_matches_oracle(tmp_path):
     ion = 9.1875
     inp = tmp_path / "sys.bin"
     write_input(inp, N, 2 * nocc_sp, naux, ion, C, F, h_ao, val, idx, sc, J, Pmn)
-    print("running corr_dropin_check", flush=True)
-    res = subprocess.run([str(ROOT / "corr_amd" / "corr_dropin_check"), str(inp)], capture_output=True, text=True,
-                         timeout=120)
+    exe = _harness.dropin_check(sanitize=sanitize)
+    print(f"running {exe.name}", flush=True)
+    env = dict(os.environ)
+    if sanitize:
+        # the HIP runtime keeps allocations until exit; leaks of the host code are checked on the CPU
+        env.update(ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="print_stacktrace=1")
+    res = subprocess.run([str(exe), str(inp)], capture_output=True, text=True, timeout=120, env=env)
     sys.stdout.write(res.stdout[-3000:])
     assert res.returncode == 0, res.stderr
+    assert "AddressSanitizer" not in res.stderr and "runtime error" not in res.stderr, res.stderr[-3000:]
     out = res.stdout
     vals = [float(x) for x in re.findall(r"^ +EMP2:\s+(-?\d+\.\d+)", out, re.M)]
     assert len(vals) == 2
@@ -88,7 +99,7 @@ def test_dropin_scf_then_post_hf_matches_oracle(tmp_path):
     with open(inp, "ab") as f:
         np.ascontiguousarray(Sm, dtype=np.float64).tofile(f)
     print("running corr_dropin_check --scf", flush=True)
-    res = subprocess.run([str(ROOT / "corr_amd" / "corr_dropin_check"), str(inp), "--scf"], capture_output=True,
+    res = subprocess.run([str(_harness.dropin_check()), str(inp), "--scf"], capture_output=True,
                          text=True, timeout=120)
     sys.stdout.write(res.stdout[-3000:])
     assert res.returncode == 0, res.stderr
@@ -131,7 +142,7 @@ def test_dropin_scf_fock_and_non_canonical_mp2(tmp_path):
     with open(inp, "ab") as f:
         np.ascontiguousarray(Sm, dtype=np.float64).tofile(f)
     print("running corr_dropin_check --scf --ncmp2", flush=True)
-    res = subprocess.run([str(ROOT / "corr_amd" / "corr_dropin_check"), str(inp), "--scf", "--ncmp2"],
+    res = subprocess.run([str(_harness.dropin_check()), str(inp), "--scf", "--ncmp2"],
                          capture_output=True, text=True, timeout=120)
     sys.stdout.write(res.stdout[-2000:])
     assert res.returncode == 0, res.stderr
