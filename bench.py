@@ -14,8 +14,10 @@ naux = 2432 auxiliary functions.  Three paths are timed, K steps each:
   fp32:  as mixed with the energy in FP32 MFMA (reported beside, with its
          error against fp64)
 
-value = algorithmic TFLOP/s of the mixed path, (W_B + W_E) / t_step, with
-W_E = 2 [o(o+1)/2] v^2 naux and W_B = 2 naux o N^2 + 2 naux o v N + 2 naux^2 o v.
+value = algorithmic TFLOP/s of the mixed path, (W_B_chol + W_E) / t_step, with
+W_E = 2 [o(o+1)/2] v^2 naux and W_B_chol = 2 naux o N^2 + 2 naux o v N + naux^2 o v
+(the metric product with the triangular Cholesky factor; a dense J^-1/2 makes
+it 2 naux^2 o v, W_B).
 
 Inputs (--data real, the default) are real chemistry computed on the GPU
 before the timed region: a seeded (H2O)32 cluster (rigid waters, O-H 0.958 A,
@@ -56,9 +58,14 @@ def sizes(nmol: int):
 
 
 def work(N, o, v, naux):
+    """Algorithmic FLOPs: W_E of the pair-symmetric energy, and W_B of B
+    formation (AO->MO products, then the metric product) with a dense metric
+    J^-1/2.  The Cholesky path applies the triangular L^-1, whose product is
+    half as much work: W_B_chol."""
     W_E = 2.0 * (o * (o + 1) // 2) * v * v * naux
     W_B = 2.0 * naux * o * N * N + 2.0 * naux * o * v * N + 2.0 * naux * naux * o * v
-    return W_E, W_B
+    W_B_chol = W_B - 1.0 * naux * naux * o * v
+    return W_E, W_B, W_B_chol
 
 
 def water_spectrum(nmol: int, rng: np.random.Generator):
@@ -141,8 +148,9 @@ def main() -> int:
         del x, y
         log("calibration copy of 1 GiB done")
     N, o, v, naux = sizes(args.nmol)
-    W_E, W_B = work(N, o, v, naux)
-    log(f"(H2O){args.nmol}: N={N} o={o} v={v} naux={naux}; W_E={W_E/1e12:.3f} TF W_B={W_B/1e12:.3f} TF")
+    W_E, W_B, W_Bc = work(N, o, v, naux)
+    log(f"(H2O){args.nmol}: N={N} o={o} v={v} naux={naux}; W_E={W_E/1e12:.3f} TF W_B={W_B/1e12:.3f} TF "
+        f"(Cholesky metric: {W_Bc/1e12:.3f} TF)")
     inputs, real = {}, None
     if args.data == "real":
         from corr_amd import chem
@@ -172,6 +180,18 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"synthetic inputs ready (unscaled E={e0:.4f})")
     ctx = Context.get(0)
+    # The reference's metric (ri.cc:50-69) is J^-1/2 by eigendecomposition
+    # with a cutoff; the step uses the Cholesky factor.  Both give the same
+    # B B^T when no eigenvalue is cut: record the kept count and the FP64
+    # energies of the two paths (outside the timed region).
+    _, kept = corr_amd.ri_metric(J, "eigen")
+    e_eig = corr_amd.rimp2_energy(corr_amd.ri_bia_from_j(C, Pmn, J, o, "eigen", "fp64")[0], eo, ev, "fp64")["EMP2"]
+    e_chol = corr_amd.rimp2_energy(corr_amd.ri_bia_from_j(C, Pmn, J, o, "cholesky", "fp64")[0], eo, ev,
+                                   "fp64")["EMP2"]
+    torch.cuda.empty_cache()
+    metric_info = {"step": "cholesky", "eigen_kept": kept, "naux": naux, "emp2_fp64_eigen": e_eig,
+                   "emp2_fp64_cholesky": e_chol, "eigen_minus_cholesky_Eh": e_eig - e_chol}
+    log(f"metric: eigen keeps {kept} of {naux}; FP64 EMP2 eigen - cholesky = {e_eig - e_chol:.3e} Eh")
 
     def step(prec_gemm: str, prec_e: str):
         # metric factorisation overlapped with the AO->MO GEMMs, then the energy
@@ -193,7 +213,7 @@ def main() -> int:
         ms, n = ctx.kernel_time("rimp2_energy")
         ctx.enable_timing(False)
         results[name] = {"s_per_step": t, "energy": e, "kernel_ms": ms / max(n, 1), "launches": n}
-        log(f"{name}: {t*1e3:.2f} ms/step, energy kernel {ms/max(n,1):.2f} ms, EMP2={e['EMP2']:.10f}")
+        log(f"{name}: {t*1e3:.2f} ms/step, energy kernel {ms/max(n,1):.2f} ms, EMP2={e['EMP2']:.15f}")
 
     mixed, f64 = results["mixed"], results["fp64"]
     err_uEh = abs(mixed["energy"]["EMP2"] - f64["energy"]["EMP2"]) * 1e6
@@ -203,11 +223,11 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(C, Pmn, J, o, eo, ev, W_E, args.cpu_seconds)
+        cpu = cpu_baseline(C, Pmn, J, o, eo, ev, W_E + W_Bc, args.cpu_seconds)
 
     line = {
         "metric": "RI-MP2 s & TFLOP/s, (H2O)32/def2-SVP: mixed vs FP64 GPU at <=1 uEh error",
-        "value": round((W_E + W_B) / mixed["s_per_step"] / 1e12, 3),
+        "value": round((W_E + W_Bc) / mixed["s_per_step"] / 1e12, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -229,14 +249,18 @@ def main() -> int:
         "error_uEh": round(err_uEh, 4),
         "emp2_fp64": f64["energy"]["EMP2"],
         "emp2_mixed": mixed["energy"]["EMP2"],
-        "fp64_tflops": round((W_E + W_B) / f64["s_per_step"] / 1e12, 3),
+        "fp64_tflops": round((W_E + W_Bc) / f64["s_per_step"] / 1e12, 3),
+        "work_TFLOP": {"W_E": round(W_E / 1e12, 4), "W_B_cholesky": round(W_Bc / 1e12, 4),
+                       "W_B_dense_metric": round(W_B / 1e12, 4),
+                       "note": "value = (W_E + W_B_cholesky) / t; the step applies the triangular Cholesky factor"},
+        "metric": metric_info,
         "fp32": {"s": round(results["fp32"]["s_per_step"], 5),
                  "speedup_vs_fp64": round(f64["s_per_step"] / results["fp32"]["s_per_step"], 3),
                  "error_uEh": round(abs(results["fp32"]["energy"]["EMP2"] - f64["energy"]["EMP2"]) * 1e6, 4),
                  "note": "B formation and energy in FP32 MFMA (FP32 accumulation, FP64 denominators/reduction)"},
-        "roofline": {"bound": "mfma", "kernel": "rimp2_energy_bf16x3", "achieved": round(achieved, 2),
+        "roofline": {"bound": "mfma", "kernel": "rimp2_energy_bf16x3_wide", "achieved": round(achieved, 2),
                      "peak": 2500.0, "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4),
-                     "traffic": recorded_traffic("rimp2_energy_bf16x3"),
+                     "traffic": recorded_traffic("rimp2_energy_bf16x3_wide"),
                      "traffic_source": "profiles/rimp2_energy_h2o32.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
                                        "scaled by a 1 GiB copy)",
                      "work": "3 x W_E (three BF16 products per FP32 product)"},
@@ -275,66 +299,6 @@ def _timed(fn, reps: int = 3):
     return float(np.median(ts))
 
 
-def unique_list_gpu(N: int, device, seed: int, dense=None):
-    """Canonical AO ERI quadruples (a>=b, c>=d, ab>=cd) in the reference's
-    stored form: resort step 1 (ops_rhf.cc:766-818) and the five permutation
-    classes (ops_rhf.cc:820-896).  Values are the entries of `dense` (the
-    molecule's (mu nu|la si)) or seeded random numbers.  Returns
-    (intval, intnums int16 [n,4], sortcount)."""
-    import torch
-    P = N * (N + 1) // 2
-    T = P * (P + 1) // 2
-    k = torch.arange(T, dtype=torch.int64, device=device)
-
-    def tri(x):
-        r = torch.floor((torch.sqrt(8.0 * x.double() + 1.0) - 1.0) / 2.0).long()
-        r = torch.where(r * (r + 1) // 2 > x, r - 1, r)
-        r = torch.where((r + 1) * (r + 2) // 2 <= x, r + 1, r)
-        return r, x - r * (r + 1) // 2
-
-    p, q = tri(k)
-    del k
-    a, b = tri(p)
-    c, d = tri(q)
-    del p, q
-    oa, ob, oc, od = a.clone(), b.clone(), c.clone(), d.clone()
-
-    def assign(m, w, x, y, z):
-        a[m], b[m], c[m], d[m] = w[m], x[m], y[m], z[m]
-
-    assign((oa == ob) & (ob == od) & (oc != od), oa, oa, oa, oc)
-    assign((oa != ob) & (oa == oc) & (oc == od), oa, oa, oa, ob)
-    assign((oa != ob) & (ob == oc) & (oc == od), ob, ob, ob, oa)
-    assign((oa == od) & (ob == oc) & (oa != ob), oa, ob, od, oc)
-    assign((ob == oc) & (ob != oa) & (ob != od) & (oa != od), ob, oa, oc, od)
-    assign((oc == od) & (oc != oa) & (oc != ob) & (oa != ob), oc, oc, oa, ob)
-    assign((oa == od) & (oa != ob) & (oa != oc) & (ob != oc), oa, ob, od, oc)
-    assign((ob == od) & (ob != oa) & (ob != oc) & (oa != oc), ob, oa, od, oc)
-    del oa, ob, oc, od
-    cls = torch.full_like(a, 4)
-    m0 = (a == b) & (b == c) & (c == d)
-    m1 = ~m0 & (a == b) & (c == d)
-    m2 = ~m0 & ~m1 & (a == c) & (b == d)
-    m3 = ~m0 & ~m1 & ~m2 & (a == b)
-    cls[m3] = 3
-    cls[m2] = 2
-    cls[m1] = 1
-    cls[m0] = 0
-    order = torch.sort(cls, stable=True).indices
-    sortcount = [int((cls <= j).sum()) for j in range(4)]
-    nums = torch.stack([a, b, c, d], dim=1)[order].to(torch.int16)
-    del a, b, c, d, cls, order
-    if dense is not None:
-        q = nums.to(torch.int64)
-        val = dense[q[:, 0], q[:, 1], q[:, 2], q[:, 3]].contiguous()
-        del q
-    else:
-        g = torch.Generator(device=device)
-        g.manual_seed(seed)
-        val = torch.randn(T, dtype=torch.float64, device=device, generator=g) * 1e-2
-    return val, nums.contiguous(), sortcount
-
-
 def run_suite(dev, seed: int, rows=(), main=None) -> dict:
     """Every other row at its own shape, on real chemistry computed on the GPU
     (seeded water clusters, def2-SVP / def2-SVP-RI, exact-integral RHF up to
@@ -343,6 +307,7 @@ def run_suite(dev, seed: int, rows=(), main=None) -> dict:
     import corr_amd
     from corr_amd import chem
     from corr_amd._abi import Context
+    from corr_amd.pipeline import sorted_unique_list
     out = {"peaks_TFLOPs": {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}}
 
     def rec(name, fn):
@@ -400,7 +365,7 @@ def run_suite(dev, seed: int, rows=(), main=None) -> dict:
         h2o8.update(r)
         N, o = r["nbf"], r["nocc"]
         G = r["G"]
-        val, nums, sc = unique_list_gpu(N, dev, seed, dense=G)
+        val, nums, sc = sorted_unique_list(G)
         n_unique = val.numel()
         t_unf = _timed(lambda: corr_amd.ao_unfold(N, val, nums, sc), reps=2)
         dense = corr_amd.ao_unfold(N, val, nums, sc)
@@ -483,11 +448,14 @@ def run_suite(dev, seed: int, rows=(), main=None) -> dict:
         res["metric_eigen_ms"] = round(_timed(lambda: corr_amd.ri_metric(J, "eigen")) * 1e3, 2)
         res["metric_cholesky_ms"] = round(_timed(lambda: corr_amd.ri_metric(J, "cholesky")) * 1e3, 2)
         M, _ = corr_amd.ri_metric(J, "cholesky")
-        WB = work(N, o, v, naux)[1]
+        _, WB, WBc = work(N, o, v, naux)
+        res["W_B_cholesky_TFLOP"] = round(WBc / 1e12, 4)
+        res["W_B_dense_metric_TFLOP"] = round(WB / 1e12, 4)
         for prec in ("fp64", "fp32"):
             t = _timed(lambda: corr_amd.ri_form_bia(C, Pmn, M, o, "cholesky", prec))
             res[f"form_bia_{prec}_ms"] = round(t * 1e3, 2)
-            res[f"form_bia_{prec}_TFLOPs"] = round(WB / t / 1e12, 2)
+            # the triangular metric product is credited at its own (half) work
+            res[f"form_bia_{prec}_TFLOPs"] = round(WBc / t / 1e12, 2)
             t = _timed(lambda: corr_amd.ri_bia_from_j(C, Pmn, J, o, "cholesky", prec))
             res[f"bia_from_J_overlapped_{prec}_ms"] = round(t * 1e3, 2)
         return res
@@ -571,10 +539,16 @@ def recorded_traffic(kernel: str):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline(C, Pmn, J, o, eo, ev, W_E, budget_s):
-    """Times the oracle's restatement of run_canonical_mp2_ri (one dgemm per
-    ordered pair, can_mp2.cc:82-112) on the host over a bounded sample of
-    pairs, and extrapolates to the full o^2 pairs."""
+def cpu_baseline(C, Pmn, J, o, eo, ev, W, budget_s):
+    """Times the oracle's restatement of the step's region in the reference's
+    own order on the host, each part on a bounded sample, extrapolated:
+      J^-1/2 by eigendecomposition with the cutoff (ri.cc:50-69), whole;
+      BPQ = J^-1/2 (P|mu nu) (ri.cc:102-103), one dgemm, on a slice of mu;
+      transform_ri (ri.cc:147-172) on a sample of Q slices;
+      run_canonical_mp2_ri (can_mp2.cc:82-112), one dgemm per ordered pair,
+      on a sample of pairs (about 60% of the budget).
+    numpy / OpenBLAS on all host threads (the reference runs transform_ri on
+    one thread).  value = W / t with the GPU step's algorithmic W."""
     import torch
     import corr_amd
     from oracle import posthf as O
@@ -584,28 +558,58 @@ def cpu_baseline(C, Pmn, J, o, eo, ev, W_E, budget_s):
                        if p.get("internal_api") == "openblas"), default=os.cpu_count())
     except Exception:
         threads = os.cpu_count()
+    naux, N = Pmn.shape[0], Pmn.shape[1]
+    v = N - o
+    parts = {}
+    Jh = J.cpu().numpy()
+    t0 = time.perf_counter()
+    Mh, _ = O.ri_metric_eigen(Jh)
+    parts["metric_eigen"] = (time.perf_counter() - t0, 1.0, "whole")
+    # BPQ on a slice of mu rows
+    s_mu = 24
+    P_s = Pmn[:, :s_mu, :].contiguous().cpu().numpy()
+    t0 = time.perf_counter()
+    Mh @ P_s.reshape(naux, -1)   # the dgemm of O.ri_bpq (ri.cc:102-103) on s_mu rows of (P|mu nu)
+    parts["bpq_dgemm"] = (time.perf_counter() - t0, N / s_mu, f"{s_mu} of {N} mu rows")
+    del P_s
+    # transform_ri on Q slices of the real BPQ
+    nq = 96
+    Mg = torch.from_numpy(Mh[:nq]).to(Pmn.device)
+    BPQ_s = (Mg @ Pmn.view(naux, -1)).view(nq, N, N).cpu().numpy()
+    Ch = C.cpu().numpy()
+    t0 = time.perf_counter()
+    O.transform_ri(Ch, BPQ_s, o)
+    parts["transform_ri"] = (time.perf_counter() - t0, naux / nq, f"{nq} of {naux} Q slices")
+    del BPQ_s, Mg
+    # energy on a sample of ordered pairs of the real B
     M, _ = corr_amd.ri_metric(J, "cholesky")
     Bia = corr_amd.ri_form_bia(C, Pmn, M, o, "cholesky", "fp64").cpu().numpy()
     eo_h, ev_h = eo.cpu().numpy(), ev.cpu().numpy()
     rng = np.random.default_rng(0)
     pairs_all = [(i, j) for i in range(o) for j in range(o)]
     order = rng.permutation(len(pairs_all))
-    done, t0 = 0, time.perf_counter()
     O.rimp2(Bia, eo_h, ev_h, pairs=[pairs_all[order[0]]])   # warm-up
+    budget_e = max(budget_s - sum(p[0] for p in parts.values()), 0.3 * budget_s)
+    done, batch = 0, 16
     t0 = time.perf_counter()
-    batch = 16
-    while time.perf_counter() - t0 < budget_s and done < len(pairs_all):
+    while time.perf_counter() - t0 < budget_e and done < len(pairs_all):
         sel = [pairs_all[k] for k in order[done:done + batch]]
         O.rimp2(Bia, eo_h, ev_h, pairs=sel)
         done += len(sel)
-    t = time.perf_counter() - t0
-    full = t * len(pairs_all) / done
-    log(f"cpu baseline: {done} of {len(pairs_all)} pairs in {t:.1f} s -> {full:.1f} s extrapolated")
-    return {"value": round(W_E / full / 1e12, 5), "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
-            "sample": f"{done} of {len(pairs_all)} ordered (i,j) pairs of the RI-MP2 energy "
-                      f"(oracle restatement of can_mp2.cc:82-112, numpy/OpenBLAS dgemm per pair) "
-                      f"in {t:.1f} s; full energy extrapolated to {full:.0f} s; B formation not included",
-            "seconds_full_energy_extrapolated": round(full, 1)}
+    parts["rimp2_energy"] = (time.perf_counter() - t0, len(pairs_all) / done, f"{done} of {len(pairs_all)} pairs")
+    sampled = sum(p[0] for p in parts.values())
+    full = {k: p[0] * p[1] for k, p in parts.items()}
+    total = sum(full.values())
+    log("cpu baseline: " + ", ".join(f"{k} {parts[k][0]:.1f} s ({parts[k][2]}) -> {full[k]:.1f} s" for k in parts)
+        + f"; step {total:.1f} s")
+    return {"value": round(W / total / 1e12, 5), "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
+            "sample": "oracle restatement of the step in the reference's order (eigen metric ri.cc:50-69; "
+                      "BPQ dgemm ri.cc:102-103; transform_ri ri.cc:147-172; per-pair dgemm energy "
+                      "can_mp2.cc:82-112), numpy/OpenBLAS on the host: "
+                      + "; ".join(f"{k}: {parts[k][2]} in {parts[k][0]:.2f} s" for k in parts)
+                      + f"; {sampled:.1f} s sampled, step extrapolated to {total:.0f} s",
+            "seconds_full_step_extrapolated": round(total, 1),
+            "seconds_by_part_extrapolated": {k: round(x, 2) for k, x in full.items()}}
 
 
 if __name__ == "__main__":

@@ -6,24 +6,28 @@
 //   D = e_a + e_b - e_i - e_j
 //
 // The reference loops over all o^2 ordered pairs with one dgemm each.  Here
-// the o*v x o*v matrix G[(i,a),(j,b)] = M^{ij}_ab is tiled with 128x128 tiles
-// whose rows are (4 occupied) x (32 virtual) and whose columns are likewise
-// (4 occupied) x (32 virtual).  A workgroup owns one tile T1 = G[I x A, J x B]
-// and its exchange partner T2[(i,a),(j,b)] = M^{ij}_ba computed as
-// B[:, J x A]^T B[:, I x B], so that M_ab and M_ba of every element sit in the
-// same lane and register: the exchange energy needs no data movement.
-// Only I <= J and A <= B tiles are launched; weights restore the ordered-pair
-// sum of the reference exactly:
-//   I <  J: x2 (the (j,i) pair gives the same sum with a<->b)
-//   A == B: x1/2 (T1 and T2 cover the same elements)
+// the o*v x o*v matrix G[(i,a),(j,b)] = M^{ij}_ab is tiled.  A workgroup owns
+// a tile T1 = G[I x A, J x B] and its exchange partner T2[(i,a),(j,b)] =
+// M^{ij}_ba computed as B[:, J x A]^T B[:, I x B], so that M_ab and M_ba of
+// every element meet in one workgroup.  Only occupied pairs i <= j and virtual
+// blocks A <= B are computed; weights restore the reference's ordered-pair sum
+// exactly: x2 for i < j (the (j,i) pair gives the same sum with a<->b), x1/2 on
+// A == B tiles (T1 and T2 cover the same elements).
 //
-// B is repacked once into panels: panel (I, A) is a contiguous
-// [Kp][128] slab (FP64/FP32), or a [Kp/16][hi/lo][128][2][8] slab of BF16
-// halves (BF16x3) already in the energy kernel's LDS order, so every K-slab a
+// FP64 / FP32 (rimp2_energy_real): 128 x 128 tiles, rows (4 occupied) x (32
+// virtual), 8 waves each holding T1 and T2 of its elements in registers.
+// BF16x3 (rimp2_energy_bf16x3_wide): 128 x 256 tiles (occupied 4-block x
+// 8-block), 16 waves, T1 and T2 on separate waves, swapped through LDS once
+// per tile; per-pair weights handle the 4 x 8 blocks that straddle i = j.
+//
+// B is repacked once into panels of 4 occupied x 32 virtual columns over K:
+// a contiguous [Kp][128] slab (FP64/FP32), or [Kp/16][hi/lo][128][2][8] of
+// BF16 halves (BF16x3) already in the kernel's LDS order, so every K-slab a
 // workgroup stages into LDS is one contiguous, fully coalesced read, moved by
-// direct global->LDS copies in all three kernels.
+// direct global->LDS copies.
 #include "common.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace corr {
@@ -85,42 +89,46 @@ __global__ void repack_real(const double* __restrict__ Bia, T* __restrict__ Bp, 
     }
 }
 
-// One block per (panel, 32-row K slab): read [32 q][128 c] coalesced along a,
-// write two 16-deep stages, each [hi/lo][128 c][2 slots][8 q], in the energy
-// kernel's LDS order (slot s of row c holds k-chunk bf16_slot(c, s)).
+// One block per (panel, 16-deep stage): read [16 q][128 c] coalesced along a,
+// write the stage as [hi/lo][128 c][2 slots][8 q] in the energy kernels' LDS
+// order (slot s of row c holds k-chunk bf16_slot(c, s)).  Stages of a panel
+// are consecutive: panel blk, stage ks at Bp + (blk Ks + ks) 4096.
 __global__ __launch_bounds__(256) void repack_bf16(const double* __restrict__ Bia,
                                                    uint16_t* __restrict__ Bp, int64_t naux, int o,
-                                                   int v, int nAC, int64_t Kc) {
-    __shared__ double S[32][PW + 1];
-    const int64_t kc = blockIdx.x % Kc;
-    const int64_t blk = blockIdx.x / Kc;
+                                                   int v, int nAC, int64_t Ks) {
+    __shared__ double S[16][PW + 1];
+    const int64_t ks = blockIdx.x % Ks;
+    const int64_t blk = blockIdx.x / Ks;
     const int ib = (int)(blk / nAC), ac = (int)(blk % nAC);
-    for (int e = threadIdx.x; e < 32 * PW; e += 256) {
+    for (int e = threadIdx.x; e < 16 * PW; e += 256) {
         const int q = e / PW, c = e % PW;
-        const int64_t Q = kc * 32 + q;
+        const int64_t Q = ks * 16 + q;
         const int i = ib * PI + c / PA, a = ac * PA + c % PA;
         S[q][c] = (Q < naux && i < o && a < v) ? Bia[(Q * o + i) * (int64_t)v + a] : 0.0;
     }
     __syncthreads();
-    // the two stages of this slab are consecutive: 2 x [2][PW][2][8] bf16
-    uint16_t* out = Bp + (blk * Kc + kc) * (2 * PW * 32);
-    for (int e = threadIdx.x; e < 2 * 2 * PW * 2; e += 256) {
-        const int h = e / (2 * PW * 2), r0 = e % (2 * PW * 2);
-        const int hl = r0 / (PW * 2), c = (r0 % (PW * 2)) / 2, kq = bf16_slot(c, r0 % 2);
-        uint16_t hv[8];
+    uint16_t* out = Bp + (blk * Ks + ks) * (2 * PW * 16);
+    {
+        const int e = threadIdx.x;                  // 256 = [hi/lo][128 rows][... one 16-byte slot]
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const double x = S[h * 16 + kq * 8 + t][c];
-            const uint16_t hi = f32_to_bf16_rn((float)x);
-            if (hl == 0) hv[t] = hi;
-            else hv[t] = f32_to_bf16_rn((float)(x - (double)bf16_to_f32(hi)));
+        for (int half = 0; half < 2; ++half) {
+            const int r0 = e + 256 * half;          // 0..511 = [hi/lo][128][2 slots]
+            const int hl = r0 / (PW * 2), c = (r0 % (PW * 2)) / 2, kq = bf16_slot(c, r0 % 2);
+            uint16_t hv[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const double x = S[kq * 8 + t][c];
+                const uint16_t hi = f32_to_bf16_rn((float)x);
+                if (hl == 0) hv[t] = hi;
+                else hv[t] = f32_to_bf16_rn((float)(x - (double)bf16_to_f32(hi)));
+            }
+            uint4 pk;
+            pk.x = hv[0] | ((uint32_t)hv[1] << 16);
+            pk.y = hv[2] | ((uint32_t)hv[3] << 16);
+            pk.z = hv[4] | ((uint32_t)hv[5] << 16);
+            pk.w = hv[6] | ((uint32_t)hv[7] << 16);
+            *reinterpret_cast<uint4*>(out + (int64_t)r0 * 8) = pk;
         }
-        uint4 pk;
-        pk.x = hv[0] | ((uint32_t)hv[1] << 16);
-        pk.y = hv[2] | ((uint32_t)hv[3] << 16);
-        pk.z = hv[4] | ((uint32_t)hv[5] << 16);
-        pk.w = hv[6] | ((uint32_t)hv[7] << 16);
-        *reinterpret_cast<uint4*>(out + (int64_t)e * 8) = pk;
     }
 }
 
@@ -215,7 +223,7 @@ template <> struct RealMfma<float> {
     }
 };
 
-// ---- BF16x3 kernel -------------------------------------------------------------
+// ---- BF16x3 pieces --------------------------------------------------------------
 // M ~= hi.hi + hi.lo + lo.hi with hi = bf16(x), lo = bf16(x - hi), FP32 accumulate,
 // on v_mfma_f32_32x32x16_bf16: lane l holds A[l%32][8(l/32) + t] (t < 8), and
 // accumulator element r of lane l is C[8(r/4) + 4(l/32) + r%4][l%32].
@@ -226,64 +234,8 @@ __device__ inline corr_f16v mma_bf16_32(const corr_u4& a, const corr_u4& b, corr
                                                    __builtin_bit_cast(corr_bf8, b), c, 0, 0, 0);
 }
 
-// K advances in 16-deep stages; a stage of one panel is [hi/lo][128 rows][16 k]
-// bf16 (8 KiB), stored in the panel exactly as in LDS: a row is two 16-byte
-// k-chunks, chunk ch of row r in slot bf16_slot(r, ch).  Stages move global ->
-// LDS with global_load_lds_dwordx4 (no VGPR round trip, no ds_write): each wave
-// copies one (panel, hi/lo) half, 4 contiguous 1 KiB segments.  Four stage
-// buffers (4 x 32 KiB, separate LDS objects) keep the copies three stages ahead
-// of the stage the matrix cores consume, with one barrier per stage.
 typedef __attribute__((address_space(3))) void* corr_lds_ptr;
 typedef __attribute__((address_space(1))) const void* corr_gbl_ptr;
-
-typedef uint16_t Bf16Stage[4][2][PW][16];   // [panel][hi/lo][row][k slot]
-
-__device__ inline corr_u4 lds_frag(const Bf16Stage& S, int p, int hl, int row, int ko) {
-    return *reinterpret_cast<const corr_u4*>(&S[p][hl][row][ko]);
-}
-
-// Copies this wave's half-panel of stage st into S (4 x 1 KiB, one instruction each).
-__device__ inline void bf16_stage_copy(Bf16Stage& S, const uint16_t* Pw, int pw, int hw, int64_t st, int lane) {
-    constexpr int SUB = 2 * PW * 16;
-    const uint16_t* src = Pw + st * SUB + hw * (PW * 16) + lane * 8;
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-        __builtin_amdgcn_global_load_lds((corr_gbl_ptr)(src + s * 512), (corr_lds_ptr)&S[pw][hw][32 * s][0],
-                                         16, 0, 0);
-}
-
-// One stage of this wave's four 32 x 32 tiles: T1_x = rows ra[x] of panel 0 x
-// rows rb of panel 1, T2_x = rows rb of panel 2 x rows ra[x] of panel 3.
-__device__ inline void bf16_stage_compute(const Bf16Stage& S, const int (&ra)[2], int rb, int ko,
-                                          corr_f16v (&acc1)[2], corr_f16v (&acc2)[2]) {
-    corr_u4 ah[2], al[2], dh[2], dl[2];
-#pragma unroll
-    for (int x = 0; x < 2; ++x) {
-        ah[x] = lds_frag(S, 0, 0, ra[x], ko);
-        al[x] = lds_frag(S, 0, 1, ra[x], ko);
-        dh[x] = lds_frag(S, 3, 0, ra[x], ko);
-        dl[x] = lds_frag(S, 3, 1, ra[x], ko);
-    }
-    const corr_u4 bh = lds_frag(S, 1, 0, rb, ko), bl = lds_frag(S, 1, 1, rb, ko);
-    const corr_u4 ch = lds_frag(S, 2, 0, rb, ko), cl = lds_frag(S, 2, 1, rb, ko);
-    // The three products of one accumulator are dependent: issue them term by
-    // term across the four accumulators, so dependent MFMAs are four apart.
-#pragma unroll
-    for (int x = 0; x < 2; ++x) {
-        acc1[x] = mma_bf16_32(al[x], bh, acc1[x]);
-        acc2[x] = mma_bf16_32(cl, dh[x], acc2[x]);
-    }
-#pragma unroll
-    for (int x = 0; x < 2; ++x) {
-        acc1[x] = mma_bf16_32(ah[x], bl, acc1[x]);
-        acc2[x] = mma_bf16_32(ch, dl[x], acc2[x]);
-    }
-#pragma unroll
-    for (int x = 0; x < 2; ++x) {
-        acc1[x] = mma_bf16_32(ah[x], bh, acc1[x]);
-        acc2[x] = mma_bf16_32(ch, dh[x], acc2[x]);
-    }
-}
 
 // Waits until this wave's outstanding global->LDS copies number at most N.
 #define CORR_WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt((N) | (0x7 << 4) | (0xF << 8))
@@ -299,128 +251,228 @@ __device__ inline void bf16_stage_compute(const Bf16Stage& S, const int (&ra)[2]
         __asm__ volatile("" ::: "memory");                      \
     }
 
-// Wave (wr, wc), wr in 0..1, wc in 0..3: T1_x = M^{ij}_ab for i = 2 wr + x,
-// j = wc (rows a, columns b, all 32 x 32 of the tile's virtual blocks), and
-// T2_x = B[:, (J,A) row j]^T B[:, (I,B) row i] = M^{ji}_ab = M^{ij}_ba, so the
-// exchange partner of every element sits in the same lane and register.
-// Accumulation is FP32 on two levels: the MFMA accumulators are added into
-// FP32 totals and cleared every FLUSH groups of four 16-deep stages (64 K
-// values), which shortens each rounding chain from 456 MFMA steps (152
-// stages at naux = 2432) to 48 plus 10 total updates.  On the (H2O)32 / def2-SVP inputs the error of the energy
-// against FP64 drops from 0.77 to 0.48 uEh (0.47 with FP64 totals, which take
-// twice the registers and ran 2.5% slower; one level: 0.77 uEh).  The totals
-// take 64 VGPRs (206 in all, no spills, 2 waves/SIMD as the 128 KiB of LDS
-// sets).
-constexpr int BF16X3_FLUSH = 4;
+// ---- BF16x3 kernel, wide tile ------------------------------------------------
+// A workgroup owns the occupied 4-block I, the occupied 8-block Jg (two
+// 4-blocks J0 = 2 Jg, J1 = 2 Jg + 1) and the virtual blocks A <= B: T1 =
+// M^{ij}_ab on (I,A) x (Jg,B), 128 x 256, and its exchange partner T2 of the
+// same shape, from six panels (I,A), (J0,B), (J1,B), (J0,A), (J1,A), (I,B):
+// 48 KiB per 16-deep stage for 384 MFMAs, where a 128 x 128 tile pair moves
+// 32 KiB for 192.  Sixteen waves (four per SIMD, 128 registers each): waves 0-7
+// hold T1 and waves 8-15 T2 in 2 x 2 blocks of 32 x 32 MFMA tiles, accumulated
+// on one FP32 level.  Pairs (i, j) carry their own weight: 2 for i < j, 1 for
+// i = j, 0 for i > j (the 4 x 8 occupied tiles straddle the diagonal).
+// Three stage buffers, copies two stages ahead.  The tile of each workgroup
+// comes from a list built on the host (ushort4: I, Jg, A, B).
+constexpr int WIDE_THREADS = 1024;
+typedef uint16_t Bf16Stage6[6][2][PW][16];   // [panel][hi/lo][row][k slot]
 
-template <int FLUSH>
-__global__ __launch_bounds__(WG_THREADS) void rimp2_energy_bf16x3(
-    const uint16_t* __restrict__ Bp, const double* __restrict__ eo, const double* __restrict__ ev,
-    int o, int v, int nIB, int nAC, int64_t Kc, int64_t nWG, int64_t per,
+__device__ inline corr_u4 lds_frag6(const Bf16Stage6& S, int p, int hl, int row, int ko) {
+    return *reinterpret_cast<const corr_u4*>(&S[p][hl][row][ko]);
+}
+
+// NC global -> LDS copies of 1 KiB of a stage: src[k] + st * 8 KiB -> LDS offset seg[k].
+template <int NC>
+__device__ inline void wide_stage_copy(Bf16Stage6& S, const uint16_t* const (&src)[NC], const int (&seg)[NC],
+                                       int64_t st) {
+    constexpr int SUB = 2 * PW * 16;
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+        __builtin_amdgcn_global_load_lds((corr_gbl_ptr)(src[k] + st * SUB),
+                                         (corr_lds_ptr)(&S[0][0][0][0] + seg[k]), 16, 0, 0);
+}
+
+// NA x NB blocks: acc[p][q] += rows rA[p] of panel pa x rows rB[q] of panel pb.
+template <int NA, int NB>
+__device__ inline void wide_stage_compute(const Bf16Stage6& S, int pa, int pb, const int (&rA)[NA],
+                                          const int (&rB)[NB], int ko, corr_f16v (&acc)[NA][NB]) {
+    corr_u4 a[NA][2], b[NB][2];
+#pragma unroll
+    for (int p = 0; p < NA; ++p) a[p][1] = lds_frag6(S, pa, 1, rA[p], ko);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) b[q][0] = lds_frag6(S, pb, 0, rB[q], ko);
+#pragma unroll
+    for (int p = 0; p < NA; ++p) a[p][0] = lds_frag6(S, pa, 0, rA[p], ko);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) b[q][1] = lds_frag6(S, pb, 1, rB[q], ko);
+    // lo.hi, hi.lo, then hi.hi (small terms first)
+#pragma unroll
+    for (int p = 0; p < NA; ++p)
+#pragma unroll
+        for (int q = 0; q < NB; ++q) acc[p][q] = mma_bf16_32(a[p][1], b[q][0], acc[p][q]);
+#pragma unroll
+    for (int p = 0; p < NA; ++p)
+#pragma unroll
+        for (int q = 0; q < NB; ++q) acc[p][q] = mma_bf16_32(a[p][0], b[q][1], acc[p][q]);
+#pragma unroll
+    for (int p = 0; p < NA; ++p)
+#pragma unroll
+        for (int q = 0; q < NB; ++q) acc[p][q] = mma_bf16_32(a[p][0], b[q][0], acc[p][q]);
+}
+
+// The K loop: stage s lives in buffer s % 3.  Iteration s copies stage s+2
+// into buffer (s+2) % 3, which held stage s-1 (read before the barrier closing
+// iteration s-1), computes stage s, then waits until this wave's copies of
+// stage s+1 have landed (the NC of s+2 may still run) and its reads are back,
+// and meets the barrier.  Copies past the end re-copy stage Kc-1.  Kc is a
+// multiple of 3.  Ends with no copy in flight.
+template <int NC, int NA, int NB>
+__device__ inline void wide_k_loop(Bf16Stage6& L0, Bf16Stage6& L1, Bf16Stage6& L2, const uint16_t* const (&src)[NC],
+                                   const int (&seg)[NC], int64_t Kc, int pa, int pb, const int (&rA)[NA],
+                                   const int (&rB)[NB], int ko, corr_f16v (&acc)[NA][NB]) {
+    wide_stage_copy<NC>(L0, src, seg, 0);
+    wide_stage_copy<NC>(L1, src, seg, Kc > 1 ? 1 : Kc - 1);
+    CORR_STAGE_BARRIER(NC)
+#define CORR_WIDE_STAGE(CUR, COPY, s_)                                                        \
+    {                                                                                         \
+        wide_stage_copy<NC>(COPY, src, seg, ((s_) + 2 < Kc) ? (s_) + 2 : Kc - 1);             \
+        wide_stage_compute<NA, NB>(CUR, pa, pb, rA, rB, ko, acc);                             \
+        CORR_STAGE_BARRIER(NC)                                                                \
+    }
+    for (int64_t s = 0; s < Kc; s += 3) {
+        CORR_WIDE_STAGE(L0, L2, s)
+        CORR_WIDE_STAGE(L1, L0, s + 1)
+        CORR_WIDE_STAGE(L2, L1, s + 2)
+    }
+#undef CORR_WIDE_STAGE
+    CORR_WAIT_VMCNT(0);
+}
+
+// 1/D for D > 0: FP32 reciprocal refined by two FP64 Newton steps.
+__device__ inline double rcp_pos(double d) {
+    double x = (double)__builtin_amdgcn_rcpf((float)d);
+    x = fma(x, fma(-d, x, 1.0), x);
+    return fma(x, fma(-d, x, 1.0), x);
+}
+
+// One element's share of the opposite-spin and exchange sums, weight w / D.
+__device__ inline void wide_energy_term(double m1, double m2, double ea, double eij, double w, bool valid,
+                                        double& sos, double& sx) {
+    const double dinv = valid ? w * rcp_pos(ea + eij) : 0.0;
+    sos += (m1 * m1 + m2 * m2) * dinv;
+    sx += m1 * m2 * dinv;
+}
+
+__global__ __launch_bounds__(WIDE_THREADS) void rimp2_energy_bf16x3_wide(
+    const uint16_t* __restrict__ Bp, const ushort4* __restrict__ tiles, const double* __restrict__ eo,
+    const double* __restrict__ ev, int o, int v, int nIB, int nAC, int64_t Kc, int64_t nT, int64_t per,
     double* __restrict__ partial) {
-    constexpr int SUB = 2 * PW * 16;            // bf16 per panel per stage (hi + lo)
-    __shared__ __attribute__((aligned(16))) Bf16Stage L0;
-    __shared__ __attribute__((aligned(16))) Bf16Stage L1;
-    __shared__ __attribute__((aligned(16))) Bf16Stage L2;
-    __shared__ __attribute__((aligned(16))) Bf16Stage L3;
-    __shared__ double sh[WG_THREADS / 64];
+    constexpr int SUB = 2 * PW * 16;
+    __shared__ __attribute__((aligned(16))) Bf16Stage6 L0;
+    __shared__ __attribute__((aligned(16))) Bf16Stage6 L1;
+    __shared__ __attribute__((aligned(16))) Bf16Stage6 L2;
+    __shared__ double sh[WIDE_THREADS / 64];
+    __shared__ double evA[PA];   // eps_vir of the virtual block A (read after the exchange barrier)
 
-    PairTile t;
-    int64_t id;
-    if (!decode_tile(nIB, nAC, nWG, per, t, id)) return;
+    const int64_t id = xcd_logical(per);
+    if (id >= nT) return;
+    const ushort4 tl = tiles[id];
+    const int IB = tl.x, JG = tl.y, AC = tl.z, BC = tl.w;
+    // loaded now, stored into evA after the K loop (no wait in the prologue)
+    const int t32 = threadIdx.x & 31;
+    const double ev_own = ev[AC * PA + t32 < v ? AC * PA + t32 : v - 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave >> 2, wc = wave & 3, r32 = lane & 31, kh = lane >> 5;
-
-    // Panels: 0 = (I,A), 1 = (J,B), 2 = (J,A), 3 = (I,B).  Wave w copies panel w/2, half w%2.
-    const int pw = wave >> 1, hw = wave & 1;
-    const int ibw = (pw == 0 || pw == 3) ? t.IB : t.JB;
-    const int acw = (pw == 0 || pw == 2) ? t.AC : t.BC;
-    const uint16_t* Pw = Bp + ((int64_t)ibw * nAC + acw) * Kc * SUB;
-
-    const int ra[2] = {(2 * wr) * PA + r32, (2 * wr + 1) * PA + r32};
-    const int rb = wc * PA + r32;
-    // Rows read by this lane all share r % 32 = r32, so they share the chunk slot.
+    const int r32 = lane & 31, kh = lane >> 5;
     const int ko = bf16_slot(r32, kh) * 8;
 
-    corr_f16v acc1[2], acc2[2];
+    // panel pointers: 0 (I,A), 1 (J0,B), 2 (J1,B), 3 (J0,A), 4 (J1,A), 5 (I,B); a J1
+    // past the last occupied block reads J0's panel (its pairs have j >= o).
+    const int J0 = 2 * JG, J1 = (2 * JG + 1 < nIB) ? 2 * JG + 1 : 2 * JG;
+    auto panel = [&](int p) -> const uint16_t* {
+        const int ob = (p == 0 || p == 5) ? IB : (p == 1 || p == 3) ? J0 : J1;
+        const int vb = (p == 0 || p == 3 || p == 4) ? AC : BC;
+        return Bp + ((int64_t)ob * nAC + vb) * Kc * SUB;
+    };
+    const double wab = AC < BC ? 1.0 : 0.5;
+    double sos = 0.0, sx = 0.0;
+
+    {
+        const int half = wave >> 3, t8 = wave & 7, ip = t8 >> 2, jp = t8 & 3;
+        const uint16_t* src[3];
+        int seg[3];
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+        for (int k = 0; k < 3; ++k) {
+            const int c = 3 * wave + k;                  // copy c of 48: panel c / 8, KiB c % 8
+            src[k] = panel(c >> 3) + (c & 7) * 512 + lane * 8;
+            seg[k] = (c >> 3) * SUB + (c & 7) * 512;
+        }
+        // T1 wave: acc[p][q] = (I,A) rows i = 2ip+p  x  (Jg,B) rows j = 2jp+q.
+        // T2 wave: acc[p][q] = (Jg,A) rows j = 2jp+p  x  (I,B) rows i = 2ip+q, i.e.
+        //          M^{ij}_ba for T1's block (x, y) = (q, p).
+        const int pa = half ? 3 + (jp >> 1) : 0;
+        const int pb = half ? 5 : 1 + (jp >> 1);
+        const int ra0 = half ? (2 * jp) & 3 : 2 * ip;
+        const int rb0 = half ? 2 * ip : (2 * jp) & 3;
+        const int rA[2] = {ra0 * PA + r32, (ra0 + 1) * PA + r32};
+        const int rB[2] = {rb0 * PA + r32, (rb0 + 1) * PA + r32};
+        corr_f16v acc[2][2];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) { acc1[x][r] = 0.0f; acc2[x][r] = 0.0f; }
-    // FP32 totals of the accumulators, flushed every FLUSH 4-stage groups
-    float d1[FLUSH > 0 ? 2 : 1][FLUSH > 0 ? 16 : 1], d2[FLUSH > 0 ? 2 : 1][FLUSH > 0 ? 16 : 1];
-    if constexpr (FLUSH > 0) {
+        for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+            for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) { d1[x][r] = 0.0f; d2[x][r] = 0.0f; }
-    }
-    auto flush = [&]() {
-        if constexpr (FLUSH > 0) {
+                for (int r = 0; r < 16; ++r) acc[p][q][r] = 0.0f;
+        wide_k_loop<3, 2, 2>(L0, L1, L2, src, seg, Kc, pa, pb, rA, rB, ko, acc);
+        if (threadIdx.x < PA) evA[threadIdx.x] = ev_own;
+
+        // The energy of block (x, y) of wave pair t8 (i = 2ip+x, j = 2jp+y) is
+        // taken by the T1 wave for x = 0 and by the T2 wave for x = 1, so the two
+        // swap one block row through LDS: the T1 wave stores its x = 1 blocks,
+        // the T2 wave its x = 0 blocks (M_ba of T1's x = 0), 16 KiB per wave
+        // pair, three pairs per 48 KiB buffer.
+        __syncthreads();
+        float* X = reinterpret_cast<float*>(t8 < 3 ? &L0[0][0][0][0] : t8 < 6 ? &L1[0][0][0][0] : &L2[0][0][0][0]) +
+                   (t8 % 3) * 4096;
 #pragma unroll
-            for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                if (half) X[2048 + (y * 16 + r) * 64 + lane] = acc[y][0][r];
+                else X[(y * 16 + r) * 64 + lane] = acc[1][y][r];
+            }
+        __syncthreads();
+        const int b = BC * PA + r32;
+        const int i = IB * PI + 2 * ip + half;
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int j = JG * 2 * PI + 2 * jp + y;
+            const double w = (i < j ? 2.0 : (i == j ? 1.0 : 0.0)) * wab;
+            if (w > 0.0 && i < o && j < o && b < v) {
+                const double eij = ev[b] - eo[i] - eo[j];
+                float mo[16];   // T1 wave: M_ba from the T2 wave; T2 wave: M_ab from the T1 wave
+#pragma unroll
+                for (int r = 0; r < 16; ++r) mo[r] = X[(half ? 0 : 2048) + (y * 16 + r) * 64 + lane];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    d1[x][r] += acc1[x][r];
-                    d2[x][r] += acc2[x][r];
-                    acc1[x][r] = 0.0f;
-                    acc2[x][r] = 0.0f;
+                    const double mr = half ? (double)acc[y][1][r] : (double)acc[0][y][r];
+                    const int al = 8 * (r / 4) + 4 * kh + (r % 4);
+                    wide_energy_term(mr, (double)mo[r], evA[al], eij, w, AC * PA + al < v, sos, sx);
                 }
-        }
-    };
-
-    // Stage s lives in buffer s % 4.  Before the barrier that opens stage s,
-    // every wave has finished reading stage s-1 and has landed its copy of
-    // stage s; after it, the buffer of stage s-1 takes the copy of stage s+3.
-    // Kc is a multiple of 4 (naux is padded to 64).
-    // Copies in flight at each barrier: this wave's 4 instructions per stage;
-    // stage s+1 must have landed while s+2 and s+3 may still run (vmcnt 8).
-    // The copy of each step is unconditional (no branch for the compiler's
-    // wait insertion to merge): the last three steps re-copy stage Kc-1 into
-    // the buffer just released, which is never read again.
-    bf16_stage_copy(L0, Pw, pw, hw, 0, lane);
-    bf16_stage_copy(L1, Pw, pw, hw, Kc > 1 ? 1 : Kc - 1, lane);
-    bf16_stage_copy(L2, Pw, pw, hw, Kc > 2 ? 2 : Kc - 1, lane);
-    CORR_STAGE_BARRIER(8)
-#define CORR_STAGE(CUR, NEXT3, s_)                                                       \
-    {                                                                                    \
-        bf16_stage_copy(NEXT3, Pw, pw, hw, ((s_) + 3 < Kc) ? (s_) + 3 : Kc - 1, lane);   \
-        bf16_stage_compute(CUR, ra, rb, ko, acc1, acc2);                                 \
-        CORR_STAGE_BARRIER(8)                                                            \
-    }
-    for (int64_t s = 0; s < Kc; s += 4) {
-        CORR_STAGE(L0, L3, s)
-        CORR_STAGE(L1, L0, s + 1)
-        CORR_STAGE(L2, L1, s + 2)
-        CORR_STAGE(L3, L2, s + 3)
-        if constexpr (FLUSH > 0) {
-            if ((s / 4 + 1) % FLUSH == 0) flush();
-        }
-    }
-#undef CORR_STAGE
-    CORR_WAIT_VMCNT(0);   // no copy outlives the loop
-    flush();
-
-    double sos = 0.0, sx = 0.0;
-    const int j = t.JB * PI + wc;
-    const int b = t.BC * PA + r32;
-#pragma unroll
-    for (int x = 0; x < 2; ++x) {
-        const int i = t.IB * PI + 2 * wr + x;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int a = t.AC * PA + 8 * (r / 4) + 4 * kh + (r % 4);
-            if (i < o && j < o && a < v && b < v) {
-                const double m1 = FLUSH > 0 ? (double)d1[x][r] : (double)acc1[x][r];
-                const double m2 = FLUSH > 0 ? (double)d2[x][r] : (double)acc2[x][r];
-                const double dinv = 1.0 / (ev[a] + ev[b] - eo[i] - eo[j]);
-                sos += (m1 * m1 + m2 * m2) * dinv;
-                sx += m1 * m2 * dinv;
             }
         }
     }
-    write_partial(sos, sx, t, id, sh, partial);
+    // fixed-order workgroup sums
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        sos += __shfl_xor(sos, off, 64);
+        sx += __shfl_xor(sx, off, 64);
+    }
+    __syncthreads();
+    if (lane == 0) sh[wave] = sos;
+    __syncthreads();
+    double tos = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < WIDE_THREADS / 64; ++w) tos += sh[w];
+    __syncthreads();
+    if (lane == 0) sh[wave] = sx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tx = 0.0;
+        for (int w = 0; w < WIDE_THREADS / 64; ++w) tx += sh[w];
+        partial[id * 2 + 0] = -tos;
+        partial[id * 2 + 1] = 2.0 * tx;
+    }
 }
+
 // ---- FP64 / FP32 kernel: stages -----------------------------------------------
 // A stage holds KS k-rows of the four 128-wide panels, [panel][k][128], moved
 // global -> LDS by global_load_lds_dwordx4 (32 x 1 KiB per stage, four per
@@ -594,14 +646,95 @@ struct RiLayout {
     int64_t Kp, nWG, per;
 };
 
-static RiLayout ri_layout(int64_t naux, int64_t o, int64_t v) {
+// kstage: K padding of the panels (64 for the 4-buffer kernels, 48 for the
+// 3-buffer wide kernel: whole groups of 16-deep stages).
+static RiLayout ri_layout(int64_t naux, int64_t o, int64_t v, int64_t kstage = 64) {
     RiLayout l;
     l.nIB = (int)cdiv(o, PI);
     l.nAC = (int)cdiv(v, PA);
-    l.Kp = rup(naux, 64);
+    l.Kp = rup(naux, kstage);
     l.nWG = (int64_t)l.nIB * (l.nIB + 1) / 2 * ((int64_t)l.nAC * (l.nAC + 1) / 2);
     l.per = cdiv(l.nWG, 8);
     return l;
+}
+
+// Tile list of the wide kernel: every (I, Jg, A, B) with A <= B and some pair
+// i <= j in I x Jg, in super-tiles of WT_GI I-blocks x one Jg x (WT_GA x WT_GA
+// virtual blocks, A-group <= B-group), super-tiles row-major (I-group, Jg,
+// group pair), tiles inside row-major (I, A, B).  Consecutive tiles, which run
+// together on one XCD, then share most of their panels in its L2 (measured:
+// 62.5 ms against 66.8 ms for plain (I, Jg, A, B) order at (H2O)32).
+constexpr int WT_GI = 4, WT_GA = 4;
+
+__host__ __device__ inline int wt_npairs(int gA, int gB, int nAC) {
+    const int nA = std::min(nAC, (gA + 1) * WT_GA) - gA * WT_GA;
+    const int nB = std::min(nAC, (gB + 1) * WT_GA) - gB * WT_GA;
+    return gA < gB ? nA * nB : nA * (nA + 1) / 2;
+}
+
+__host__ __device__ inline int wt_count(int64_t st, int nIB, int nAC, int nJG, int nSP, int& iG, int& G, int& gA, int& gB) {
+    const int sp = (int)(st % nSP);
+    G = (int)((st / nSP) % nJG);
+    iG = (int)(st / ((int64_t)nSP * nJG));
+    // (gA, gB), gA <= gB, row-major over nAG groups
+    const int nAG = (nAC + WT_GA - 1) / WT_GA;
+    int r = 0, rem = sp;
+    while (rem >= nAG - r) { rem -= nAG - r; ++r; }
+    gA = r;
+    gB = r + rem;
+    int nI = 0;
+    for (int I = iG * WT_GI; I < std::min(nIB, (iG + 1) * WT_GI); ++I) nI += (8 * G + 7 >= 4 * I) ? 1 : 0;
+    return nI * wt_npairs(gA, gB, nAC);
+}
+
+// One workgroup of 1024 threads: thread t takes a contiguous chunk of the NS
+// super-tiles, counts its tiles, a workgroup scan gives its offset, and it
+// writes its tiles in order.
+__global__ __launch_bounds__(1024) void wide_tile_list(ushort4* __restrict__ tiles, int nIB, int nAC, int nJG,
+                                                      int nSP, int64_t NS) {
+    __shared__ int64_t sc[1024];
+    const int t = threadIdx.x;
+    const int64_t C = (NS + 1023) / 1024;
+    const int64_t s0 = std::min<int64_t>(NS, t * C), s1 = std::min<int64_t>(NS, s0 + C);
+    int64_t n = 0;
+    int iG, G, gA, gB;
+    for (int64_t st = s0; st < s1; ++st) n += wt_count(st, nIB, nAC, nJG, nSP, iG, G, gA, gB);
+    sc[t] = n;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t x = t >= off ? sc[t - off] : 0;
+        __syncthreads();
+        sc[t] += x;
+        __syncthreads();
+    }
+    int64_t k = sc[t] - n;   // exclusive offset
+    for (int64_t st = s0; st < s1; ++st) {
+        wt_count(st, nIB, nAC, nJG, nSP, iG, G, gA, gB);
+        for (int I = iG * WT_GI; I < std::min(nIB, (iG + 1) * WT_GI); ++I) {
+            if (8 * G + 7 < 4 * I) continue;   // every pair has i > j
+            for (int A = gA * WT_GA; A < std::min(nAC, (gA + 1) * WT_GA); ++A)
+                for (int B = std::max(A, gB * WT_GA); B < std::min(nAC, (gB + 1) * WT_GA); ++B)
+                    tiles[k++] = make_ushort4((unsigned short)I, (unsigned short)G, (unsigned short)A,
+                                              (unsigned short)B);
+        }
+    }
+}
+
+struct WideTiles {
+    int nJG, nSP;
+    int64_t NS, nT;
+};
+
+static WideTiles wide_tiles(int nIB, int nAC) {
+    WideTiles w;
+    w.nJG = (nIB + 1) / 2;
+    const int nIG = (nIB + WT_GI - 1) / WT_GI, nAG = (nAC + WT_GA - 1) / WT_GA;
+    w.nSP = nAG * (nAG + 1) / 2;
+    w.NS = (int64_t)nIG * w.nJG * w.nSP;
+    w.nT = 0;
+    int iG, G, gA, gB;
+    for (int64_t st = 0; st < w.NS; ++st) w.nT += wt_count(st, nIB, nAC, w.nJG, w.nSP, iG, G, gA, gB);
+    return w;
 }
 
 // Repacks Bia into the panel layout for the given precision; returns the
@@ -613,10 +746,10 @@ int ri_repack(corr_ctx* ctx, const RiLayout& l, int64_t naux, int o, int v, cons
     void* Bp = nullptr;
     CORR_TRY(scratch(ctx, "rimp2_panels", elems * esz, &Bp));
     if (prec == CORR_PREC_BF16X3) {
-        const int64_t Kc = l.Kp / 32;
-        const int64_t blocks = (int64_t)l.nIB * l.nAC * Kc;
+        const int64_t Ks = l.Kp / 16;
+        const int64_t blocks = (int64_t)l.nIB * l.nAC * Ks;
         hipLaunchKernelGGL(repack_bf16, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, Bia,
-                           (uint16_t*)Bp, naux, o, v, l.nAC, Kc);
+                           (uint16_t*)Bp, naux, o, v, l.nAC, Ks);
     } else {
         const int64_t grid = std::min<int64_t>(cdiv(elems, 256), 65536 * 4);
         if (prec == CORR_PREC_FP64)
@@ -631,6 +764,38 @@ int ri_repack(corr_ctx* ctx, const RiLayout& l, int64_t naux, int o, int v, cons
     return CORR_OK;
 }
 
+// BF16x3: the wide-tile kernel over a tile list built on the device.
+static int rimp2_energy_bf16x3(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
+                               const double* eo, const double* ev, corr_mp2_energy* out) {
+    const RiLayout l = ri_layout(naux, nocc, nvir, 48);
+    CORR_REQUIRE(ctx, l.nIB < 65535 && l.nAC < 65535, "rimp2: nocc/nvir too large for the tile list");
+    const WideTiles wt = wide_tiles(l.nIB, l.nAC);
+    const int64_t per = cdiv(wt.nT, 8);
+    CORR_REQUIRE(ctx, per * 8 < ((int64_t)1 << 31), "rimp2: too many tiles");
+    const void* Bp = nullptr;
+    CORR_TRY(ri_repack(ctx, l, naux, (int)nocc, (int)nvir, Bia, CORR_PREC_BF16X3, &Bp));
+    ushort4* d_tiles = nullptr;
+    double* partial = nullptr;
+    CORR_TRY(scratch(ctx, "rimp2_tiles", sizeof(ushort4) * wt.nT, (void**)&d_tiles));
+    CORR_TRY(scratch(ctx, "rimp2_partial", sizeof(double) * 2 * wt.nT, (void**)&partial));
+    hipLaunchKernelGGL(wide_tile_list, dim3(1), dim3(1024), 0, ctx->stream, d_tiles, l.nIB, l.nAC, wt.nJG, wt.nSP,
+                       wt.NS);
+    CORR_LAUNCHED(ctx, "wide_tile_list");
+    KernelTimer timer(ctx, "rimp2_energy");
+    hipLaunchKernelGGL(rimp2_energy_bf16x3_wide, dim3((unsigned)(per * 8)), dim3(WIDE_THREADS), 0, ctx->stream,
+                       (const uint16_t*)Bp, d_tiles, eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp / 16, wt.nT,
+                       per, partial);
+    CORR_LAUNCHED(ctx, "rimp2_energy_bf16x3_wide");
+    timer.stop();
+    double s[2];
+    CORR_TRY(reduce_sum(ctx, partial, wt.nT, 2, s));
+    out->e_os = s[0];
+    out->e_ss = s[0] + s[1];
+    out->e_mp2 = out->e_os + out->e_ss;
+    out->e_scs = out->e_ss / 3.0 + out->e_os * 6.0 / 5.0;
+    return CORR_OK;
+}
+
 int rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
                  const double* eo, const double* ev, int prec, corr_mp2_energy* out) {
     CORR_REQUIRE(ctx, naux > 0 && nocc > 0 && nvir > 0, "rimp2: sizes must be positive");
@@ -638,6 +803,7 @@ int rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const 
     CORR_REQUIRE(ctx, Bia && eo && ev && out, "rimp2: null pointer");
     CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
                  "rimp2: unknown precision");
+    if (prec == CORR_PREC_BF16X3) return rimp2_energy_bf16x3(ctx, naux, nocc, nvir, Bia, eo, ev, out);
     const RiLayout l = ri_layout(naux, nocc, nvir);
     CORR_REQUIRE(ctx, l.per * 8 < ((int64_t)1 << 31), "rimp2: too many tiles");
     const void* Bp = nullptr;
@@ -649,12 +815,9 @@ int rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const 
     if (prec == CORR_PREC_FP64)
         hipLaunchKernelGGL(rimp2_energy_real<double>, grid, block, 0, ctx->stream, (const double*)Bp,
                            eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp, l.nWG, l.per, partial);
-    else if (prec == CORR_PREC_FP32)
+    else
         hipLaunchKernelGGL(rimp2_energy_real<float>, grid, block, 0, ctx->stream, (const float*)Bp,
                            eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp, l.nWG, l.per, partial);
-    else
-        hipLaunchKernelGGL(rimp2_energy_bf16x3<BF16X3_FLUSH>, grid, block, 0, ctx->stream, (const uint16_t*)Bp,
-                           eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp / 16, l.nWG, l.per, partial);
     CORR_LAUNCHED(ctx, "rimp2_energy");
     timer.stop();
     double s[2];

@@ -197,6 +197,16 @@ __global__ __launch_bounds__(256) void canonical_mp2_kernel(const double* __rest
 
 int gemm_prec(int prec) { return prec == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_PREC_FP32; }
 
+// Precision of the metric product Y = M X.  The dense J^-1/2 of the eigen
+// metric has entries of order lambda_min^-1/2 whose products cancel in Y, so
+// FP32 rounding of that product is amplified (4.1 uEh in EMP2 on the
+// (H2O)32 / def2-SVP-RI inputs, against 0.02 uEh with the triangular
+// Cholesky factor): with the eigen metric it always runs in FP64, as the
+// factorisation does.
+static int metric_gemm_prec(int mode, int prec) {
+    return mode == CORR_METRIC_EIGEN ? CORR_PREC_FP64 : gemm_prec(prec);
+}
+
 }  // namespace
 
 int build_fmo(corr_ctx* ctx, int64_t N, const double* C, const double* F, double* FMo) {
@@ -355,8 +365,8 @@ int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double*
     CORR_TRY(scratch(ctx, "ri_Yvo", sizeof(double) * naux * o * v, (void**)&Y));
     CORR_TRY(ao2vo(ctx, naux, N, o, C, Pmn, X, p));
     // Y[Q][(a i)] = sum_P M[Q][P] X[P][(a i)]   (M lower triangular for CHOLESKY)
-    CORR_TRY(gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Y, o * v, 0, 1, p,
-                  mode == CORR_METRIC_CHOLESKY));
+    CORR_TRY(gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Y, o * v, 0, 1,
+                  metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY));
     return transpose_vo(ctx, naux, v, o, Y, Bia);
 }
 
@@ -393,8 +403,8 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
         return rc;
     }
     CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_join, 0));
-    CORR_TRY(gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Y, o * v, 0, 1, p,
-                  mode == CORR_METRIC_CHOLESKY));
+    CORR_TRY(gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Y, o * v, 0, 1,
+                  metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY));
     return transpose_vo(ctx, naux, v, o, Y, Bia);
 }
 

@@ -30,6 +30,66 @@ def _gpu(x, device):
     return torch.from_numpy(np.array(x, dtype=np.float64, copy=True)).to(device)
 
 
+def sorted_unique_list(dense: torch.Tensor):
+    """The reference's sorted unique integral list (TEints: intval, intnums,
+    sortcount; util.h:61-65) of a dense (mu nu|la si) tensor on the GPU.
+
+    Every canonical quadruple (a >= b, c >= d, ab >= cd) is put in the
+    reference's stored form by resort step 1 (ops_rhf.cc:766-818), classified
+    into its five permutation classes (ops_rhf.cc:820-896: all four equal;
+    (aa|cc); (ab|ab); (aa|cd); the rest) and stably sorted by class.  Values
+    are gathered from `dense`; nothing is screened.  Returns (intval float64
+    [n], intnums int16 [n, 4], sortcount [4] as Python ints)."""
+    N = dense.shape[0]
+    device = dense.device
+    P = N * (N + 1) // 2
+    T = P * (P + 1) // 2
+    k = torch.arange(T, dtype=torch.int64, device=device)
+
+    def tri(x):
+        r = torch.floor((torch.sqrt(8.0 * x.double() + 1.0) - 1.0) / 2.0).long()
+        r = torch.where(r * (r + 1) // 2 > x, r - 1, r)
+        r = torch.where((r + 1) * (r + 2) // 2 <= x, r + 1, r)
+        return r, x - r * (r + 1) // 2
+
+    p, q = tri(k)
+    del k
+    a, b = tri(p)
+    c, d = tri(q)
+    del p, q
+    oa, ob, oc, od = a.clone(), b.clone(), c.clone(), d.clone()
+
+    def assign(m, w, x, y, z):
+        a[m], b[m], c[m], d[m] = w[m], x[m], y[m], z[m]
+
+    assign((oa == ob) & (ob == od) & (oc != od), oa, oa, oa, oc)
+    assign((oa != ob) & (oa == oc) & (oc == od), oa, oa, oa, ob)
+    assign((oa != ob) & (ob == oc) & (oc == od), ob, ob, ob, oa)
+    assign((oa == od) & (ob == oc) & (oa != ob), oa, ob, od, oc)
+    assign((ob == oc) & (ob != oa) & (ob != od) & (oa != od), ob, oa, oc, od)
+    assign((oc == od) & (oc != oa) & (oc != ob) & (oa != ob), oc, oc, oa, ob)
+    assign((oa == od) & (oa != ob) & (oa != oc) & (ob != oc), oa, ob, od, oc)
+    assign((ob == od) & (ob != oa) & (ob != oc) & (oa != oc), ob, oa, od, oc)
+    del oa, ob, oc, od
+    cls = torch.full_like(a, 4)
+    m0 = (a == b) & (b == c) & (c == d)
+    m1 = ~m0 & (a == b) & (c == d)
+    m2 = ~m0 & ~m1 & (a == c) & (b == d)
+    m3 = ~m0 & ~m1 & ~m2 & (a == b)
+    cls[m3] = 3
+    cls[m2] = 2
+    cls[m1] = 1
+    cls[m0] = 0
+    order = torch.sort(cls, stable=True).indices
+    sortcount = [int((cls <= j).sum()) for j in range(4)]
+    nums = torch.stack([a, b, c, d], dim=1)[order].to(torch.int16)
+    del a, b, c, d, cls, order
+    q = nums.to(torch.int64)
+    val = dense[q[:, 0], q[:, 1], q[:, 2], q[:, 3]].contiguous()
+    del q
+    return val, nums.contiguous(), sortcount
+
+
 def run_from_files(prefix: str, prec: str = "fp64", ri_prec: str | None = None, scf_maxiter: int = 100,
                    ccsd_maxiter: int = 20, ccsd: bool = True, device: str | int | None = None,
                    log=print) -> dict:

@@ -155,7 +155,9 @@ int corr_ri_metric(corr_ctx* ctx, int64_t naux, const double* d_J,
  * corr_ri_form_bia: Bia = M * ((P|mu nu) -> (P|ia)), transform first, then
  *                   the metric: 1.97 instead of 7.8 TFLOP at N=768, naux=2432.
  *                   metric_mode tells how M was made (CHOLESKY: M is lower
- *                   triangular and its zero half is skipped).
+ *                   triangular and its zero half is skipped).  With EIGEN the
+ *                   metric product runs in FP64 whatever prec asks: the dense
+ *                   J^-1/2 amplifies FP32 rounding (4 uEh in EMP2 at (H2O)32).
  * GEMM steps requested in CORR_PREC_BF16X3 run in FP32 MFMA. */
 int corr_ri_bpq(corr_ctx* ctx, int64_t naux, int64_t nroao, const double* d_M,
                 const double* d_Pmn, double* d_BPQ, int prec);

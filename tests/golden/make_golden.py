@@ -65,7 +65,36 @@ def ccsd_case():
              energies=np.array(r["energies"]), emp2=r["emp2"], e_hf_check=r["e_hf_check"], T1=r["T1"], T2=r["T2"])
 
 
+def water_case():
+    """RI-MP2 of one water (O-H 0.958 A, H-O-H 104.5 deg) in def2-SVP /
+    def2-SVP-RI: the oracle's integrals, RHF to |dE| <= 1e-12 (dense Fock),
+    J^-1/2 with the reference's eigenvalue cutoff (ri.cc:50-69), B by
+    transform_ri (ri.cc:147-172), energies by run_canonical_mp2_ri
+    (can_mp2.cc:53-132).  A real-chemistry B: the reduced-precision paths
+    are checked on it in micro-hartree."""
+    from oracle import integrals as I
+    from oracle import scf as SCF
+    r, th = 0.958 * I.ANGSTROM_TO_BOHR, np.deg2rad(104.5)
+    X = np.array([[0, 0, 0], [r, 0, 0], [r * np.cos(th), r * np.sin(th), 0]])
+    atoms, Z = list(zip(["O", "H", "H"], X)), [8.0, 1.0, 1.0]
+    bas = os.path.join(ROOT, "corr_amd", "basis")
+    sh = I.build_shells(atoms, I.parse_g94(os.path.join(bas, "def2-svp.g94")))
+    ax = I.build_shells(atoms, I.parse_g94(os.path.join(bas, "def2-svp-ri.g94")))
+    S, T, V, G = I.overlap(sh), I.kinetic(sh), I.nuclear(sh, Z, list(X)), I.coulomb4(sh)
+    H = T + V
+    res = SCF.run_scf(H, S, lambda P: SCF.fock_dense(G, P, H), SCF.diag_fmat(H, SCF.calc_s12(S))[0], 10,
+                      ion_rep=I.nuclear_repulsion(Z, list(X)), e_tol=1e-12, maxiter=200)
+    M, kept = O.ri_metric_eigen(I.coulomb2(ax))
+    B = O.transform_ri(res["C"], O.ri_bpq(M, I.coulomb3(ax, sh)), 5)
+    eps = np.diag(O.build_fmo(res["C"], res["F"]))
+    e = O.rimp2(B, eps[:5], eps[5:])
+    np.savez(os.path.join(HERE, "rimp2_water_def2svp.npz"), Bia=B, eps_occ=eps[:5], eps_vir=eps[5:],
+             e_rhf=res["total"], naux_kept=kept, e_os=e["EMP2_OS"], e_ss=e["EMP2_SS"], e_mp2=e["EMP2"],
+             e_scs=e["EMP2_SCS"])
+
+
 if __name__ == "__main__":
+    water_case()
     rimp2_case()
     fourindex_case()
     ccsd_case()

@@ -18,17 +18,20 @@ def test_more_than_one_gpu_exits_with_a_message():
 
 def test_sizes_and_work_of_the_h2o32_step():
     """N, o, v, naux of (H2O)32 in def2-SVP / def2-SVP-RI and the algorithmic
-    work: W_E = 2 [o(o+1)/2] v^2 naux, W_B = 2 naux o N^2 + 2 naux o v N + 2 naux^2 o v."""
+    work: W_E = 2 [o(o+1)/2] v^2 naux, W_B = 2 naux o N^2 + 2 naux o v N + 2 naux^2 o v
+    with a dense metric, and naux^2 o v for the metric product with the
+    triangular Cholesky factor that the step applies."""
     N, o, v, naux = bench.sizes(32)
     assert (N, o, v, naux) == (768, 160, 608, 2432)
-    W_E, W_B = bench.work(N, o, v, naux)
+    W_E, W_B, W_Bc = bench.work(N, o, v, naux)
     assert W_E == 2.0 * (160 * 161 // 2) * 608 * 608 * 2432
     assert abs(W_E / 1e12 - 23.159) < 1e-3
     assert abs(W_B / 1e12 - 1.973) < 1e-3
+    assert W_B - W_Bc == 1.0 * naux * naux * o * v and abs(W_Bc / 1e12 - 1.398) < 1e-3
 
 
 def test_recorded_traffic_is_read_from_the_profile():
-    t = bench.recorded_traffic("rimp2_energy_bf16x3")
+    t = bench.recorded_traffic("rimp2_energy_bf16x3_wide")
     assert isinstance(t, int) and 0.95e9 < t < 400e9   # at least the algorithmic bytes, at most a few re-reads
 
 

@@ -112,3 +112,26 @@ def test_metric_and_bpq_roundtrip(tmp_path):
     assert np.array_equal(FM.read_bpq(str(tmp_path / "a.rimp2"), 6, 3), B)
     with pytest.raises(ValueError):
         FM.read_bpq(str(tmp_path / "a.rimp2"), 7, 3)
+
+
+def test_sorted_unique_list_torch_matches_oracle():
+    """corr_amd.pipeline.sorted_unique_list (torch, any device) against the
+    oracle's restatement of calculate_libint_tei's selection + resort_integrals
+    (ops_rhf.cc:682-693, 760-896): the same class boundaries and, within each
+    class, the same (a, b, c, d, value) entries; the unfolded tensor equals
+    the input."""
+    import torch
+    from corr_amd.pipeline import sorted_unique_list
+    for N, seed in ((5, 2), (7, 9)):
+        dense = S.eri_like(N, seed=seed)
+        val, idx, sc = O.sorted_unique_list(dense, thresh=-1.0)
+        tv, ti, tsc = sorted_unique_list(torch.from_numpy(dense))
+        assert list(tsc) == [int(x) for x in sc]
+        tv, ti = tv.numpy(), ti.numpy().astype(np.int64)
+        bounds = [0, *sc, len(val)]
+        for k in range(5):
+            lo, hi = bounds[k], bounds[k + 1]
+            mine = sorted(zip(map(tuple, ti[lo:hi]), tv[lo:hi]))
+            ref = sorted(zip(map(tuple, idx[lo:hi].astype(np.int64)), val[lo:hi]))
+            assert mine == ref
+        assert np.array_equal(O.unfold_sorted(N, tv, ti.astype(np.uint16), np.array(tsc)), dense)

@@ -114,3 +114,24 @@ def test_gpu_ccsd_fixture(corr):
     assert np.allclose(r["energies"], g["energies"], rtol=0, atol=1e-10)
     assert abs(r["emp2"] - float(g["emp2"])) < 1e-10
     assert abs(r["e_hf_check"] - float(g["e_hf_check"])) < 1e-10
+
+
+def test_oracle_water_fixture():
+    """The real-water fixture: the oracle's RI-MP2 on the stored B gives the
+    stored energies; the pair-loop and dense forms agree; RHF and MP2 lie in
+    the known def2-SVP ranges; no auxiliary function was cut by the metric;
+    and n separated copies (shuffled indices) give exactly n times the
+    energies (size extensivity, a closed form for block-diagonal B)."""
+    g = load("rimp2_water_def2svp.npz")
+    B, eo, ev = g["Bia"], g["eps_occ"], g["eps_vir"]
+    e = O.rimp2(B, eo, ev)
+    d = O.rimp2_dense(B, eo, ev)
+    for k, f in (("EMP2_OS", "e_os"), ("EMP2_SS", "e_ss"), ("EMP2", "e_mp2"), ("EMP2_SCS", "e_scs")):
+        assert abs(e[k] - float(g[f])) < 1e-14 and abs(d[k] - e[k]) < 1e-13
+    assert -76.0 < float(g["e_rhf"]) < -75.9 and -0.21 < float(g["e_mp2"]) < -0.20
+    assert int(g["naux_kept"]) == B.shape[0]
+    from tests.test_gpu_posthf import _separated_waters
+    Bn, eon, evn, ref = _separated_waters(3, seed=5)
+    en = O.rimp2(Bn, eon, evn)
+    for k in ref:
+        assert abs(en[k] - ref[k]) < 1e-13

@@ -2,7 +2,8 @@
 
 Tolerances: FP64 paths agree with the FP64 oracle to 1e-12 relative (energies
 to 1e-10 Eh absolute); FP32 MFMA paths to 1e-5 relative on matrix entries;
-BF16x3 energies to 1e-6 relative."""
+BF16x3 and FP32 energies to 2e-6 relative on random tensors and to 0.1 uEh
+per water on real-chemistry B."""
 import sys
 
 import numpy as np
@@ -146,6 +147,37 @@ def test_cholesky_metric_not_positive_definite(corr, entry):
     assert abs(got["EMP2"] - O.rimp2(B, eo, ev)["EMP2"]) < 1e-12
 
 
+def test_calls_on_two_streams_are_ordered(corr):
+    """One context serves every torch stream of a device and its scratch is
+    shared, while ri_form_bia returns before its kernels finish.  Calls issued
+    back to back on two streams (no host synchronisation between them) must
+    give the same bits as the same calls run one after the other."""
+    naux, N, o = 608, 192, 40
+    _say("ri_form_bia on two torch streams")
+    rng = np.random.default_rng(81)
+    ins = []
+    for seed in (82, 83):
+        J = S.metric(naux, seed=seed, cond=1e4)
+        P = rng.standard_normal((naux, N, N))
+        ins.append((dev(S.orthonormal(N, seed=seed)), dev(P + P.transpose(0, 2, 1)), dev(J)))
+    serial = []
+    for C, P, J in ins:
+        M, _ = corr.ri_metric(J, "cholesky")
+        serial.append((M, corr.ri_form_bia(C, P, M, o, "cholesky", "fp32").cpu()))
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for s in (s1, s2):
+        s.wait_stream(torch.cuda.current_stream())
+    outs = []
+    for rep in range(3):
+        for (C, P, _), (M, _), s in zip(ins, serial, (s1, s2)):
+            with torch.cuda.stream(s):
+                outs.append(corr.ri_form_bia(C, P, M, o, "cholesky", "fp32"))
+    torch.cuda.synchronize()
+    for n, B in enumerate(outs):
+        assert torch.equal(B.cpu(), serial[n % 2][1]), n
+
+
 @pytest.mark.parametrize("mode", ["cholesky", "eigen"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_ri_bia_from_j_equals_two_calls(corr, mode, prec):
@@ -206,12 +238,63 @@ def test_rimp2_reduced_precision(corr, prec, rtol):
         assert abs(got[k] - ref[k]) <= rtol * abs(ref[k]), (k, got[k], ref[k])
 
 
-def test_rimp2_deterministic(corr):
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_rimp2_deterministic(corr, prec):
+    """Fixed-order reductions: repeated calls give bit-identical energies in
+    every precision."""
     naux, o, v = 140, 13, 70
     B, (eo, ev) = S.bia(naux, o, v, seed=9), S.orbital_energies(o, v, seed=9)
-    a = corr.rimp2_energy(dev(B), dev(eo), dev(ev))
-    b = corr.rimp2_energy(dev(B), dev(eo), dev(ev))
-    assert a == b
+    a = corr.rimp2_energy(dev(B), dev(eo), dev(ev), prec)
+    for _ in range(3):
+        assert corr.rimp2_energy(dev(B), dev(eo), dev(ev), prec) == a
+
+
+def _water_fixture():
+    from pathlib import Path
+    w = np.load(Path(__file__).resolve().parent / "golden" / "rimp2_water_def2svp.npz")
+    return w["Bia"], w["eps_occ"], w["eps_vir"], {"EMP2_OS": float(w["e_os"]), "EMP2_SS": float(w["e_ss"]),
+                                                  "EMP2": float(w["e_mp2"]), "EMP2_SCS": float(w["e_scs"])}
+
+
+def _separated_waters(n, seed):
+    """n copies of the water's B with no overlap between molecules (B^Q_ia = 0
+    unless Q, i and a belong to one water), occupied, virtual and auxiliary
+    indices each shuffled: M^{ij}_ab vanishes across molecules, so EMP2 is
+    exactly n times the water's (size extensivity).  Realistic magnitudes at
+    o = 5n, v = 19n, naux = 76n."""
+    Bw, eo, ev, e = _water_fixture()
+    naux1, o1, v1 = Bw.shape
+    B = np.zeros((n * naux1, n * o1, n * v1))
+    for m in range(n):
+        B[m * naux1:(m + 1) * naux1, m * o1:(m + 1) * o1, m * v1:(m + 1) * v1] = Bw
+    rng = np.random.default_rng(seed)
+    po, pv, pq = rng.permutation(n * o1), rng.permutation(n * v1), rng.permutation(n * naux1)
+    B = B[pq][:, po][:, :, pv]
+    return B, np.tile(eo, n)[po], np.tile(ev, n)[pv], {k: n * x for k, x in e.items()}
+
+
+@pytest.mark.parametrize("prec,bound_uEh", [("fp64", 1e-4), ("fp32", 0.05), ("bf16x3", 0.3)])
+def test_rimp2_water_def2svp_micro_hartree(corr, prec, bound_uEh):
+    """Real one-water B (def2-SVP / def2-SVP-RI, oracle integrals and RHF,
+    eigen metric): every energy component within an absolute bound in
+    micro-hartree (measured: FP32 0.004, BF16x3 0.125)."""
+    B, eo, ev, ref = _water_fixture()
+    got = corr.rimp2_energy(dev(B), dev(eo), dev(ev), prec=prec)
+    err = max(abs(got[k] - ref[k]) for k in ref) * 1e6
+    _say(f"rimp2 water def2-SVP {prec}: max |diff| = {err:.6f} uEh")
+    assert err <= bound_uEh
+
+
+@pytest.mark.parametrize("prec,bound_uEh", [("fp64", 1e-4), ("fp32", 0.4), ("bf16x3", 2.4)])
+def test_rimp2_separated_waters_size_extensive(corr, prec, bound_uEh):
+    """Eight separated waters (o = 40, v = 152, naux = 608, shuffled indices):
+    EMP2 = 8 x EMP2(water) exactly; the reduced precisions within the
+    single-water bounds times 8."""
+    B, eo, ev, ref = _separated_waters(8, seed=3)
+    got = corr.rimp2_energy(dev(B), dev(eo), dev(ev), prec=prec)
+    err = max(abs(got[k] - ref[k]) for k in ref) * 1e6
+    _say(f"rimp2 8 separated waters {prec}: max |diff| = {err:.6f} uEh")
+    assert err <= bound_uEh
 
 
 @pytest.mark.parametrize("naux", [7, 64, 65, 200])
