@@ -38,8 +38,8 @@ struct KernelTimer {
     corr_ctx* ctx;
     const char* name;
     hipEvent_t a = nullptr, b = nullptr;
-    KernelTimer(corr_ctx* c, const char* n) : ctx(c), name(n) {
-        if (ctx->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+    KernelTimer(corr_ctx* c, const char* n, bool on = true) : ctx(c), name(n) {
+        if (on && ctx->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
             (void)hipEventRecord(a, ctx->stream);
         else
             a = b = nullptr;
@@ -81,6 +81,35 @@ bool diis_solve(std::vector<double> B, int m, std::vector<double>& c);
 // Deterministic sum of n doubles (device) into *h_out (host), nser (<= 16)
 // interleaved series: data[i*nser + s].
 int reduce_sum(corr_ctx* ctx, const double* d_data, int64_t n, int nser, double* h_out);
+// The same with caller-provided device workspace: d_out holds 16 doubles,
+// d_mid 16 * REDUCE_NCHUNK (used when n > 64 * REDUCE_NCHUNK).
+constexpr int64_t REDUCE_NCHUNK = 256;
+int reduce_sum_ws(corr_ctx* ctx, const double* d_data, int64_t n, int nser, double* h_out, double* d_mid,
+                  double* d_out);
+
+// Device workspace of one RI-MP2 energy call (rimp2.hip): the repacked
+// panels, the per-tile partial sums, the BF16x3 tile list and the reduction's
+// two buffers.
+struct RimpWs {
+    void* panels = nullptr;
+    double* partial = nullptr;
+    ushort4* tiles = nullptr;
+    double* red_mid = nullptr;
+    double* red_out = nullptr;
+};
+// Carves the workspace of one energy call out of base (nullptr: sizes only)
+// in 256-byte aligned pieces; returns the bytes needed, or -1 for arguments
+// the energy rejects.
+int64_t rimp2_workspace(int64_t naux, int64_t nocc, int64_t nvir, int prec, void* base, RimpWs* ws);
+// The energy on a caller-provided workspace (no allocation, no kernel timing).
+int rimp2_energy_with(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
+                      const double* eo, const double* ev, int prec, corr_mp2_energy* out, const RimpWs* ws);
+
+// Device address of the allocation-free entry points' static scratch on this
+// device (noalloc.hip), looked up once per device by corr_ctx_create with the
+// device current; nullptr if the lookup failed.
+int noalloc_register_device(int device);
+void* noalloc_scratch(int device, size_t* bytes);
 
 }  // namespace corr
 

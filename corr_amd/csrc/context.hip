@@ -58,6 +58,9 @@ int corr_ctx_create(int device, corr_ctx** out) {
         return CORR_EHIP;
     }
     c->stream = c->own_stream;
+    // Device address of the allocation-free entry point's static scratch on
+    // this device (current here); a failed lookup only disables that entry.
+    (void)corr::noalloc_register_device(device);
     *out = c;
     return CORR_OK;
 }

@@ -524,14 +524,19 @@ __global__ __launch_bounds__(256) void reduce_chunks_kernel(const double* __rest
 }
 
 int reduce_sum(corr_ctx* ctx, const double* d_data, int64_t n, int nser, double* h_out) {
-    double* d_out = nullptr;
+    double *d_out = nullptr, *d_mid = nullptr;
     CORR_TRY(scratch(ctx, "reduce_out", sizeof(double) * 16, (void**)&d_out));
+    if (n > 64 * REDUCE_NCHUNK)
+        CORR_TRY(scratch(ctx, "reduce_mid", sizeof(double) * 16 * REDUCE_NCHUNK, (void**)&d_mid));
+    return reduce_sum_ws(ctx, d_data, n, nser, h_out, d_mid, d_out);
+}
+
+int reduce_sum_ws(corr_ctx* ctx, const double* d_data, int64_t n, int nser, double* h_out, double* d_mid,
+                  double* d_out) {
     CORR_REQUIRE(ctx, nser >= 1 && nser <= 16, "reduce_sum: 1..16 series");
     // Long inputs: 256 fixed chunks first, so the single-workgroup pass is short.
-    constexpr int64_t NCHUNK = 256;
+    constexpr int64_t NCHUNK = REDUCE_NCHUNK;
     if (n > 64 * NCHUNK) {
-        double* d_mid = nullptr;
-        CORR_TRY(scratch(ctx, "reduce_mid", sizeof(double) * 16 * NCHUNK, (void**)&d_mid));
         hipLaunchKernelGGL(reduce_chunks_kernel, dim3((unsigned)NCHUNK), dim3(256), 0, ctx->stream, d_data, n, nser,
                            cdiv(n, NCHUNK), d_mid);
         CORR_LAUNCHED(ctx, "reduce_chunks_kernel");

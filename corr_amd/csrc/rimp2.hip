@@ -737,14 +737,34 @@ static WideTiles wide_tiles(int nIB, int nAC) {
     return w;
 }
 
-// Repacks Bia into the panel layout for the given precision; returns the
-// device pointer in *out.
-int ri_repack(corr_ctx* ctx, const RiLayout& l, int64_t naux, int o, int v, const double* Bia,
-              int prec, const void** out) {
-    const int64_t elems = (int64_t)l.nIB * l.nAC * l.Kp * PW;
+struct RimpPlan {
+    RiLayout l;
+    WideTiles wt;    // BF16x3 only
+    size_t b_panels, b_partial, b_tiles;
+};
+
+static RimpPlan rimp2_plan(int64_t naux, int64_t o, int64_t v, int prec) {
+    RimpPlan p;
+    const bool bf = prec == CORR_PREC_BF16X3;
+    p.l = ri_layout(naux, o, v, bf ? 48 : 64);
     const size_t esz = prec == CORR_PREC_FP64 ? 8 : 4;   // bf16 hi+lo = 4 B
-    void* Bp = nullptr;
-    CORR_TRY(scratch(ctx, "rimp2_panels", elems * esz, &Bp));
+    p.b_panels = (size_t)p.l.nIB * p.l.nAC * p.l.Kp * PW * esz;
+    if (bf) {
+        p.wt = wide_tiles(p.l.nIB, p.l.nAC);
+        p.b_partial = sizeof(double) * 2 * p.wt.nT;
+        p.b_tiles = sizeof(ushort4) * p.wt.nT;
+    } else {
+        p.wt = WideTiles{0, 0, 0, 0};
+        p.b_partial = sizeof(double) * 2 * p.l.nWG;
+        p.b_tiles = 0;
+    }
+    return p;
+}
+
+// Repacks Bia into the panel layout for the given precision at Bp.
+static int ri_repack(corr_ctx* ctx, const RiLayout& l, int64_t naux, int o, int v, const double* Bia, int prec,
+                     void* Bp) {
+    const int64_t elems = (int64_t)l.nIB * l.nAC * l.Kp * PW;
     if (prec == CORR_PREC_BF16X3) {
         const int64_t Ks = l.Kp / 16;
         const int64_t blocks = (int64_t)l.nIB * l.nAC * Ks;
@@ -760,35 +780,60 @@ int ri_repack(corr_ctx* ctx, const RiLayout& l, int64_t naux, int o, int v, cons
                                Bia, (float*)Bp, naux, o, v, l.nAC, l.Kp, elems);
     }
     CORR_LAUNCHED(ctx, "rimp2 repack");
-    *out = Bp;
     return CORR_OK;
 }
 
-// BF16x3: the wide-tile kernel over a tile list built on the device.
-static int rimp2_energy_bf16x3(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
-                               const double* eo, const double* ev, corr_mp2_energy* out) {
-    const RiLayout l = ri_layout(naux, nocc, nvir, 48);
-    CORR_REQUIRE(ctx, l.nIB < 65535 && l.nAC < 65535, "rimp2: nocc/nvir too large for the tile list");
-    const WideTiles wt = wide_tiles(l.nIB, l.nAC);
-    const int64_t per = cdiv(wt.nT, 8);
+// The energy.  ws == nullptr: workspace from the context's cached scratch;
+// otherwise the caller's (sized by rimp2_workspace_bytes).  timed: bracket
+// the energy kernel with events when the context's timing is on.
+static int rimp2_energy_impl(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
+                             const double* eo, const double* ev, int prec, corr_mp2_energy* out, const RimpWs* ws,
+                             bool timed) {
+    CORR_REQUIRE(ctx, naux > 0 && nocc > 0 && nvir > 0, "rimp2: sizes must be positive");
+    CORR_REQUIRE(ctx, nocc < (1 << 20) && nvir < (1 << 20), "rimp2: nocc/nvir too large");
+    CORR_REQUIRE(ctx, Bia && eo && ev && out, "rimp2: null pointer");
+    CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
+                 "rimp2: unknown precision");
+    const RimpPlan p = rimp2_plan(naux, nocc, nvir, prec);
+    const RiLayout& l = p.l;
+    const bool bf = prec == CORR_PREC_BF16X3;
+    CORR_REQUIRE(ctx, !bf || (l.nIB < 65535 && l.nAC < 65535), "rimp2: nocc/nvir too large for the tile list");
+    const int64_t nWG = bf ? p.wt.nT : l.nWG, per = cdiv(nWG, 8);
     CORR_REQUIRE(ctx, per * 8 < ((int64_t)1 << 31), "rimp2: too many tiles");
-    const void* Bp = nullptr;
-    CORR_TRY(ri_repack(ctx, l, naux, (int)nocc, (int)nvir, Bia, CORR_PREC_BF16X3, &Bp));
-    ushort4* d_tiles = nullptr;
-    double* partial = nullptr;
-    CORR_TRY(scratch(ctx, "rimp2_tiles", sizeof(ushort4) * wt.nT, (void**)&d_tiles));
-    CORR_TRY(scratch(ctx, "rimp2_partial", sizeof(double) * 2 * wt.nT, (void**)&partial));
-    hipLaunchKernelGGL(wide_tile_list, dim3(1), dim3(1024), 0, ctx->stream, d_tiles, l.nIB, l.nAC, wt.nJG, wt.nSP,
-                       wt.NS);
-    CORR_LAUNCHED(ctx, "wide_tile_list");
-    KernelTimer timer(ctx, "rimp2_energy");
-    hipLaunchKernelGGL(rimp2_energy_bf16x3_wide, dim3((unsigned)(per * 8)), dim3(WIDE_THREADS), 0, ctx->stream,
-                       (const uint16_t*)Bp, d_tiles, eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp / 16, wt.nT,
-                       per, partial);
-    CORR_LAUNCHED(ctx, "rimp2_energy_bf16x3_wide");
-    timer.stop();
+    RimpWs w;
+    if (ws) {
+        w = *ws;
+    } else {
+        CORR_TRY(scratch(ctx, "rimp2_panels", p.b_panels, &w.panels));
+        CORR_TRY(scratch(ctx, "rimp2_partial", p.b_partial, (void**)&w.partial));
+        if (bf) CORR_TRY(scratch(ctx, "rimp2_tiles", p.b_tiles, (void**)&w.tiles));
+        CORR_TRY(scratch(ctx, "reduce_out", sizeof(double) * 16, (void**)&w.red_out));
+        CORR_TRY(scratch(ctx, "reduce_mid", sizeof(double) * 16 * REDUCE_NCHUNK, (void**)&w.red_mid));
+    }
+    CORR_TRY(ri_repack(ctx, l, naux, (int)nocc, (int)nvir, Bia, prec, w.panels));
+    if (bf) {
+        hipLaunchKernelGGL(wide_tile_list, dim3(1), dim3(1024), 0, ctx->stream, w.tiles, l.nIB, l.nAC, p.wt.nJG,
+                           p.wt.nSP, p.wt.NS);
+        CORR_LAUNCHED(ctx, "wide_tile_list");
+    }
+    {
+        KernelTimer timer(ctx, "rimp2_energy", timed);
+        if (bf)
+            hipLaunchKernelGGL(rimp2_energy_bf16x3_wide, dim3((unsigned)(per * 8)), dim3(WIDE_THREADS), 0,
+                               ctx->stream, (const uint16_t*)w.panels, w.tiles, eo, ev, (int)nocc, (int)nvir, l.nIB,
+                               l.nAC, l.Kp / 16, nWG, per, w.partial);
+        else if (prec == CORR_PREC_FP64)
+            hipLaunchKernelGGL(rimp2_energy_real<double>, dim3((unsigned)(per * 8)), dim3(WG_THREADS), 0,
+                               ctx->stream, (const double*)w.panels, eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp,
+                               nWG, per, w.partial);
+        else
+            hipLaunchKernelGGL(rimp2_energy_real<float>, dim3((unsigned)(per * 8)), dim3(WG_THREADS), 0,
+                               ctx->stream, (const float*)w.panels, eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp,
+                               nWG, per, w.partial);
+        CORR_LAUNCHED(ctx, "rimp2_energy");
+    }
     double s[2];
-    CORR_TRY(reduce_sum(ctx, partial, wt.nT, 2, s));
+    CORR_TRY(reduce_sum_ws(ctx, w.partial, nWG, 2, s, w.red_mid, w.red_out));
     out->e_os = s[0];
     out->e_ss = s[0] + s[1];
     out->e_mp2 = out->e_os + out->e_ss;
@@ -798,35 +843,32 @@ static int rimp2_energy_bf16x3(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_
 
 int rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
                  const double* eo, const double* ev, int prec, corr_mp2_energy* out) {
-    CORR_REQUIRE(ctx, naux > 0 && nocc > 0 && nvir > 0, "rimp2: sizes must be positive");
-    CORR_REQUIRE(ctx, nocc < (1 << 20) && nvir < (1 << 20), "rimp2: nocc/nvir too large");
-    CORR_REQUIRE(ctx, Bia && eo && ev && out, "rimp2: null pointer");
-    CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
-                 "rimp2: unknown precision");
-    if (prec == CORR_PREC_BF16X3) return rimp2_energy_bf16x3(ctx, naux, nocc, nvir, Bia, eo, ev, out);
-    const RiLayout l = ri_layout(naux, nocc, nvir);
-    CORR_REQUIRE(ctx, l.per * 8 < ((int64_t)1 << 31), "rimp2: too many tiles");
-    const void* Bp = nullptr;
-    CORR_TRY(ri_repack(ctx, l, naux, (int)nocc, (int)nvir, Bia, prec, &Bp));
-    double* partial = nullptr;
-    CORR_TRY(scratch(ctx, "rimp2_partial", sizeof(double) * 2 * l.nWG, (void**)&partial));
-    const dim3 grid((unsigned)(l.per * 8)), block(WG_THREADS);
-    KernelTimer timer(ctx, "rimp2_energy");
-    if (prec == CORR_PREC_FP64)
-        hipLaunchKernelGGL(rimp2_energy_real<double>, grid, block, 0, ctx->stream, (const double*)Bp,
-                           eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp, l.nWG, l.per, partial);
-    else
-        hipLaunchKernelGGL(rimp2_energy_real<float>, grid, block, 0, ctx->stream, (const float*)Bp,
-                           eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp, l.nWG, l.per, partial);
-    CORR_LAUNCHED(ctx, "rimp2_energy");
-    timer.stop();
-    double s[2];
-    CORR_TRY(reduce_sum(ctx, partial, l.nWG, 2, s));
-    out->e_os = s[0];
-    out->e_ss = s[0] + s[1];
-    out->e_mp2 = out->e_os + out->e_ss;
-    out->e_scs = out->e_ss / 3.0 + out->e_os * 6.0 / 5.0;
-    return CORR_OK;
+    return rimp2_energy_impl(ctx, naux, nocc, nvir, Bia, eo, ev, prec, out, nullptr, true);
+}
+
+int64_t rimp2_workspace(int64_t naux, int64_t nocc, int64_t nvir, int prec, void* base, RimpWs* ws) {
+    if (naux <= 0 || nocc <= 0 || nvir <= 0 || nocc >= (1 << 20) || nvir >= (1 << 20)) return -1;
+    if (prec != CORR_PREC_FP64 && prec != CORR_PREC_FP32 && prec != CORR_PREC_BF16X3) return -1;
+    const RimpPlan p = rimp2_plan(naux, nocc, nvir, prec);
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void* {
+        void* q = base ? (void*)((char*)base + off) : nullptr;
+        off += (bytes + 255) / 256 * 256;
+        return q;
+    };
+    RimpWs w;
+    w.panels = take(p.b_panels);
+    w.partial = (double*)take(p.b_partial);
+    w.tiles = (ushort4*)take(p.b_tiles);
+    w.red_mid = (double*)take(sizeof(double) * 16 * REDUCE_NCHUNK);
+    w.red_out = (double*)take(sizeof(double) * 16);
+    if (ws) *ws = w;
+    return (int64_t)off;
+}
+
+int rimp2_energy_with(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia, const double* eo,
+                      const double* ev, int prec, corr_mp2_energy* out, const RimpWs* ws) {
+    return rimp2_energy_impl(ctx, naux, nocc, nvir, Bia, eo, ev, prec, out, ws, false);
 }
 
 }  // namespace corr

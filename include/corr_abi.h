@@ -38,6 +38,7 @@
 #ifndef CORR_ABI_H
 #define CORR_ABI_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -183,6 +184,20 @@ int corr_ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,
 int corr_rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir,
                       const double* d_Bia, const double* d_eps_occ,
                       const double* d_eps_vir, int prec, corr_mp2_energy* h_out);
+
+/* corr_rimp2_energy_noalloc: corr_rimp2_energy without any device allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture setup, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its workspace is a static device array of
+ * CORR_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends; problems needing more return CORR_EINVAL with the size in
+ * the error string.  Same kernels and results as corr_rimp2_energy (bit for
+ * bit); no kernel timing is recorded.  Calls that overlap on one GPU share the
+ * array and race: serialise them. */
+#define CORR_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int corr_rimp2_energy_noalloc(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir,
+                              const double* d_Bia, const double* d_eps_occ,
+                              const double* d_eps_vir, int prec, corr_mp2_energy* h_out);
 
 /* ---- RI-MP2 energy gradient (no reference counterpart: the reference has no
  * gradient).  Writes dE/dBia (same layout as Bia), dE/d eps_occ, dE/d eps_vir
