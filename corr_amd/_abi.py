@@ -102,6 +102,7 @@ SIGNATURES = {
     "corr_ctx_synchronize": [P],
     "corr_ctx_last_error": [P],
     "corr_ctx_trim": [P],
+    "corr_ctx_set_scratch_limit": [P, I64],
     "corr_ctx_enable_timing": [P, I],
     "corr_ctx_kernel_time": [P, C.c_char_p, C.POINTER(D), C.POINTER(I64)],
     "corr_dmalloc": [P, C.POINTER(P), I64],

@@ -16,6 +16,7 @@ struct corr_ctx {
     std::string last_error;
     struct Buf { void* p = nullptr; size_t bytes = 0; };
     std::map<std::string, Buf> scratch;
+    size_t scratch_limit = 0;          // bytes of cached scratch allowed (0: no limit)
     void* blas_handle = nullptr;   // rocBLAS handle used by the eigen metric
     hipStream_t aux_stream = nullptr;   // second stream (default priority) for overlapped work
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;

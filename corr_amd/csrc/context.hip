@@ -13,6 +13,16 @@ int scratch(corr_ctx* ctx, const char* name, size_t bytes, void** out) {
     corr_ctx::Buf& b = ctx->scratch[name];
     if (bytes == 0) bytes = 256;
     if (b.bytes < bytes) {
+        if (ctx->scratch_limit) {
+            size_t held = 0;
+            for (const auto& kv : ctx->scratch) held += kv.second.bytes;
+            if (held - b.bytes + bytes > ctx->scratch_limit)
+                return set_error(ctx, CORR_ENOMEM,
+                                 std::string("scratch '") + name + "' of " + std::to_string(bytes) +
+                                     " bytes would exceed the context's scratch limit of " +
+                                     std::to_string(ctx->scratch_limit) + " bytes (" + std::to_string(held) +
+                                     " held)");
+        }
         if (b.p) {
             // The buffer may still be in use by queued work.
             hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -110,6 +120,13 @@ int corr_ctx_set_stream(corr_ctx* ctx, void* hip_stream) {
     CORR_HIP(ctx, hipEventRecord(ctx->ev_switch, ctx->stream));
     CORR_HIP(ctx, hipStreamWaitEvent(next, ctx->ev_switch, 0));
     ctx->stream = next;
+    return CORR_OK;
+}
+
+int corr_ctx_set_scratch_limit(corr_ctx* ctx, int64_t bytes) {
+    CORR_CHECK_CTX(ctx);
+    CORR_REQUIRE(ctx, bytes >= 0, "corr_ctx_set_scratch_limit: negative limit");
+    ctx->scratch_limit = (size_t)bytes;
     return CORR_OK;
 }
 

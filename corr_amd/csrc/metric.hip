@@ -10,6 +10,8 @@
 //                       X21 = -X22 L21 X11).  Both give
 //                       M with M^T M = J^-1, hence identical B^T B products.
 #include "common.hpp"
+
+#include <string>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 #include <vector>
@@ -170,7 +172,8 @@ __global__ void fill_kernel(double* __restrict__ A, int64_t n, double v) {
         A[e] = v;
 }
 
-// rows of W scaled: W[c][:] *= s(lambda_c) with the reference cutoff.
+// rows of W scaled: W[c][:] *= s(lambda_c) with the reference cutoff;
+// nkept[0] counts the kept eigenvalues, nkept[1] the non-finite ones.
 __global__ void eig_scale_kernel(double* __restrict__ W, const double* __restrict__ lam, int64_t n,
                                  int64_t* __restrict__ nkept) {
     const double lmax = lam[n - 1];
@@ -181,6 +184,7 @@ __global__ void eig_scale_kernel(double* __restrict__ W, const double* __restric
         const double s = (l / lmax < 1.0e-10 || l < 0.0) ? 0.0 : 1.0 / sqrt(l);
         W[e] *= s;
         if (e % n == 0 && s != 0.0) atomicAdd((unsigned long long*)nkept, 1ull);
+        if (e % n == 0 && !isfinite(l)) atomicAdd((unsigned long long*)(nkept + 1), 1ull);
     }
 }
 
@@ -308,10 +312,13 @@ static int eigen_invsqrt(corr_ctx* ctx, int64_t n, const double* J, double* M, i
     hipLaunchKernelGGL(eig_scale_kernel, dim3(grid_for(n * n)), dim3(256), 0, ctx->stream, Ws, lam, n, kept);
     CORR_LAUNCHED(ctx, "eig_scale_kernel");
     CORR_TRY(gemm(ctx, 1, 0, n, n, n, 1.0, W, n, 0, Ws, n, 0, 0.0, M, n, 0, 1, CORR_PREC_FP64));
-    if (nkept) {
-        CORR_HIP(ctx, hipMemcpyAsync(nkept, kept, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-        CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    }
+    int64_t h_kept[2] = {0, 0};
+    CORR_HIP(ctx, hipMemcpyAsync(h_kept, kept, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_kept[1] != 0)
+        return set_error(ctx, CORR_ENOTPD, "ri metric: J has " + std::to_string(h_kept[1]) +
+                                               " non-finite eigenvalues (NaN or Inf in J)");
+    if (nkept) *nkept = h_kept[0];
     return CORR_OK;
 }
 

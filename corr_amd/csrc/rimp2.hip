@@ -337,11 +337,13 @@ __device__ inline void wide_k_loop(Bf16Stage6& L0, Bf16Stage6& L1, Bf16Stage6& L
     CORR_WAIT_VMCNT(0);
 }
 
-// 1/D for D > 0: FP32 reciprocal refined by two FP64 Newton steps.
+// 1/D for D > 0: FP32 reciprocal refined by two FP64 Newton steps.  D = +inf
+// (an infinite orbital energy) gives 0, as IEEE division does; NaN stays NaN.
 __device__ inline double rcp_pos(double d) {
-    double x = (double)__builtin_amdgcn_rcpf((float)d);
+    const double x0 = (double)__builtin_amdgcn_rcpf((float)d);
+    double x = fma(x0, fma(-d, x0, 1.0), x0);
     x = fma(x, fma(-d, x, 1.0), x);
-    return fma(x, fma(-d, x, 1.0), x);
+    return x0 == 0.0 ? 0.0 : x;
 }
 
 // One element's share of the opposite-spin and exchange sums, weight w / D.

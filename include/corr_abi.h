@@ -92,6 +92,11 @@ int         corr_ctx_synchronize(corr_ctx* ctx);
 const char* corr_ctx_last_error(const corr_ctx* ctx);
 /* Releases the context's cached device scratch. */
 int         corr_ctx_trim(corr_ctx* ctx);
+/* Caps the device scratch the context may cache, in bytes (0: no cap).  A
+ * call whose workspace would exceed it fails with CORR_ENOMEM and leaves the
+ * context usable; corr_ctx_trim releases what is held.  No reference
+ * counterpart (the reference never checks allocations). */
+int         corr_ctx_set_scratch_limit(corr_ctx* ctx, int64_t bytes);
 /* Kernel timing: when enabled, the main kernel of each entry point is
  * bracketed by HIP events on the context stream.  corr_ctx_kernel_time
  * returns the summed duration (ms) and launch count of one kernel name
