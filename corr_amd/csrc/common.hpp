@@ -106,6 +106,12 @@ int64_t rimp2_workspace(int64_t naux, int64_t nocc, int64_t nvir, int prec, void
 int rimp2_energy_with(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
                       const double* eo, const double* ev, int prec, corr_mp2_energy* out, const RimpWs* ws);
 
+// Pair-tile part of the RI-MP2 gradient (rimp2.hip): the symmetric matrix R
+// (ov x ov doubles at R, every element written), dE/deps and the energies.
+int rimp2_grad_tiles(corr_ctx* ctx, int64_t naux, int64_t o, int64_t v, const double* Bia, const double* eo,
+                     const double* ev, double c_os, double c_ss, int prec, double* R, double* deo, double* dev,
+                     corr_mp2_energy* out);
+
 // Device address of the allocation-free entry points' static scratch on this
 // device (noalloc.hip), looked up once per device by corr_ctx_create with the
 // device current; nullptr if the lookup failed.

@@ -537,11 +537,155 @@ __device__ inline void real_stage_compute(const RealStage<T>& S, const int (&c1)
     }
 }
 
-template <typename T>
+// Gradient outputs of the pair-tile kernel (GRAD = true): R = [(c_os + c_ss)
+// M_ab - c_ss M_ba] / D into the symmetric ov x ov matrix R (rows (i a)), and
+// 72 FP64 partials of dE/deps per tile: [A block 32 | B block 32 | I block 4 |
+// J block 4].
+struct RGradArgs {
+    double* R;
+    double* peps;
+    double cs, css;
+};
+
+// Epilogue of the gradient: dE/deps partials (a 128 x 128 FP64 matrix of
+// g = w N / D^2 in LDS, row and column sums in a fixed order), then the four
+// symmetric positions of R for every element pair: R1 = R[(i a)][(j b)] and
+// R2^T = R[(j a)][(i b)] straight from the registers, R2 = R[(i b)][(j a)] and
+// R1^T = R[(j b)][(i a)] through a per-wave 32 x 33 LDS transpose (rows of 32
+// contiguous doubles).  A position reached twice (A = B, i = j, and the i > j
+// pairs of I = J tiles) is written once.
+template <typename ACC>
+__device__ inline void grad_epilogue(const ACC (&acc1)[4][2], const ACC (&acc2)[4][2], const PairTile& t, int o,
+                                     int v, const double* __restrict__ eo, const double* __restrict__ ev,
+                                     const RGradArgs& g, double* b0, double* b1, double* b2, double* b3,
+                                     int64_t id, double& sos, double& sx) {
+    // the four stage buffers are separate LDS objects: pick one by index
+    auto Lb = [&](int k) { return k == 0 ? b0 : k == 1 ? b1 : k == 2 ? b2 : b3; };
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 2, wc = wave & 3;
+    const int j = t.JB * PI + wc;
+    const double wt = (t.IB < t.JB ? 2.0 : 1.0) * (t.AC < t.BC ? 1.0 : 0.5);
+    const int64_t ov = (int64_t)o * v;
+    // phase 1: energy sums and g into LDS: row (il a) -> buffer il, row a of it,
+    // column (wc b) stored at (col + a) & 127 (bank rotation)
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        const int il = 2 * wr + rb / 2, i = t.IB * PI + il;
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+            const int bl = cb * 16 + (lane & 15), b = t.BC * PA + bl;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int al = (rb % 2) * 16 + AccLayout<ACC>::row(lane, r), a = t.AC * PA + al;
+                double gv = 0.0;
+                if (i < o && j < o && a < v && b < v) {
+                    const double m1 = (double)acc1[rb][cb][r], m2 = (double)acc2[rb][cb][r];
+                    const double dinv = 1.0 / (ev[a] + ev[b] - eo[i] - eo[j]);
+                    sos += (m1 * m1 + m2 * m2) * dinv;
+                    sx += m1 * m2 * dinv;
+                    const double nt = g.cs * (m1 * m1 + m2 * m2) - 2.0 * g.css * m1 * m2;
+                    gv = wt * nt * dinv * dinv;
+                }
+                Lb(il)[al * PW + ((wc * PA + bl + al) & (PW - 1))] = gv;
+            }
+        }
+    }
+    __syncthreads();
+    // row sums (threads 0..127: row (il a)) and column sums (128..255: column (jl b))
+    __shared__ double sums[2 * PW];
+    if (threadIdx.x < 2 * PW) {
+        double s = 0.0;
+        if (threadIdx.x < PW) {
+            const int row = threadIdx.x, il = row / PA, al = row % PA;
+            for (int c = 0; c < PW; ++c) s += Lb(il)[al * PW + ((c + al) & (PW - 1))];
+        } else {
+            const int col = threadIdx.x - PW;
+            for (int row = 0; row < PW; ++row) {
+                const int il = row / PA, al = row % PA;
+                s += Lb(il)[al * PW + ((col + al) & (PW - 1))];
+            }
+        }
+        sums[threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 72) {
+        const int k = threadIdx.x;
+        double s = 0.0;
+        if (k < 32) {
+            for (int il = 0; il < PI; ++il) s += sums[il * PA + k];                 // eps_a, a in A
+        } else if (k < 64) {
+            for (int jl = 0; jl < PI; ++jl) s += sums[PW + jl * PA + (k - 32)];     // eps_b, b in B
+        } else if (k < 68) {
+            for (int al = 0; al < PA; ++al) s -= sums[(k - 64) * PA + al];          // eps_i, i in I
+        } else {
+            for (int bl = 0; bl < PA; ++bl) s -= sums[PW + (k - 68) * PA + bl];     // eps_j, j in J
+        }
+        g.peps[id * 72 + k] = s;
+    }
+    __syncthreads();
+    // phase 2: R, one occupied pair (i = 2wr + p, j) at a time
+    double* V = Lb(wave / 3) + (wave % 3) * (PA * (PA + 1));
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const int i = t.IB * PI + 2 * wr + p;
+        const bool live = i < o && j < o && !(t.IB == t.JB && i > j);
+        const bool w1t = live && !(i == j && t.AC == t.BC);
+        const bool w2 = live && t.AC != t.BC && i != j;
+        double r1[2][2][4], r2[2][2][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                const int rb = 2 * p + h;
+                const int bl = cb * 16 + (lane & 15), b = t.BC * PA + bl;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int al = h * 16 + AccLayout<ACC>::row(lane, r), a = t.AC * PA + al;
+                    const double m1 = (double)acc1[rb][cb][r], m2 = (double)acc2[rb][cb][r];
+                    const bool ok = live && a < v && b < v;
+                    const double dinv = ok ? 1.0 / (ev[a] + ev[b] - eo[i] - eo[j]) : 0.0;
+                    r1[h][cb][r] = (g.cs * m1 - g.css * m2) * dinv;
+                    r2[h][cb][r] = (g.cs * m2 - g.css * m1) * dinv;
+                    if (ok) {
+                        g.R[((int64_t)i * v + a) * ov + (int64_t)j * v + b] = r1[h][cb][r];
+                        if (w2) g.R[((int64_t)j * v + a) * ov + (int64_t)i * v + b] = r2[h][cb][r];
+                    }
+                }
+            }
+        // transposed positions through LDS: V[a][b], read along a
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int al = h * 16 + AccLayout<ACC>::row(lane, r), bl = cb * 16 + (lane & 15);
+                        V[al * (PA + 1) + bl] = pass == 0 ? r2[h][cb][r] : r1[h][cb][r];
+                    }
+            __syncthreads();
+            const bool wr_ok = pass == 0 ? w2 : w1t;
+            const int al = lane & 31, a = t.AC * PA + al;
+            if (wr_ok && a < v) {
+                for (int bb = lane >> 5; bb < PA; bb += 2) {
+                    const int b = t.BC * PA + bb;
+                    if (b >= v) continue;
+                    const double x = V[al * (PA + 1) + bb];
+                    if (pass == 0) g.R[((int64_t)i * v + b) * ov + (int64_t)j * v + a] = x;   // R2 at (i b),(j a)
+                    else g.R[((int64_t)j * v + b) * ov + (int64_t)i * v + a] = x;            // R1^T at (j b),(i a)
+                }
+            }
+        }
+    }
+}
+
+template <typename T, bool GRAD>
 __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_real(
     const T* __restrict__ Bp, const double* __restrict__ eo, const double* __restrict__ ev,
     int o, int v, int nIB, int nAC, int64_t Kp, int64_t nWG, int64_t per,
-    double* __restrict__ partial) {
+    double* __restrict__ partial, RGradArgs gargs) {
     constexpr int KS = RealMfma<T>::KS;
     __shared__ __attribute__((aligned(16))) RealStage<T> L0;
     __shared__ __attribute__((aligned(16))) RealStage<T> L1;
@@ -630,7 +774,19 @@ __global__ __launch_bounds__(WG_THREADS) void rimp2_energy_real(
 #undef CORR_RSTAGE
     CORR_WAIT_VMCNT(0);   // no copy outlives the loop
     double sos = 0.0, sx = 0.0;
-    if constexpr (TWO_LEVEL) {
+    if constexpr (GRAD) {
+        __syncthreads();   // the stage buffers become the epilogue's LDS
+        double* b0 = reinterpret_cast<double*>(&L0[0][0][0]);
+        double* b1 = reinterpret_cast<double*>(&L1[0][0][0]);
+        double* b2 = reinterpret_cast<double*>(&L2[0][0][0]);
+        double* b3 = reinterpret_cast<double*>(&L3[0][0][0]);
+        if constexpr (TWO_LEVEL) {
+            flush();
+            grad_epilogue(tot1, tot2, t, o, v, eo, ev, gargs, b0, b1, b2, b3, id, sos, sx);
+        } else {
+            grad_epilogue(acc1, acc2, t, o, v, eo, ev, gargs, b0, b1, b2, b3, id, sos, sx);
+        }
+    } else if constexpr (TWO_LEVEL) {
         flush();
         energy_epilogue(tot1, tot2, t, o, v, eo, ev, sos, sx);
     } else {
@@ -825,13 +981,13 @@ static int rimp2_energy_impl(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t 
                                ctx->stream, (const uint16_t*)w.panels, w.tiles, eo, ev, (int)nocc, (int)nvir, l.nIB,
                                l.nAC, l.Kp / 16, nWG, per, w.partial);
         else if (prec == CORR_PREC_FP64)
-            hipLaunchKernelGGL(rimp2_energy_real<double>, dim3((unsigned)(per * 8)), dim3(WG_THREADS), 0,
-                               ctx->stream, (const double*)w.panels, eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp,
-                               nWG, per, w.partial);
+            hipLaunchKernelGGL((rimp2_energy_real<double, false>), dim3((unsigned)(per * 8)), dim3(WG_THREADS),
+                               0, ctx->stream, (const double*)w.panels, eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC,
+                               l.Kp, nWG, per, w.partial, RGradArgs{nullptr, nullptr, 0.0, 0.0});
         else
-            hipLaunchKernelGGL(rimp2_energy_real<float>, dim3((unsigned)(per * 8)), dim3(WG_THREADS), 0,
-                               ctx->stream, (const float*)w.panels, eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC, l.Kp,
-                               nWG, per, w.partial);
+            hipLaunchKernelGGL((rimp2_energy_real<float, false>), dim3((unsigned)(per * 8)), dim3(WG_THREADS),
+                               0, ctx->stream, (const float*)w.panels, eo, ev, (int)nocc, (int)nvir, l.nIB, l.nAC,
+                               l.Kp, nWG, per, w.partial, RGradArgs{nullptr, nullptr, 0.0, 0.0});
         CORR_LAUNCHED(ctx, "rimp2_energy");
     }
     double s[2];
@@ -846,6 +1002,93 @@ static int rimp2_energy_impl(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t 
 int rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
                  const double* eo, const double* ev, int prec, corr_mp2_energy* out) {
     return rimp2_energy_impl(ctx, naux, nocc, nvir, Bia, eo, ev, prec, out, nullptr, true);
+}
+
+// dE/deps from the per-tile partials, in a fixed order: block t < o sums,
+// for occupied i = t, the I-slots of tiles (I = i/4, J >= I) and the J-slots
+// of tiles (I <= J = i/4) over every virtual pair; block o + a likewise the
+// A-slots of tiles (A = a/32, B >= A) and the B-slots of (A <= B = a/32).
+// Thread k takes the terms k, k + 256, ...; a fixed LDS tree adds the 256.
+__device__ inline int64_t tri_index(int64_t r, int64_t c, int64_t n) { return r * n - r * (r - 1) / 2 + (c - r); }
+
+__global__ __launch_bounds__(256) void grad_eps_reduce(const double* __restrict__ peps, int nIB, int nAC, int o,
+                                                       int v, double* __restrict__ deo, double* __restrict__ dev) {
+    __shared__ double sh[256];
+    const int t = blockIdx.x, k = threadIdx.x;
+    const int64_t nAB = (int64_t)nAC * (nAC + 1) / 2, nIJ = (int64_t)nIB * (nIB + 1) / 2;
+    double s = 0.0;
+    if (t < o) {
+        const int Y = t / PI, li = t % PI;
+        const int64_t n1 = (int64_t)(nIB - Y) * nAB, n = n1 + (int64_t)(Y + 1) * nAB;
+        for (int64_t e = k; e < n; e += 256) {
+            if (e < n1) {
+                const int64_t J = Y + e / nAB;
+                s += peps[(tri_index(Y, J, nIB) * nAB + e % nAB) * 72 + 64 + li];
+            } else {
+                const int64_t I = (e - n1) / nAB;
+                s += peps[(tri_index(I, Y, nIB) * nAB + (e - n1) % nAB) * 72 + 68 + li];
+            }
+        }
+    } else {
+        const int a = t - o, X = a / PA, la = a % PA;
+        const int64_t n = nIJ * (nAC + 1);
+        for (int64_t e = k; e < n; e += 256) {
+            const int64_t ij = e / (nAC + 1), r = e % (nAC + 1);
+            if (r < nAC - X) s += peps[(ij * nAB + tri_index(X, X + r, nAC)) * 72 + la];
+            else s += peps[(ij * nAB + tri_index(r - (nAC - X), X, nAC)) * 72 + 32 + la];
+        }
+    }
+    sh[k] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (k < w) sh[k] += sh[k + w];
+        __syncthreads();
+    }
+    if (k == 0) {
+        if (t < o) deo[t] = sh[0];
+        else dev[t - o] = sh[0];
+    }
+}
+
+// The pair-tile part of the gradient: R (symmetric ov x ov, rows (i a)),
+// dE/deps and the energies, for E = c_os EMP2_OS + c_ss EMP2_SS; FP64 or FP32
+// (two-level accumulation) MFMA.  dE/dB = -4 B R is left to the caller.
+int rimp2_grad_tiles(corr_ctx* ctx, int64_t naux, int64_t o, int64_t v, const double* Bia, const double* eo,
+                     const double* ev, double c_os, double c_ss, int prec, double* R, double* deo, double* dev,
+                     corr_mp2_energy* out) {
+    const int p = prec == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_PREC_FP32;
+    const RimpPlan pl = rimp2_plan(naux, o, v, p);
+    const RiLayout& l = pl.l;
+    const int64_t nWG = l.nWG, per = cdiv(nWG, 8);
+    CORR_REQUIRE(ctx, per * 8 < ((int64_t)1 << 31), "rimp2_gradient: too many tiles");
+    void* panels = nullptr;
+    double *partial = nullptr, *peps = nullptr;
+    CORR_TRY(scratch(ctx, "rimp2_panels", pl.b_panels, &panels));
+    CORR_TRY(scratch(ctx, "rimp2_partial", pl.b_partial, (void**)&partial));
+    CORR_TRY(scratch(ctx, "grad_peps", sizeof(double) * 72 * nWG, (void**)&peps));
+    CORR_TRY(ri_repack(ctx, l, naux, (int)o, (int)v, Bia, p, panels));
+    const RGradArgs g{R, peps, c_os + c_ss, c_ss};
+    if (p == CORR_PREC_FP64)
+        hipLaunchKernelGGL((rimp2_energy_real<double, true>), dim3((unsigned)(per * 8)), dim3(WG_THREADS), 0,
+                           ctx->stream, (const double*)panels, eo, ev, (int)o, (int)v, l.nIB, l.nAC, l.Kp, nWG, per,
+                           partial, g);
+    else
+        hipLaunchKernelGGL((rimp2_energy_real<float, true>), dim3((unsigned)(per * 8)), dim3(WG_THREADS), 0,
+                           ctx->stream, (const float*)panels, eo, ev, (int)o, (int)v, l.nIB, l.nAC, l.Kp, nWG, per,
+                           partial, g);
+    CORR_LAUNCHED(ctx, "rimp2_energy_real (gradient)");
+    hipLaunchKernelGGL(grad_eps_reduce, dim3((unsigned)(o + v)), dim3(256), 0, ctx->stream, peps, l.nIB,
+                       l.nAC, (int)o, (int)v, deo, dev);
+    CORR_LAUNCHED(ctx, "grad_eps_reduce");
+    double s[2];
+    CORR_TRY(reduce_sum(ctx, partial, nWG, 2, s));
+    if (out) {
+        out->e_os = s[0];
+        out->e_ss = s[0] + s[1];
+        out->e_mp2 = out->e_os + out->e_ss;
+        out->e_scs = out->e_ss / 3.0 + out->e_os * 6.0 / 5.0;
+    }
+    return CORR_OK;
 }
 
 int64_t rimp2_workspace(int64_t naux, int64_t nocc, int64_t nvir, int prec, void* base, RimpWs* ws) {

@@ -8,7 +8,10 @@
 //   dE/dB^Q_ia = -4 sum_{jb} R^{ij}_ab B^Q_jb
 //   dE/de_a = 2 sum_{ijb} N^{ij}_ab / D^2,   dE/de_i = -2 sum_{jab} N^{ij}_ab / D^2
 //
-// Occupied orbitals are processed in blocks I: the pair blocks
+// Up to 48 GB of R (o v <= ~77000) the pair-tile MFMA kernel of the energy
+// recomputes M on its symmetric (I <= J, A <= B) tiles and writes all of R
+// and per-tile dE/de partials from registers; one GEMM gives dE/dB = -4 B R.
+// Larger problems: occupied orbitals are processed in blocks I: the pair blocks
 // G_I[(i a)][(j b)] = M^{ij}_ab for i in I are recomputed with one MFMA GEMM
 // (never the full o^2 v^2 set), a fused kernel turns them into R_I and the
 // FP64 partial sums for dE/de and E, and one more GEMM adds -4 B R_I^T to
@@ -20,6 +23,10 @@ namespace corr {
 namespace {
 
 constexpr int RT = 32;   // (a, b) tile edge of the pair-block kernel
+// Largest R (ov x ov doubles) that the fused path holds at once; larger
+// problems (or R above half of a context scratch limit) take the
+// occupied-blocked path with slabs of ~6 GB (a quarter of the limit).
+constexpr double FUSED_R_BYTES = 48.0e9;
 
 struct GradArgs {
     const double* G;      // [bo*v][o*v] pair blocks of this occupied block
@@ -164,8 +171,26 @@ int rimp2_gradient(corr_ctx* ctx, int64_t naux, int64_t o, int64_t v, const doub
                  "rimp2_gradient: unknown precision");
     const int gp = prec == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_PREC_FP32;
     const int64_t ov = o * v;
+    // a context scratch limit also bounds R (at most half the limit) and the slabs
+    const double lim = ctx->scratch_limit ? (double)ctx->scratch_limit : 0.0;
+    const double rbytes = 8.0 * (double)ov * (double)ov;
+    if (rbytes <= FUSED_R_BYTES && (lim == 0.0 || rbytes <= 0.5 * lim)) {
+        // fused: the pair-tile kernel writes the whole symmetric R and the
+        // dE/deps partials, one GEMM forms dE/dB = -4 B R
+        double* R;
+        CORR_TRY(scratch(ctx, "grad_R", sizeof(double) * ov * ov, (void**)&R));
+        KernelTimer timer(ctx, "rimp2_gradient");
+        corr_mp2_energy e{};
+        CORR_TRY(rimp2_grad_tiles(ctx, naux, o, v, B, eo, ev, c_os, c_ss, gp, R, deo, dev, &e));
+        CORR_TRY(gemm(ctx, 0, 0, naux, ov, ov, -4.0, B, ov, 0, R, ov, 0, 0.0, dB, ov, 0, 1, gp));
+        timer.stop();
+        CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (out) *out = e;
+        return CORR_OK;
+    }
     // occupied block size: G and R slabs of at most ~6 GB each
-    int64_t bo = (int64_t)(6.0e9 / (8.0 * (double)v * (double)ov));
+    const double slab = lim == 0.0 ? 6.0e9 : std::min(6.0e9, 0.25 * lim);
+    int64_t bo = (int64_t)(slab / (8.0 * (double)v * (double)ov));
     if (bo < 1) bo = 1;
     if (bo > o) bo = o;
     const int nT = (int)cdiv(v, RT);

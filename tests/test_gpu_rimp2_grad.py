@@ -74,3 +74,39 @@ def test_autograd_function(corr):
     assert np.abs(B.grad.cpu().numpy() - dB).max() < 1e-10 * np.abs(dB).max()
     assert np.abs(eo.grad.cpu().numpy() - deo).max() < 1e-10 * np.abs(deo).max()
     assert np.abs(ev.grad.cpu().numpy() - dev_).max() < 1e-10 * np.abs(dev_).max()
+
+
+def test_fused_and_blocked_paths_agree(corr):
+    """Up to half the context's scratch limit R (o v x o v) is formed whole by
+    the pair-tile kernel; above it the occupied-blocked path runs.  Both give
+    the oracle's gradient; reruns of the fused path are bit-identical."""
+    from corr_amd._abi import Context, lib
+    naux, o, v = 120, 13, 100     # ragged: 4 occupied blocks, 4 virtual blocks
+    print(f"fused vs blocked naux={naux} o={o} v={v}", flush=True)
+    B = S.bia(naux, o, v, seed=5)
+    eo, ev = S.orbital_energies(o, v, seed=6)
+    E, dB, deo, dev_ = O.rimp2_gradient(B, eo, ev, 1.2, 1.0 / 3.0)
+    d = (dev(B), dev(eo), dev(ev))
+    fused = corr.rimp2_gradient(*d, 1.2, 1.0 / 3.0, prec="fp64")
+    again = corr.rimp2_gradient(*d, 1.2, 1.0 / 3.0, prec="fp64")
+    for x, y in zip(fused[1:], again[1:]):
+        assert torch.equal(x, y)
+    assert fused[0] == again[0]
+    ctx = Context.get(0)
+    ov = o * v
+    ctx.check("corr_ctx_trim", lib().corr_ctx_trim(ctx.handle))
+    ctx.check("corr_ctx_set_scratch_limit", lib().corr_ctx_set_scratch_limit(ctx.handle, 8 * ov * ov))
+    try:
+        blocked = corr.rimp2_gradient(*d, 1.2, 1.0 / 3.0, prec="fp64")
+    finally:
+        ctx.check("corr_ctx_set_scratch_limit", lib().corr_ctx_set_scratch_limit(ctx.handle, 0))
+    sc = np.abs(dB).max()
+    for g in (fused, blocked):
+        assert abs(g[0] - E) < 1e-12 * abs(E)
+        assert np.abs(g[1].cpu().numpy() - dB).max() < 1e-10 * sc
+        assert np.abs(g[2].cpu().numpy() - deo).max() < 1e-10 * np.abs(deo).max()
+        assert np.abs(g[3].cpu().numpy() - dev_).max() < 1e-10 * np.abs(dev_).max()
+    f32 = corr.rimp2_gradient(*d, 1.2, 1.0 / 3.0, prec="fp32")
+    assert abs(f32[0] - E) < 1e-6 * abs(E)
+    assert np.abs(f32[1].cpu().numpy() - dB).max() < 1e-5 * sc
+    assert np.abs(f32[3].cpu().numpy() - dev_).max() < 1e-5 * np.abs(dev_).max()
