@@ -41,3 +41,18 @@ def test_graft_entry_exposes_build_and_smoke():
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
     assert callable(mod.build) and callable(mod.smoke)
+
+
+def test_metric_is_the_headline_metric_and_line_keys_are_unique():
+    """The JSON line's "metric" is the headline metric string, and no dict
+    literal in bench.py repeats a key (a repeated key silently overwrites)."""
+    import ast
+    import json
+    with open(os.path.join(ROOT, "BASELINE.json")) as f:
+        assert bench.METRIC == json.load(f)["metric"]
+    with open(os.path.join(ROOT, "bench.py")) as f:
+        tree = ast.parse(f.read())
+    for node in ast.walk(tree):
+        if isinstance(node, ast.Dict):
+            keys = [k.value for k in node.keys if isinstance(k, ast.Constant)]
+            assert len(keys) == len(set(keys)), keys
