@@ -312,6 +312,22 @@ static int ao2vo(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double
     return CORR_OK;
 }
 
+// X[Q][i][a] = sum_{mu nu} C[o+a,mu] C[i,nu] Y[Q][mu][nu] for Y[Q] symmetric in
+// (mu, nu) (3-index Coulomb integrals and their metric products are), in the
+// [Q][i][a] layout the metric GEMM and the energy read, with no transpose:
+//   Zt[mu][Q][i] = sum_nu Y[Q][mu][nu] C[i][nu]    (batched over mu: rows Q
+//                  at stride N^2, K-contiguous; naux x o x N per batch)
+//   X[(Q i)][a]  = sum_mu Zt[mu][(Q i)] C[o+a][mu]  (one GEMM, naux o x v x N)
+static int ao2ov_sym(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Y,
+                     double* X, int p) {
+    const int64_t v = N - o;
+    double* Zt = nullptr;
+    CORR_TRY(scratch(ctx, "ri_Znui", sizeof(double) * naux * N * o, (void**)&Zt));
+    CORR_TRY(gemm(ctx, 0, 1, naux, o, N, 1.0, Y, N * N, N, C, N, 0, 0.0, Zt, o, naux * o, N, p));
+    CORR_TRY(gemm(ctx, 1, 1, naux * o, v, N, 1.0, Zt, naux * o, 0, C + o * N, N, 0, 0.0, X, v, 0, 1, p));
+    return CORR_OK;
+}
+
 // dst[q][i][a] = src[q][a][i]  (batched transpose through LDS)
 __global__ __launch_bounds__(256) void transpose_vo_kernel(const double* __restrict__ src,
                                                            double* __restrict__ dst, int64_t v, int64_t o,
@@ -360,14 +376,12 @@ int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double*
     CORR_REQUIRE(ctx, mode == CORR_METRIC_EIGEN || mode == CORR_METRIC_CHOLESKY, "ri_form_bia: unknown metric mode");
     const int64_t v = N - o;
     const int p = gemm_prec(prec);
-    double *X = nullptr, *Y = nullptr;
-    CORR_TRY(scratch(ctx, "ri_Xvo", sizeof(double) * naux * o * v, (void**)&X));
-    CORR_TRY(scratch(ctx, "ri_Yvo", sizeof(double) * naux * o * v, (void**)&Y));
-    CORR_TRY(ao2vo(ctx, naux, N, o, C, Pmn, X, p));
-    // Y[Q][(a i)] = sum_P M[Q][P] X[P][(a i)]   (M lower triangular for CHOLESKY)
-    CORR_TRY(gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Y, o * v, 0, 1,
-                  metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY));
-    return transpose_vo(ctx, naux, v, o, Y, Bia);
+    double* X = nullptr;
+    CORR_TRY(scratch(ctx, "ri_Xov", sizeof(double) * naux * o * v, (void**)&X));
+    CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, p));
+    // Bia[Q][(i a)] = sum_P M[Q][P] X[P][(i a)]   (M lower triangular for CHOLESKY)
+    return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1,
+                metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY);
 }
 
 // B from J and (P|mu nu) with the metric factorisation overlapped: the AO->MO
@@ -382,16 +396,15 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
     CORR_REQUIRE(ctx, J != M, "ri_bia_from_j: J and M must not alias");
     const int64_t v = N - o;
     const int p = gemm_prec(prec);
-    double *X = nullptr, *Y = nullptr;
-    CORR_TRY(scratch(ctx, "ri_Xvo", sizeof(double) * naux * o * v, (void**)&X));
-    CORR_TRY(scratch(ctx, "ri_Yvo", sizeof(double) * naux * o * v, (void**)&Y));
+    double* X = nullptr;
+    CORR_TRY(scratch(ctx, "ri_Xov", sizeof(double) * naux * o * v, (void**)&X));
     if (!M) CORR_TRY(scratch(ctx, "ri_M", sizeof(double) * naux * naux, (void**)&M));
     if (!ctx->aux_stream) CORR_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
     if (!ctx->ev_fork) CORR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
     if (!ctx->ev_join) CORR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     hipStream_t main = ctx->stream;
     CORR_HIP(ctx, hipEventRecord(ctx->ev_fork, main));
-    CORR_TRY(ao2vo(ctx, naux, N, o, C, Pmn, X, p));
+    CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, p));
     CORR_HIP(ctx, hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
     ctx->stream = ctx->aux_stream;
     int rc = ri_metric(ctx, naux, J, mode, M, nkept);
@@ -403,9 +416,8 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
         return rc;
     }
     CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_join, 0));
-    CORR_TRY(gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Y, o * v, 0, 1,
-                  metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY));
-    return transpose_vo(ctx, naux, v, o, Y, Bia);
+    return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1,
+                metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY);
 }
 
 }  // namespace corr

@@ -164,6 +164,10 @@ int corr_ri_metric(corr_ctx* ctx, int64_t naux, const double* d_J,
  *                   triangular and its zero half is skipped).  With EIGEN the
  *                   metric product runs in FP64 whatever prec asks: the dense
  *                   J^-1/2 amplifies FP32 rounding (4 uEh in EMP2 at (H2O)32).
+ *                   (P|mu nu) must be symmetric in (mu, nu), as 3-index
+ *                   Coulomb integrals are: nu is contracted with the occupied
+ *                   and mu with the virtual block, and Bia is written by the
+ *                   metric GEMM in place (no transpose).
  * GEMM steps requested in CORR_PREC_BF16X3 run in FP32 MFMA. */
 int corr_ri_bpq(corr_ctx* ctx, int64_t naux, int64_t nroao, const double* d_M,
                 const double* d_Pmn, double* d_BPQ, int prec);
