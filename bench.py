@@ -383,16 +383,19 @@ def run_suite(dev, seed: int, rows=(), main=None) -> dict:
         torch.cuda.empty_cache()
         Cm = r["C"]
         FMo = corr_amd.build_FMo(Cm, r["F"])
-        W4 = 8.0 * N ** 5
+        # four full quarter transforms: 8 N^5; executed: 6 N^5 (quarters 3 and 4
+        # run on the packed pairs i <= j of the (i, j)-symmetric half transform)
+        W4, W4x = 8.0 * N ** 5, 6.0 * N ** 5
         res = {"shape": "(H2O)8 N=192 o=40, real integrals", "e_rhf": r["scf"]["e_total"],
                "rhf_iterations": r["scf"]["iterations"], "unique_integrals": n_unique,
                "unfold_ms": round(t_unf * 1e3, 2), "unfold_GBs": round(nbytes_unf / t_unf / 1e9, 1),
-               "unfold_vs_tensor_max_abs": unf_diff, "W_4_TFLOP": round(W4 / 1e12, 3)}
+               "unfold_vs_tensor_max_abs": unf_diff, "W_4_full_quarters_TFLOP": round(W4 / 1e12, 3),
+               "W_4_executed_TFLOP": round(W4x / 1e12, 3)}
         emp2 = {}
         for prec in ("fp64", "fp32"):
             t = _timed(lambda: corr_amd.MOtrans(Cm, G, prec), reps=2)
             res[f"motrans_{prec}_ms"] = round(t * 1e3, 2)
-            res[f"motrans_{prec}_TFLOPs"] = round(W4 / t / 1e12, 2)
+            res[f"motrans_{prec}_executed_TFLOPs"] = round(W4x / t / 1e12, 2)
             ints = corr_amd.MOtrans(Cm, G, prec)
             emp2[prec] = corr_amd.run_canonical_mp2(ints, FMo, o)["EMP2"]
             if prec == "fp64":

@@ -66,6 +66,13 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
          const double* B, int64_t ldb, int64_t sB, double beta, double* C,
          int64_t ldc, int64_t sC, int64_t batch, int prec, bool a_lower = false);
 
+// op(A) op(B) (M x N, alpha 1) written through an output mapping of the
+// 4-index transform (gemm.hip): mapping 1 (PAIRS): row X n + i, column j ->
+// C[X P + tri(i, j)] for j >= i, P = n (n + 1) / 2; mapping 2 (MIRROR): row
+// tri(i, j) n + k, column l -> C[((i n + j) n + k) n + l] and (j i k l).
+int gemm_mapped(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K, const double* A,
+                int64_t lda, const double* B, int64_t ldb, double* C, int mapping, int64_t n, int prec);
+
 // J^-1/2 (EIGEN) or L^-1 of J = L L^T (CHOLESKY) into M (metric.hip).
 int ri_metric(corr_ctx* ctx, int64_t naux, const double* J, int mode, double* M, int64_t* nkept);
 

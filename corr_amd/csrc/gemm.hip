@@ -35,7 +35,31 @@ struct GemmArgs {
     int a_lower;
     int64_t ksplit;   // > 0: blockIdx.y is a K chunk of this depth (C = partial slice), not a batch index
     int dma;          // FP64 operands 16-byte aligned (rows, batch and K-chunk offsets): global_load_lds staging
+    int epi;          // output mapping (GEMM_EPI_*), beta = 0 unless PLAIN
+    int64_t en;       // index extent of the PAIRS / MIRROR mappings
 };
+
+// Output mappings of element (row, col) of op(A) op(B), with n = en and
+// P = n (n + 1) / 2 (used by the 4-index transform):
+//  PLAIN : C[row * ldc + col]
+//  PAIRS : row = X n + i, col = j: C[X P + tri(i, j)] for j >= i only, tri(i, j)
+//          = i n - i (i - 1) / 2 + (j - i) (the packed upper pairs of (i, j))
+//  MIRROR: row = q n + k with q = tri(i, j), col = l: C[((i n + j) n + k) n + l]
+//          and, for i != j, C[((j n + i) n + k) n + l]
+enum { GEMM_EPI_PLAIN = 0, GEMM_EPI_PAIRS = 1, GEMM_EPI_MIRROR = 2 };
+
+// (i, j), i <= j < n, of the packed pair index q = tri(i, j).
+__device__ inline void gemm_pair_decode(int64_t q, int64_t n, int64_t& i, int64_t& j) {
+    const double b = 2.0 * (double)n + 1.0;
+    int64_t r = (int64_t)((b - sqrt(b * b - 8.0 * (double)q)) * 0.5);
+    if (r < 0) r = 0;
+    if (r > n - 1) r = n - 1;
+    auto start = [n](int64_t x) { return x * n - x * (x - 1) / 2; };
+    while (r > 0 && start(r) > q) --r;
+    while (r + 1 < n && start(r + 1) <= q) ++r;
+    i = r;
+    j = r + (q - start(r));
+}
 
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
@@ -239,7 +263,7 @@ struct GemmTile {
     }
 };
 
-template <int TA, int TB, typename T, int WM, int WN>
+template <int TA, int TB, typename T, int WM, int WN, int EPI>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
     typedef GemmTile<TA, TB, T, WM, WN> GT;
     constexpr int BM = GT::BM, BN = GT::BN, BK = GT::BK;
@@ -332,24 +356,53 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
         GT::regs_loop(g, A, B, m0, n0, kEnd, 0, kTiles, As[0], As[1], Bs[0], Bs[1], wm, wn, lg, lc, acc);
     }
 
+    // Mapped outputs: each of this lane's 16 rows (row = X n + ri, ri < n)
+    // gets its destination base once, from the tile's first row and a
+    // bounded subtraction (no division per element).
+    const int64_t x0 = EPI == GEMM_EPI_PLAIN ? 0 : m0 / g.en, r0 = EPI == GEMM_EPI_PLAIN ? 0 : m0 % g.en;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int64_t col = n0 + wn * 32 + j * 16 + lc;
+        for (int r = 0; r < 4; ++r) {
+            const int off = wm * 64 + i * 16 + AccLayout<acc_t>::row(lane, r);
+            const int64_t row = m0 + off;
+            if (row >= g.M) continue;
+            int64_t d0 = 0, d1 = -1, ri = 0;
+            if constexpr (EPI != GEMM_EPI_PLAIN) {
+                const int64_t n = g.en;
+                int64_t x = x0;
+                ri = r0 + off;
+                while (ri >= n) { ri -= n; ++x; }
+                if constexpr (EPI == GEMM_EPI_PAIRS) {
+                    // row (x, i = ri), column j -> x P + i n - i (i - 1) / 2 + j - i
+                    d0 = x * (n * (n + 1) / 2) + ri * n - ri * (ri - 1) / 2 - ri;
+                } else {
+                    // row (q = x, k = ri), column l -> ((pi n + pj) n + k) n + l and (pj pi k l)
+                    int64_t pi, pj;
+                    gemm_pair_decode(x, n, pi, pj);
+                    d0 = ((pi * n + pj) * n + ri) * n;
+                    if (pi != pj) d1 = ((pj * n + pi) * n + ri) * n;
+                }
+            }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t row = m0 + wm * 64 + i * 16 + AccLayout<acc_t>::row(lane, r);
-                if (row < g.M && col < g.N) {
-                    double v = g.alpha * (double)acc[i][j][r];
+            for (int j = 0; j < 2; ++j) {
+                const int64_t col = n0 + wn * 32 + j * 16 + lc;
+                if (col >= g.N) continue;
+                double v = g.alpha * (double)acc[i][j][r];
+                if constexpr (EPI == GEMM_EPI_PLAIN) {
                     if (g.beta != 0.0) v += g.beta * C[row * g.ldc + col];
                     C[row * g.ldc + col] = v;
+                } else if constexpr (EPI == GEMM_EPI_PAIRS) {
+                    if (col >= ri) C[d0 + col] = v;
+                } else {
+                    C[d0 + col] = v;
+                    if (d1 >= 0) C[d1 + col] = v;
                 }
             }
         }
 }
 
-template <int TA, int TB, typename T>
+template <int TA, int TB, typename T, int EPI = GEMM_EPI_PLAIN>
 static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
     // Tile shape with the least weighted padded area of C: 128 x 128, 128 x 192 (N = 192
     // of the (H2O)8 4-index quarters in one tile, so A is read once), 128 x 160
@@ -388,13 +441,13 @@ static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
         dim3 grid((unsigned)tiles, (unsigned)nb);
         KernelTimer timer(ctx, "gemm");
         if (cfg == 1)
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2>), grid, dim3(nt), 0, ctx->stream, s);
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2, EPI>), grid, dim3(nt), 0, ctx->stream, s);
         else if (cfg == 3)
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 6>), grid, dim3(nt), 0, ctx->stream, s);
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 6, EPI>), grid, dim3(nt), 0, ctx->stream, s);
         else if (cfg == 2)
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 5>), grid, dim3(nt), 0, ctx->stream, s);
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 5, EPI>), grid, dim3(nt), 0, ctx->stream, s);
         else
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 4>), grid, dim3(nt), 0, ctx->stream, s);
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 4, EPI>), grid, dim3(nt), 0, ctx->stream, s);
         CORR_LAUNCHED(ctx, "gemm_kernel");
     }
     return CORR_OK;
@@ -472,6 +525,8 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
             CORR_TRY(scratch(ctx, ws, sizeof(double) * S * M * N, (void**)&P));
             GemmArgs a{M, N, K, A, lda, 0, B, ldb, 0, P, N, M * N, 1.0, 0.0, 0, 0, kc};
             a.dma = dma_ok(f64, A, lda, 0, B, ldb, 0) && kc % 2 == 0;
+            a.epi = GEMM_EPI_PLAIN;
+            a.en = 0;
             CORR_TRY(dispatch(ctx, transa, transb, f64, a, S));
             const int64_t grid = std::min<int64_t>(cdiv(M * N, 256), 4096);
             hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, P, S, M, N,
@@ -482,7 +537,26 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
     }
     GemmArgs a{M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, alpha, beta, 0, a_lower ? 1 : 0, 0};
     a.dma = dma_ok(f64, A, lda, sA, B, ldb, sB);
+    a.epi = GEMM_EPI_PLAIN;
+    a.en = 0;
     return dispatch(ctx, transa, transb, f64, a, batch);
+}
+
+int gemm_mapped(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K, const double* A,
+                int64_t lda, const double* B, int64_t ldb, double* C, int mapping, int64_t n, int prec) {
+    CORR_REQUIRE(ctx, M > 0 && N > 0 && K > 0 && A && B && C && n > 0, "gemm_mapped: bad arguments");
+    CORR_REQUIRE(ctx, (mapping == 1 || mapping == 2) && M % n == 0 && (mapping == 2 || N == n),
+                 "gemm_mapped: rows must be whole blocks of n (and N = n for PAIRS)");
+    CORR_REQUIRE(ctx, lda >= (transa ? M : K) && ldb >= (transb ? K : N), "gemm_mapped: leading dimension too small");
+    CORR_REQUIRE(ctx, transa == 1 && transb == 1, "gemm_mapped: only op(A) = A^T, op(B) = B^T");
+    const int f64 = (prec == CORR_PREC_FP64) ? 1 : 0;
+    GemmArgs a{M, N, K, A, lda, 0, B, ldb, 0, C, N, 0, 1.0, 0.0, 0, 0, 0};
+    a.dma = dma_ok(f64, A, lda, 0, B, ldb, 0);
+    a.epi = mapping;
+    a.en = n;
+    if (mapping == GEMM_EPI_PAIRS)
+        return f64 ? launch_t<1, 1, double, GEMM_EPI_PAIRS>(ctx, a, 1) : launch_t<1, 1, float, GEMM_EPI_PAIRS>(ctx, a, 1);
+    return f64 ? launch_t<1, 1, double, GEMM_EPI_MIRROR>(ctx, a, 1) : launch_t<1, 1, float, GEMM_EPI_MIRROR>(ctx, a, 1);
 }
 
 // Deterministic reduction: one workgroup, fixed strided order, fixed tree.

@@ -255,17 +255,28 @@ int ao_unfold(corr_ctx* ctx, int64_t N, int64_t nrofint, const int64_t* sc, cons
     return CORR_OK;
 }
 
+// (ij|kl) = sum C_i,mu C_j,nu C_k,la C_l,si (mu nu|la si) as four quarter
+// transforms, each contracting the leading index of the current tensor and
+// appending the new index last (one (N^3 x N) x (N x N) GEMM reading the
+// tensor as [first][rest]).  The AO tensor is symmetric in (mu, nu), so the
+// half-transformed T2[la si][i j] is symmetric in (i, j): quarter 2 writes
+// only the packed pairs i <= j (P = N(N+1)/2 of N^2), quarters 3 and 4 run
+// on them (half the work each: 6 N^5 instead of 8 N^5 FLOP in total), and
+// quarter 4 writes every result to (ij|kl) and (ji|kl).
+//   Q1 ao[mu][nu la si]           -> work[nu la si][i]
+//   Q2 work[nu][la si i]          -> out[la si][(ij)]     (pairs)
+//   Q3 out[la][si (ij)]           -> work[si (ij)][k]
+//   Q4 work[si][(ij) k]           -> out[i j k][l], out[j i k][l]
 int motrans(corr_ctx* ctx, int64_t N, const double* C, const double* ao, double* work, double* out,
             int prec) {
     CORR_REQUIRE(ctx, N > 0 && C && ao && work && out, "motrans: bad arguments");
     CORR_REQUIRE(ctx, ao != out && work != out && work != ao, "motrans: buffers must not alias");
-    const int64_t N3 = N * N * N;
+    const int64_t N2 = N * N, N3 = N2 * N, P = N * (N + 1) / 2;
     const int p = gemm_prec(prec);
-    const double* in[4] = {ao, work, out, work};
-    double* dst[4] = {work, out, work, out};
-    for (int s = 0; s < 4; ++s)
-        CORR_TRY(gemm(ctx, 1, 1, N3, N, N, 1.0, in[s], N3, 0, C, N, 0, 0.0, dst[s], N, 0, 1, p));
-    return CORR_OK;
+    CORR_TRY(gemm(ctx, 1, 1, N3, N, N, 1.0, ao, N3, 0, C, N, 0, 0.0, work, N, 0, 1, p));
+    CORR_TRY(gemm_mapped(ctx, 1, 1, N3, N, N, work, N3, C, N, out, 1, N, p));
+    CORR_TRY(gemm(ctx, 1, 1, N * P, N, N, 1.0, out, N * P, 0, C, N, 0, 0.0, work, N, 0, 1, p));
+    return gemm_mapped(ctx, 1, 1, P * N, N, N, work, P * N, C, N, out, 2, N, p);
 }
 
 int canonical_mp2(corr_ctx* ctx, int64_t N, int64_t o, const double* I4, const double* FMo,
