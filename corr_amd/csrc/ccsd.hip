@@ -227,6 +227,91 @@ __global__ void zp_kernel(const double* __restrict__ T1, const double* __restric
     }
 }
 
+// ---- spin-sorted compound indices ---------------------------------------------
+// Spin orbitals alternate alpha / beta (spin = index % 2, as in ints_so_kernel).
+// An (occupied p, virtual x) pair falls in class 0 (alpha alpha), 1 (beta beta),
+// 2 (alpha beta) or 3 (beta alpha); sorted index = class * (o/2)(v/2) +
+// (p/2)(v/2) + x/2.  Pair matrices of the ring terms (rows and columns such
+// pairs) are block-sparse in this order: amplitude-like ones (t_imae, <mn||ef>,
+// Z_nfjb) vanish outside blocks (0|1, 0|1), (2, 3), (3, 2); W_mbej-like ones
+// outside (0|1, 0|1), (2, 2), (3, 3).
+struct SpinDims {
+    int64_t o, v;   // even
+};
+__device__ inline int64_t spin_sorted(int64_t p, int64_t x, int64_t vh, int64_t B4) {
+    const int64_t sp = p & 1, sx = x & 1;
+    const int64_t cls = sp == sx ? sp : 2 + sp;
+    return cls * B4 + (p >> 1) * vh + (x >> 1);
+}
+__device__ inline void spin_unsorted(int64_t r, int64_t vh, int64_t B4, int64_t& p, int64_t& x) {
+    const int64_t cls = r / B4, loc = r % B4;
+    const int64_t sp = cls < 2 ? cls : cls - 2, sx = cls < 2 ? cls : 3 - cls;
+    p = 2 * (loc / vh) + sp;
+    x = 2 * (loc % vh) + sx;
+}
+
+// dst[r'][c'] (sorted, ov x ov) = src[p s0 + x s1 + q s2 + y s3] for r' = sorted(p, x),
+// c' = sorted(q, y).
+__global__ void spin_gather_kernel(double* __restrict__ dst, const double* __restrict__ src, SpinDims d,
+                                   int64_t s0, int64_t s1, int64_t s2, int64_t s3) {
+    const int64_t ov = d.o * d.v, vh = d.v / 2, B4 = ov / 4;
+    GRID_STRIDE(e, ov * ov) {
+        int64_t p, x, q, y;
+        spin_unsorted(e / ov, vh, B4, p, x);
+        spin_unsorted(e % ov, vh, B4, q, y);
+        dst[e] = src[p * s0 + x * s1 + q * s2 + y * s3];
+    }
+}
+
+// dst (dims n0..n3, contiguous) = beta dst + alpha S[sorted(p, x)][sorted(q, y)], where
+// destination index k is the role[k]-th of (p, x, q, y).
+struct SpinScatter {
+    int64_t n[4];
+    int role[4];
+};
+__global__ void spin_scatter_kernel(double* __restrict__ dst, const double* __restrict__ S, SpinDims d, SpinScatter m,
+                                    double alpha, double beta) {
+    const int64_t ov = d.o * d.v, vh = d.v / 2, B4 = ov / 4;
+    const int64_t total = m.n[0] * m.n[1] * m.n[2] * m.n[3];
+    GRID_STRIDE(e, total) {
+        int64_t idx[4], rem = e;
+        for (int k = 3; k >= 0; --k) {
+            idx[k] = rem % m.n[k];
+            rem /= m.n[k];
+        }
+        int64_t pxqy[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k) pxqy[m.role[k]] = idx[k];
+        const double val = S[spin_sorted(pxqy[0], pxqy[1], vh, B4) * ov + spin_sorted(pxqy[2], pxqy[3], vh, B4)];
+        dst[e] = beta == 0.0 ? alpha * val : beta * dst[e] + alpha * val;
+    }
+}
+
+// Largest |amplitude| that spin forbids: T1[i][a] with spin(i) != spin(a), T2[i][j][a][b]
+// with spin(i) + spin(j) != spin(a) + spin(b).  One maximum per block (order-free).
+__global__ __launch_bounds__(256) void spin_forbidden_kernel(const double* __restrict__ T1,
+                                                             const double* __restrict__ T2, int64_t o, int64_t v,
+                                                             double* __restrict__ partial) {
+    __shared__ double sh[256];
+    double m = 0.0;
+    const int64_t L1 = o * v, L = L1 + o * o * v * v;
+    GRID_STRIDE(e, L) {
+        if (e < L1) {
+            const int64_t i = e / v, a = e % v;
+            if ((i & 1) != (a & 1)) m = fmax(m, fabs(T1[e]));
+        } else {
+            const int64_t t = e - L1, b = t % v, a = (t / v) % v, j = (t / (v * v)) % o, i = t / (v * v * o);
+            if (((i & 1) + (j & 1)) != ((a & 1) + (b & 1))) m = fmax(m, fabs(T2[t]));
+        }
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
 __global__ void dot_kernel(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
                            double* __restrict__ partial) {
     __shared__ double sh[256];
@@ -297,6 +382,61 @@ struct CCSD {
     double *tau_s, *tau, *tauS_a, *Fae, *Faep, *Fmi, *Fmip, *Fme, *Yk, *Wmnij, *WabefP, *XabP, *tauP, *Ypp, *Wmbej;
     double *Zp, *W2, *Wp, *T2p, *T2a, *X1, *X2, *Zr, *U, *Q, *Tmp1;
     double* partial;
+    // Spin-blocked ring terms (V from ints_so_kernel: spin = index % 2).  Exact
+    // only while the amplitudes have no spin-forbidden part: the blocks that
+    // spin forbids are skipped.  The caller checks the new amplitudes after
+    // each update (spin_forbidden_max) and leaves this path for good once a
+    // forbidden amplitude exceeds SPIN_FORBIDDEN_TOL: for an RHF reference whose
+    // spin-flip sector is unstable, rounding noise there grows as in the dense
+    // spin-orbital iteration, which then runs on to its spin-broken root.
+    bool spin = false;
+    static constexpr double SPIN_FORBIDDEN_TOL = 1e-12;
+    int spin_forbidden_max(const double* T1, const double* T2, double* out) {
+        const int nb = 256;
+        hipLaunchKernelGGL(spin_forbidden_kernel, dim3(nb), dim3(256), 0, ctx->stream, T1, T2, o, v, partial);
+        CORR_LAUNCHED(ctx, "spin_forbidden_kernel");
+        double h[256];
+        CORR_HIP(ctx, hipMemcpyAsync(h, partial, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        double m = 0.0;
+        for (int k = 0; k < nb; ++k) m = std::max(m, h[k]);
+        *out = m;
+        return CORR_OK;
+    }
+    double *PmenfS = nullptr, *ZpS = nullptr, *W2S = nullptr, *T2S = nullptr, *WpS = nullptr, *ZrS = nullptr;
+
+    int spin_gather(double* dst, const double* src, int64_t s0, int64_t s1, int64_t s2, int64_t s3) {
+        const int64_t ov = o * v;
+        hipLaunchKernelGGL(spin_gather_kernel, dim3(grid_for(ov * ov)), dim3(256), 0, ctx->stream, dst, src,
+                           SpinDims{o, v}, s0, s1, s2, s3);
+        CORR_LAUNCHED(ctx, "spin_gather_kernel");
+        return CORR_OK;
+    }
+    int spin_scatter(double* dst, const double* S, std::initializer_list<int64_t> dims,
+                     std::initializer_list<int> roles, double alpha, double beta) {
+        SpinScatter m;
+        int k = 0;
+        for (auto d : dims) m.n[k++] = d;
+        k = 0;
+        for (auto r : roles) m.role[k++] = r;
+        hipLaunchKernelGGL(spin_scatter_kernel, dim3(grid_for(m.n[0] * m.n[1] * m.n[2] * m.n[3])), dim3(256), 0,
+                           ctx->stream, dst, S, SpinDims{o, v}, m, alpha, beta);
+        CORR_LAUNCHED(ctx, "spin_scatter_kernel");
+        return CORR_OK;
+    }
+    // C = A B on sorted pair matrices: the (0|1 x 0|1) super-block in one GEMM,
+    // then the two cross blocks: A (ra, ca) B (ca, cb) -> C (ra, cb) for
+    // (ra, ca, cb) = x[0..2] and x[3..5].
+    int spin_ring(const double* A, const double* B, double* C, const int (&x)[6]) {
+        const int64_t ov = o * v, B4 = ov / 4;
+        CORR_TRY(mm(0, 0, 2 * B4, 2 * B4, 2 * B4, 1.0, A, ov, 0, B, ov, 0, 0.0, C, ov, 0));
+        for (int k = 0; k < 2; ++k) {
+            const int64_t ra = x[3 * k] * B4, ca = x[3 * k + 1] * B4, cb = x[3 * k + 2] * B4;
+            CORR_TRY(mm(0, 0, B4, B4, B4, 1.0, A + ra * ov + ca, ov, 0, B + ca * ov + cb, ov, 0, 0.0, C + ra * ov + cb,
+                        ov, 0));
+        }
+        return CORR_OK;
+    }
 
     int perm(double* dst, const double* src, std::initializer_list<int64_t> dims,
              std::initializer_list<int64_t> strides, double alpha = 1.0, double beta = 0.0) {
@@ -447,6 +587,21 @@ struct CCSD {
         CORR_TRY(alloc(&Q, "cc_Q", O2V2));
         CORR_TRY(alloc(&Tmp1, "cc_Tmp1", o * v));
         CORR_TRY(alloc(&partial, "cc_partial", 512));
+        if (spin) {
+            const int64_t ov = o * v;
+            CORR_REQUIRE(ctx, o % 2 == 0 && v % 2 == 0, "ccsd: spin blocks need even nocc and nvir");
+            CORR_TRY(alloc(&PmenfS, "cc_PmenfS", ov * ov));
+            CORR_TRY(alloc(&ZpS, "cc_ZpS", ov * ov));
+            CORR_TRY(alloc(&W2S, "cc_W2S", ov * ov));
+            CORR_TRY(alloc(&T2S, "cc_T2S", ov * ov));
+            CORR_TRY(alloc(&WpS, "cc_WpS", ov * ov));
+            CORR_TRY(alloc(&ZrS, "cc_ZrS", ov * ov));
+            // the products never write the blocks that vanish by spin: zero them once
+            CORR_HIP(ctx, hipMemsetAsync(W2S, 0, sizeof(double) * ov * ov, ctx->stream));
+            CORR_HIP(ctx, hipMemsetAsync(ZrS, 0, sizeof(double) * ov * ov, ctx->stream));
+            // PmenfS[(m e)][(n f)] = <mn||ef>
+            CORR_TRY(spin_gather(PmenfS, Pmenf, v * ov, ov, v, 1));
+        }
         return CORR_OK;
     }
 
@@ -518,8 +673,16 @@ struct CCSD {
         CORR_TRY(mm(1, 0, v, v * o, o, -1.0, T1, v, 0, Voovo, v * o, o * v * o, 1.0, Wmbej, v * o, v * v * o, o));
         hipLaunchKernelGGL(zp_kernel, dim3(g2), dim3(256), 0, ctx->stream, T1, T2, o, v, Zp);
         CORR_LAUNCHED(ctx, "zp_kernel");
-        CORR_TRY(mm(0, 0, ov, ov, ov, 1.0, Pmenf, ov, 0, Zp, ov, 0, 0.0, W2, ov, 0));  // W2[m][e][j][b]
-        CORR_TRY(perm(Wmbej, W2, {o, v, v, o}, {v * ov, 1, ov, v}, -1.0, 1.0));
+        if (spin) {
+            // W2S = PmenfS ZpS: blocks (2,3)(3,2) and (3,2)(2,3) beside the 0|1 super-block
+            CORR_TRY(spin_gather(ZpS, Zp, v * ov, ov, v, 1));
+            CORR_TRY(spin_ring(PmenfS, ZpS, W2S, {2, 3, 2, 3, 2, 3}));
+            // Wmbej[m][b][e][j] -= W2[(m e)][(j b)]
+            CORR_TRY(spin_scatter(Wmbej, W2S, {o, v, v, o}, {0, 3, 1, 2}, -1.0, 1.0));
+        } else {
+            CORR_TRY(mm(0, 0, ov, ov, ov, 1.0, Pmenf, ov, 0, Zp, ov, 0, 0.0, W2, ov, 0));  // W2[m][e][j][b]
+            CORR_TRY(perm(Wmbej, W2, {o, v, v, o}, {v * ov, 1, ov, v}, -1.0, 1.0));
+        }
 
         // ---- T1 update (ccsd.cc:266-333)
         CORR_TRY(perm(T1n, f + o, {1, 1, o, v}, {0, 0, n, 1}));
@@ -566,8 +729,16 @@ struct CCSD {
             CORR_LAUNCHED(ctx, "scatter_pp_kernel");
         }
         // ring: Zr[(ia)][(jb)] = t_imae W_mbej - t_ie t_ma <mb||ej>
-        CORR_TRY(perm(Wp, Wmbej, {o, v, o, v}, {v2 * o, o, 1, v * o}));   // Wp[m][e][j][b]
-        CORR_TRY(mm(0, 0, ov, ov, ov, 1.0, T2p, ov, 0, Wp, ov, 0, 0.0, Zr, ov, 0));
+        if (spin) {
+            // ZrS = T2S WpS: blocks (2,3)(3,3) and (3,2)(2,2) beside the 0|1 super-block
+            CORR_TRY(spin_gather(T2S, T2, o * v2, v, v2, 1));              // t_imae: (i a) (m e)
+            CORR_TRY(spin_gather(WpS, Wmbej, v2 * o, o, 1, v * o));        // W_mbej: (m e) (j b)
+            CORR_TRY(spin_ring(T2S, WpS, ZrS, {2, 3, 3, 3, 2, 2}));
+            CORR_TRY(spin_scatter(Zr, ZrS, {o, v, o, v}, {0, 1, 2, 3}, 1.0, 0.0));
+        } else {
+            CORR_TRY(perm(Wp, Wmbej, {o, v, o, v}, {v2 * o, o, 1, v * o}));   // Wp[m][e][j][b]
+            CORR_TRY(mm(0, 0, ov, ov, ov, 1.0, T2p, ov, 0, Wp, ov, 0, 0.0, Zr, ov, 0));
+        }
         CORR_TRY(mm(0, 1, ov * o, o, v, 1.0, Pmbje, v, 0, T1, v, 0, 0.0, U, o, 0));   // U[m][b][j][i]
         CORR_TRY(mm(1, 0, v, v * o2, o, 1.0, T1, v, 0, U, v * o2, 0, 0.0, Q, v * o2, 0));  // Q[a][b][j][i]
         CORR_TRY(perm(Zr, Q, {o, v, o, v}, {1, v * o2, o, o2}, -1.0, 1.0));
@@ -680,6 +851,7 @@ int ccsd_run(corr_ctx* ctx, int64_t N, int64_t nroe, const double* I, const doub
     CORR_LAUNCHED(ctx, "form_fock_kernel");
 
     CCSD cc{ctx, o, v, n, opts->prec};
+    cc.spin = true;   // V from ints_so_kernel: alternating alpha / beta spin orbitals
     CORR_TRY(cc.setup(V));
     // HF check
     {
@@ -734,6 +906,11 @@ int ccsd_run(corr_ctx* ctx, int64_t N, int64_t nroe, const double* I, const doub
         {
             KernelTimer t(ctx, "ccsd_update");
             CORR_TRY(cc.update(f, X, X + L1, Xn, Xn + L1));
+        }
+        if (cc.spin) {
+            double fm;
+            CORR_TRY(cc.spin_forbidden_max(Xn, Xn + L1, &fm));
+            if (!(fm <= CCSD::SPIN_FORBIDDEN_TOL)) cc.spin = false;   // NaN included
         }
         if (!nd) {
             std::swap(X, Xn);

@@ -80,6 +80,12 @@ def test_ccsd_ur_reference_mode(corr):
 
 
 def test_ccsd_diis_converges_to_oracle(corr):
+    """DIIS to the oracle's root.  This system's spin-flip sector is unstable
+    (plain iterations diverge; rounding noise in spin-forbidden amplitudes grows
+    to 0.06 and DIIS lands on a spin-broken root), so it also checks that the
+    spin-blocked ring terms hand over to the dense ones once forbidden
+    amplitudes appear: without that the run stops 1e-5 Eh away, at amplitudes
+    that are not a root (oracle residual 4.5e-3)."""
     N, nocc_sp = 7, 2
     _say("ccsd diis N=7")
     h_ao, C, prec = system(N, nocc_sp, seed=23)
