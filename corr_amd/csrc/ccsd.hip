@@ -375,6 +375,7 @@ struct CCSD {
     // packed over antisymmetric pairs: <ab||ef> [(a<b)][(e<f)], <am||ef> [a][m][(e<f)]
     double *VvvvvP, *VvovvP;
     int64_t Po = 0, Pv = 0;            // o(o-1)/2, v(v-1)/2
+    int64_t po_cls[4] = {0, 0, 0, 0}, pv_cls[4] = {0, 0, 0, 0};   // class starts of the pair lists
     int32_t *po_i = nullptr, *po_j = nullptr, *pv_a = nullptr, *pv_b = nullptr;
     double *PeOOf, *Pmenf, *Pmine, *Pianf, *Pmbje, *Pabje;
     double *fo;   // f (n x n)
@@ -494,23 +495,30 @@ struct CCSD {
         CORR_TRY(alloc(&Vovvo, "cc_Vovvo", O2V2));
         CORR_TRY(alloc(&Vovvv, "cc_Vovvv", OV3));
         CORR_TRY(alloc(&Vvovv, "cc_Vvovv", OV3));
-        // pair index arrays of the packed pp ladder
+        // pair index arrays of the packed pp ladder, x < y, ordered by spin
+        // class (both alpha, both beta, mixed; spin = index % 2): with
+        // spin-symmetric amplitudes the ladder is block-diagonal in the classes
         Po = o * (o - 1) / 2;
         Pv = v * (v - 1) / 2;
         {
             std::vector<int32_t> h(2 * (Po + Pv) + 4);
-            int64_t k = 0;
-            for (int64_t x = 0; x < o; ++x)
-                for (int64_t y = x + 1; y < o; ++y, ++k) {
-                    h[k] = (int32_t)x;
-                    h[Po + k] = (int32_t)y;
+            auto fill = [&](int64_t n, int32_t* first, int32_t* second, int64_t* bounds) {
+                int64_t k = 0;
+                const int cls_order[3] = {0, 2, 1};   // (x&1)+(y&1): alpha alpha, beta beta, mixed
+                for (int c = 0; c < 3; ++c) {
+                    bounds[c] = k;
+                    for (int64_t x = 0; x < n; ++x)
+                        for (int64_t y = x + 1; y < n; ++y)
+                            if ((x & 1) + (y & 1) == cls_order[c]) {
+                                first[k] = (int32_t)x;
+                                second[k] = (int32_t)y;
+                                ++k;
+                            }
                 }
-            k = 0;
-            for (int64_t x = 0; x < v; ++x)
-                for (int64_t y = x + 1; y < v; ++y, ++k) {
-                    h[2 * Po + k] = (int32_t)x;
-                    h[2 * Po + Pv + k] = (int32_t)y;
-                }
+                bounds[3] = k;
+            };
+            fill(o, h.data(), h.data() + Po, po_cls);
+            fill(v, h.data() + 2 * Po, h.data() + 2 * Po + Pv, pv_cls);
             int32_t* d = nullptr;
             CORR_TRY(scratch(ctx, "cc_pairs", sizeof(int32_t) * h.size(), (void**)&d));
             CORR_HIP(ctx, hipMemcpyAsync(d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -599,6 +607,7 @@ struct CCSD {
             // the products never write the blocks that vanish by spin: zero them once
             CORR_HIP(ctx, hipMemsetAsync(W2S, 0, sizeof(double) * ov * ov, ctx->stream));
             CORR_HIP(ctx, hipMemsetAsync(ZrS, 0, sizeof(double) * ov * ov, ctx->stream));
+            CORR_HIP(ctx, hipMemsetAsync(Ypp, 0, sizeof(double) * std::max<int64_t>(Po * Pv, 1), ctx->stream));
             // PmenfS[(m e)][(n f)] = <mn||ef>
             CORR_TRY(spin_gather(PmenfS, Pmenf, v * ov, ov, v, 1));
         }
@@ -723,7 +732,17 @@ struct CCSD {
             hipLaunchKernelGGL(pack_tau_kernel, gp, dim3(256), 0, ctx->stream, tau, o, v, po_i, po_j, pv_a, pv_b, Po, Pv,
                                tauP);
             CORR_LAUNCHED(ctx, "pack_tau_kernel");
-            CORR_TRY(mm(0, 1, Po, Pv, Pv, 1.0, tauP, Pv, 0, WabefP, Pv, 0, 0.0, Ypp, Pv, 0));
+            if (spin) {
+                // diagonal class blocks only (the others stay zero from setup)
+                for (int c = 0; c < 3; ++c) {
+                    const int64_t r0 = po_cls[c], nr = po_cls[c + 1] - r0, k0 = pv_cls[c], nk = pv_cls[c + 1] - k0;
+                    if (nr > 0 && nk > 0)
+                        CORR_TRY(mm(0, 1, nr, nk, nk, 1.0, tauP + r0 * Pv + k0, Pv, 0, WabefP + k0 * Pv + k0, Pv, 0,
+                                    0.0, Ypp + r0 * Pv + k0, Pv, 0));
+                }
+            } else {
+                CORR_TRY(mm(0, 1, Po, Pv, Pv, 1.0, tauP, Pv, 0, WabefP, Pv, 0, 0.0, Ypp, Pv, 0));
+            }
             hipLaunchKernelGGL(scatter_pp_kernel, gp, dim3(256), 0, ctx->stream, Ypp, o, v, po_i, po_j, pv_a, pv_b, Po,
                                Pv, T2n);
             CORR_LAUNCHED(ctx, "scatter_pp_kernel");
