@@ -287,29 +287,31 @@ __global__ void spin_scatter_kernel(double* __restrict__ dst, const double* __re
 }
 
 // Largest |amplitude| that spin forbids: T1[i][a] with spin(i) != spin(a), T2[i][j][a][b]
-// with spin(i) + spin(j) != spin(a) + spin(b).  One maximum per block (order-free).
+// with spin(i) + spin(j) != spin(a) + spin(b).  Block (i, j) scans T2[i][j] (and
+// block 0 also T1); one maximum per block (order-free).
 __global__ __launch_bounds__(256) void spin_forbidden_kernel(const double* __restrict__ T1,
-                                                             const double* __restrict__ T2, int64_t o, int64_t v,
+                                                             const double* __restrict__ T2, int o, int v,
                                                              double* __restrict__ partial) {
     __shared__ double sh[256];
+    const int ij = blockIdx.x, i = ij / o, j = ij % o, sij = (i & 1) + (j & 1);
+    const double* t = T2 + (int64_t)ij * v * v;
     double m = 0.0;
-    const int64_t L1 = o * v, L = L1 + o * o * v * v;
-    GRID_STRIDE(e, L) {
-        if (e < L1) {
-            const int64_t i = e / v, a = e % v;
-            if ((i & 1) != (a & 1)) m = fmax(m, fabs(T1[e]));
-        } else {
-            const int64_t t = e - L1, b = t % v, a = (t / v) % v, j = (t / (v * v)) % o, i = t / (v * v * o);
-            if (((i & 1) + (j & 1)) != ((a & 1) + (b & 1))) m = fmax(m, fabs(T2[t]));
-        }
+    for (int e = threadIdx.x; e < v * v; e += 256) {
+        const int a = e / v, b = e - a * v;
+        if (((a & 1) + (b & 1)) != sij) m = fmax(m, fabs(t[e]));
     }
+    if (ij == 0)
+        for (int e = threadIdx.x; e < o * v; e += 256) {
+            const int ii = e / v, a = e - ii * v;
+            if ((ii & 1) != (a & 1)) m = fmax(m, fabs(T1[e]));
+        }
     sh[threadIdx.x] = m;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) partial[ij] = sh[0];
 }
 
 __global__ void dot_kernel(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
@@ -372,6 +374,7 @@ struct CCSD {
     int prec;
     // constant blocks of <pq||rs>
     double *Voooo, *Vooov, *Voovo, *Voovv, *Vovoo, *Vovvo, *Vovvv, *Vvovv;
+    double* VaeK;   // VaeK[(a e)][(m f)] = <am||fe>: the Fae term as one matrix-vector product
     // packed over antisymmetric pairs: <ab||ef> [(a<b)][(e<f)], <am||ef> [a][m][(e<f)]
     double *VvvvvP, *VvovvP;
     int64_t Po = 0, Pv = 0;            // o(o-1)/2, v(v-1)/2
@@ -393,18 +396,20 @@ struct CCSD {
     bool spin = false;
     static constexpr double SPIN_FORBIDDEN_TOL = 1e-12;
     int spin_forbidden_max(const double* T1, const double* T2, double* out) {
-        const int nb = 256;
-        hipLaunchKernelGGL(spin_forbidden_kernel, dim3(nb), dim3(256), 0, ctx->stream, T1, T2, o, v, partial);
+        const int64_t nb = o * o;
+        hipLaunchKernelGGL(spin_forbidden_kernel, dim3((unsigned)nb), dim3(256), 0, ctx->stream, T1, T2, (int)o, (int)v,
+                           spinmax);
         CORR_LAUNCHED(ctx, "spin_forbidden_kernel");
-        double h[256];
-        CORR_HIP(ctx, hipMemcpyAsync(h, partial, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<double> h((size_t)nb);
+        CORR_HIP(ctx, hipMemcpyAsync(h.data(), spinmax, sizeof(double) * nb, hipMemcpyDeviceToHost, ctx->stream));
         CORR_HIP(ctx, hipStreamSynchronize(ctx->stream));
         double m = 0.0;
-        for (int k = 0; k < nb; ++k) m = std::max(m, h[k]);
+        for (double x : h) m = (x > m || x != x) ? x : m;   // a NaN is kept
         *out = m;
         return CORR_OK;
     }
     double *PmenfS = nullptr, *ZpS = nullptr, *W2S = nullptr, *T2S = nullptr, *WpS = nullptr, *ZrS = nullptr;
+    double* spinmax = nullptr;   // o^2 block maxima of spin_forbidden_kernel
 
     int spin_gather(double* dst, const double* src, int64_t s0, int64_t s1, int64_t s2, int64_t s3) {
         const int64_t ov = o * v;
@@ -546,6 +551,8 @@ struct CCSD {
         CORR_TRY(perm(Vovvo, at(0, ob, ob, 0), {o, v, v, o}, {s0, s1, s2, 1}));
         CORR_TRY(perm(Vovvv, at(0, ob, ob, ob), {o, v, v, v}, {s0, s1, s2, 1}));
         CORR_TRY(perm(Vvovv, at(ob, 0, ob, ob), {v, o, v, v}, {s0, s1, s2, 1}));
+        CORR_TRY(alloc(&VaeK, "cc_VaeK", OV3));
+        CORR_TRY(perm(VaeK, Vvovv, {v, v, o, v}, {o * v * v, 1, v * v, v}));
         if (Pv > 0) {
             hipLaunchKernelGGL(pack_vvvv_kernel, dim3((unsigned)cdiv(Pv, 256), (unsigned)std::min<int64_t>(Pv, 65535)),
                                dim3(256), 0, ctx->stream, V, n, o, pv_a, pv_b, Pv, VvvvvP);
@@ -604,6 +611,8 @@ struct CCSD {
             CORR_TRY(alloc(&T2S, "cc_T2S", ov * ov));
             CORR_TRY(alloc(&WpS, "cc_WpS", ov * ov));
             CORR_TRY(alloc(&ZrS, "cc_ZrS", ov * ov));
+            CORR_TRY(alloc(&spinmax, "cc_spinmax", o * o));
+            CORR_REQUIRE(ctx, o * o < ((int64_t)1 << 31) && v * v < ((int64_t)1 << 31), "ccsd: too many orbitals");
             // the products never write the blocks that vanish by spin: zero them once
             CORR_HIP(ctx, hipMemsetAsync(W2S, 0, sizeof(double) * ov * ov, ctx->stream));
             CORR_HIP(ctx, hipMemsetAsync(ZrS, 0, sizeof(double) * ov * ov, ctx->stream));
@@ -638,8 +647,9 @@ struct CCSD {
         // ---- Fae, Fmi, Fme
         hipLaunchKernelGGL(f_init_kernel, dim3(grid_for(v2 + o2)), dim3(256), 0, ctx->stream, f, T1, o, v, Fae, Fmi);
         CORR_LAUNCHED(ctx, "f_init_kernel");
-        // Fae[a][e] += sum_mf t_mf <ma||fe> = - sum_(mf) t_(mf) <am||fe>
-        CORR_TRY(mm(0, 0, 1, v, ov, -1.0, T1, ov, 0, Vvovv, v, ov * v, 1.0, Fae, v, v, v));
+        // Fae[a][e] += sum_mf t_mf <ma||fe> = - sum_(mf) t_(mf) <am||fe> (one matrix-vector
+        // product over the rows (a e) of VaeK)
+        CORR_TRY(mm(0, 0, v * v, 1, ov, -1.0, VaeK, ov, 0, T1, 1, 0, 1.0, Fae, 1, 0));
         // Fae[a][e] -= 1/2 sum_(mn f) tau~[mn a f] <mn||ef>
         CORR_TRY(perm(tauS_a, tau_s, {v, o, o, v}, {v, o * v2, v2, 1}));
         CORR_TRY(mm(0, 1, v, v, o2 * v, -0.5, tauS_a, o2 * v, 0, PeOOf, o2 * v, 0, 1.0, Fae, v, 0));

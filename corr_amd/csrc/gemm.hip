@@ -485,6 +485,66 @@ static int dispatch(corr_ctx* ctx, int transa, int transb, int f64, GemmArgs& a,
     }
 }
 
+// Matrix-vector products (M = 1 or N = 1; memory-bound, FP64 on the vector
+// units whatever prec asks).  Output t of batch b: y[t] = alpha sum_k
+// W[t st + k sk] x[k sx] + beta y[t], with W, x, y offset by the batch strides.
+// sk == 1: one wave per output, lanes stride k, fixed xor-shuffle sum; else
+// one thread per output, k in order.  Both deterministic.
+struct GemvArgs {
+    int64_t T, K;
+    const double* W; int64_t st, sk, sW;
+    const double* x; int64_t sx, sX;
+    double* y; int64_t sy, sY;
+    double alpha, beta;
+};
+__global__ __launch_bounds__(256) void gemv_rows_kernel(GemvArgs g) {
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6), b = blockIdx.y;
+    if (t >= g.T) return;
+    const double* W = g.W + b * g.sW + t * g.st;
+    const double* x = g.x + b * g.sX;
+    double s = 0.0;
+    for (int64_t k = lane; k < g.K; k += 64) s += W[k] * x[k * g.sx];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) {
+        double* y = g.y + b * g.sY + t * g.sy;
+        *y = g.alpha * s + (g.beta != 0.0 ? g.beta * *y : 0.0);
+    }
+}
+__global__ __launch_bounds__(256) void gemv_cols_kernel(GemvArgs g) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x, b = blockIdx.y;
+    if (t >= g.T) return;
+    const double* W = g.W + b * g.sW + t * g.st;
+    const double* x = g.x + b * g.sX;
+    double s = 0.0;
+    for (int64_t k = 0; k < g.K; ++k) s += W[k * g.sk] * x[k * g.sx];
+    double* y = g.y + b * g.sY + t * g.sy;
+    *y = g.alpha * s + (g.beta != 0.0 ? g.beta * *y : 0.0);
+}
+
+static int gemv(corr_ctx* ctx, const GemvArgs& a, int64_t batch) {
+    for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+        const int64_t nb = std::min<int64_t>(65535, batch - b0);
+        GemvArgs s = a;
+        s.W += b0 * a.sW;
+        s.x += b0 * a.sX;
+        s.y += b0 * a.sY;
+        if (a.sk == 1) {
+            CORR_REQUIRE(ctx, cdiv(a.T, 4) < ((int64_t)1 << 31), "gemv: too many rows");
+            hipLaunchKernelGGL(gemv_rows_kernel, dim3((unsigned)cdiv(a.T, 4), (unsigned)nb), dim3(256), 0, ctx->stream,
+                               s);
+            CORR_LAUNCHED(ctx, "gemv_rows_kernel");
+        } else {
+            CORR_REQUIRE(ctx, cdiv(a.T, 256) < ((int64_t)1 << 31), "gemv: too many columns");
+            hipLaunchKernelGGL(gemv_cols_kernel, dim3((unsigned)cdiv(a.T, 256), (unsigned)nb), dim3(256), 0,
+                               ctx->stream, s);
+            CORR_LAUNCHED(ctx, "gemv_cols_kernel");
+        }
+    }
+    return CORR_OK;
+}
+
 // FP64 operands whose slab rows, batch strides and K-chunk offsets all start
 // on 16-byte boundaries can be staged by global_load_lds_dwordx4.
 static int dma_ok(int f64, const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB) {
@@ -506,6 +566,17 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
     CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
                  "gemm: unknown precision");
     const int f64 = (prec == CORR_PREC_FP64) ? 1 : 0;
+    if ((N == 1 || M == 1) && !a_lower && K > 0) {
+        GemvArgs g;
+        if (N == 1) {   // y[m] = sum_k op(A)(m, k) op(B)(k, 0), y at C[m ldc]
+            g = GemvArgs{M, K, A, transa ? 1 : lda, transa ? lda : 1, sA, B, transb ? 1 : ldb, sB, C, ldc, sC,
+                         alpha, beta};
+        } else {        // y[n] = sum_k op(A)(0, k) op(B)(k, n), y at C[n]
+            g = GemvArgs{N, K, B, transb ? ldb : 1, transb ? 1 : ldb, sB, A, transa ? lda : 1, sA, C, 1, sC,
+                         alpha, beta};
+        }
+        return gemv(ctx, g, batch);
+    }
     // Split K when the output has fewer tiles than the 256 CUs and K is deep
     // (matrix-vector products included): chunks of depth kc >= 512 write
     // partial products to a workspace, then a fixed-order sum applies alpha
