@@ -19,7 +19,9 @@ struct corr_ctx {
     size_t scratch_limit = 0;          // bytes of cached scratch allowed (0: no limit)
     void* blas_handle = nullptr;   // rocBLAS handle used by the eigen metric
     hipStream_t aux_stream = nullptr;   // second stream (default priority) for overlapped work
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t part_stream = nullptr;  // stream on the CUs that aux_stream does not use
+    bool aux_masked = false;            // aux_stream and part_stream split the CUs
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_part = nullptr;
     hipEvent_t ev_switch = nullptr;     // orders a new stream after the previous one
     bool timing = false;
     struct Timed { std::string name; hipEvent_t start, stop; };

@@ -101,8 +101,13 @@ int corr_ctx_destroy(corr_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->aux_stream);
         (void)hipStreamDestroy(ctx->aux_stream);
     }
+    if (ctx->part_stream) {
+        (void)hipStreamSynchronize(ctx->part_stream);
+        (void)hipStreamDestroy(ctx->part_stream);
+    }
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_part) (void)hipEventDestroy(ctx->ev_part);
     if (ctx->ev_switch) (void)hipEventDestroy(ctx->ev_switch);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;

@@ -200,33 +200,33 @@ struct DmaOpnd {
     }
 };
 
-// WM x WN waves (8, 10 or 12; two or more per SIMD); each wave owns 64 x 32 of C
-// (4 x 2 MFMA 16x16 tiles).  BM = 64 WM, BN = 32 WN.
-template <int TA, int TB, typename T, int WM, int WN>
+// WM x WN waves (8, 10 or 12; two or more per SIMD); each wave owns BI x BJ
+// MFMA 16x16 tiles of C (16 BI x 16 BJ).  BM = 16 BI WM, BN = 16 BJ WN.
+template <int TA, int TB, typename T, int WM, int WN, int BI = 4, int BJ = 2>
 struct GemmTile {
     static constexpr int NT = 64 * WM * WN;
-    static constexpr int BM = 64 * WM, BN = 32 * WN, BK = GemmSlab<T>::BK, KV = GemmSlab<T>::KV;
+    static constexpr int BM = 16 * BI * WM, BN = 16 * BJ * WN, BK = GemmSlab<T>::BK, KV = GemmSlab<T>::KV;
     typedef Opnd<BM, TA == 0, NT, T> OA;   // op(A)(m,k): TA==0 -> A[m*lda+k]; TA==1 -> A[k*lda+m]
     typedef Opnd<BN, TB == 1, NT, T> OB;   // op(B)(k,n): TB==0 -> B[k*ldb+n]; TB==1 -> B[n*ldb+k]
     typedef typename Mfma<T>::acc_t acc_t;
 
     // MFMAs of one staged slab.
     __device__ static inline void compute(const T* As, const T* Bs, int wm, int wn, int lg, int lc,
-                                          acc_t (&acc)[4][2]) {
+                                          acc_t (&acc)[BI][BJ]) {
         constexpr int PER = OA::PER;
 #pragma unroll
         for (int h = 0; h < KV / PER; ++h) {
-            T fa[4][PER], fb[2][PER];
+            T fa[BI][PER], fb[BJ][PER];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) OA::frag(As, lg, h, wm * 64 + i * 16 + lc, fa[i]);
+            for (int i = 0; i < BI; ++i) OA::frag(As, lg, h, wm * 16 * BI + i * 16 + lc, fa[i]);
 #pragma unroll
-            for (int j = 0; j < 2; ++j) OB::frag(Bs, lg, h, wn * 32 + j * 16 + lc, fb[j]);
+            for (int j = 0; j < BJ; ++j) OB::frag(Bs, lg, h, wn * 16 * BJ + j * 16 + lc, fb[j]);
 #pragma unroll
             for (int t = 0; t < PER; ++t)
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < BI; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(fa[i][t], fb[j][t], acc[i][j]);
+                    for (int j = 0; j < BJ; ++j) acc[i][j] = Mfma<T>::mma(fa[i][t], fb[j][t], acc[i][j]);
         }
     }
 
@@ -234,7 +234,7 @@ struct GemmTile {
     // As1/Bs1): global -> registers -> ds_write, loads two slabs ahead.
     __device__ static inline void regs_loop(const GemmArgs& g, const double* A, const double* B, int64_t m0,
                                             int64_t n0, int64_t kEnd, int64_t kt0, int64_t kTiles, T* As0, T* As1,
-                                            T* Bs0, T* Bs1, int wm, int wn, int lg, int lc, acc_t (&acc)[4][2]) {
+                                            T* Bs0, T* Bs1, int wm, int wn, int lg, int lc, acc_t (&acc)[BI][BJ]) {
         corr_d2 ra[OA::PPT], rb[OB::PPT];
         if (kt0 < kTiles) {
             OA::load(A, g.lda, m0, g.M, kt0 * BK, kEnd, ra);
@@ -263,9 +263,9 @@ struct GemmTile {
     }
 };
 
-template <int TA, int TB, typename T, int WM, int WN, int EPI>
+template <int TA, int TB, typename T, int WM, int WN, int EPI, int BI = 4, int BJ = 2>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
-    typedef GemmTile<TA, TB, T, WM, WN> GT;
+    typedef GemmTile<TA, TB, T, WM, WN, BI, BJ> GT;
     constexpr int BM = GT::BM, BN = GT::BN, BK = GT::BK;
     typedef typename GT::OA OA;
     typedef typename GT::OB OB;
@@ -292,11 +292,11 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
     if (g.a_lower) kEnd = (m0 + BM < kEnd) ? (m0 + BM) : kEnd;
     const int64_t kTiles = (kEnd + BK - 1) / BK;
 
-    acc_t acc[4][2];
+    acc_t acc[BI][BJ];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < BI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+        for (int j = 0; j < BJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     if constexpr (std::is_same<T, double>::value) {
         // Three slab buffers [A slab | B slab] (the register path uses the first two).
@@ -361,10 +361,10 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
     // bounded subtraction (no division per element).
     const int64_t x0 = EPI == GEMM_EPI_PLAIN ? 0 : m0 / g.en, r0 = EPI == GEMM_EPI_PLAIN ? 0 : m0 % g.en;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < BI; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int off = wm * 64 + i * 16 + AccLayout<acc_t>::row(lane, r);
+            const int off = wm * 16 * BI + i * 16 + AccLayout<acc_t>::row(lane, r);
             const int64_t row = m0 + off;
             if (row >= g.M) continue;
             int64_t d0 = 0, d1 = -1, ri = 0;
@@ -385,8 +385,8 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
                 }
             }
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int64_t col = n0 + wn * 32 + j * 16 + lc;
+            for (int j = 0; j < BJ; ++j) {
+                const int64_t col = n0 + wn * 16 * BJ + j * 16 + lc;
                 if (col >= g.N) continue;
                 double v = g.alpha * (double)acc[i][j][r];
                 if constexpr (EPI == GEMM_EPI_PLAIN) {
@@ -402,27 +402,40 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
         }
 }
 
+#ifndef CORR_GEMM_FORCE
+#define CORR_GEMM_FORCE (-1)
+#endif
+
 template <int TA, int TB, typename T, int EPI = GEMM_EPI_PLAIN>
 static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
-    // Tile shape with the least weighted padded area of C: 128 x 128, 128 x 192 (N = 192
-    // of the (H2O)8 4-index quarters in one tile, so A is read once), 128 x 160
-    // (o = 160 of (H2O)32) or 256 x 64 (narrow N); ties keep that order.
-    const int BMs[4] = {128, 256, 128, 128}, BNs[4] = {128, 64, 160, 192};
+    // Tile shapes: 0: 128 x 128 (8 waves of 4 x 2 MFMA tiles), 1: 256 x 64 (8 x 4 x 2),
+    // 2: 128 x 160 (10 x 4 x 2), 3: 128 x 192 (12 x 4 x 2), 4: 128 x 160 (8 waves of
+    // 2 x 5), 5: 128 x 256 (8 x 4 x 4), 6: 256 x 128 (8 x 4 x 4).
+    const int BMs[7] = {128, 256, 128, 128, 128, 128, 256}, BNs[7] = {128, 64, 160, 192, 160, 256, 128};
     int cfg = 0;
     int64_t best = -1;
-    // Padded areas are weighted by the measured time per useful FLOP on
-    // MI355X at equal padding (FP64 and FP32, 6080^3 .. 23104^2 x 1600):
+    // The shape with the least weighted padded area of C wins; ties keep the
+    // order below.  Padded areas are weighted by the measured time per useful
+    // FLOP on MI355X at equal padding (FP64 and FP32, 6080^3 .. 23104^2 x 1600):
     // 128 x 192 runs 12 waves, 3 per SIMD (1.00); 128 x 128 (1.06) and
-    // 256 x 64 (1.11) run 2 per SIMD; 128 x 160 runs 10 waves on 4 SIMDs
-    // (3, 3, 2, 2) and idles matrix cores (1.20).  Outputs with fewer than
-    // 512 tiles of 128 x 128 keep unweighted areas: there the tile count
-    // (filling the 256 CUs) matters more than per-tile speed.
+    // 256 x 64 (1.11) run 2 per SIMD; 128 x 160 on 10 waves runs (3, 3, 2, 2)
+    // waves per SIMD and idles matrix cores (1.20).  FP64 only, from the
+    // B-formation GEMMs of (H2O)32: 128 x 160 on 8 waves of 2 x 5 (1.03;
+    // 7.6 ms against 9.5 ms for 128 x 192 at N = o = 160) and 256 x 128 on 8
+    // waves of 4 x 4 (1.01; 6.2 against 6.6 ms for 128 x 128 at
+    // 389120 x 608 x 768); 128 x 256 of 4 x 4 is not faster than 128 x 192
+    // and stays out.  Outputs with fewer than 512 tiles of 128 x 128 keep
+    // unweighted areas: there the tile count (filling the 256 CUs) matters
+    // more than per-tile speed.
     const bool big = cdiv(a.M, 128) * cdiv(a.N, 128) * batch >= 512;
-    const int64_t weight[4] = {big ? 106 : 100, big ? 111 : 100, big ? 120 : 100, 100};
-    for (int c : {0, 3, 2, 1}) {
+    const bool f64 = std::is_same<T, double>::value;
+    const int64_t weight[7] = {big ? 106 : 100, big ? 111 : 100, big ? 120 : 100, 100,
+                               f64 ? (big ? 103 : 100) : 9999, 9999, f64 ? (big ? 101 : 100) : 9999};
+    for (int c : {0, 3, 2, 1, 4, 5, 6}) {
         const int64_t area = cdiv(a.M, BMs[c]) * BMs[c] * (cdiv(a.N, BNs[c]) * BNs[c]) / 100 * weight[c];
         if (best < 0 || area < best) { best = area; cfg = c; }
     }
+    if (CORR_GEMM_FORCE >= 0) cfg = CORR_GEMM_FORCE;
     const int BM = BMs[cfg], BN = BNs[cfg];
     const int nt = cfg == 3 ? 768 : (cfg == 2 ? 640 : 512);
     a.tilesM = cdiv(a.M, BM);
@@ -440,14 +453,15 @@ static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
         s.C += b0 * a.sC;
         dim3 grid((unsigned)tiles, (unsigned)nb);
         KernelTimer timer(ctx, "gemm");
-        if (cfg == 1)
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2, EPI>), grid, dim3(nt), 0, ctx->stream, s);
-        else if (cfg == 3)
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 6, EPI>), grid, dim3(nt), 0, ctx->stream, s);
-        else if (cfg == 2)
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 5, EPI>), grid, dim3(nt), 0, ctx->stream, s);
-        else
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 4, EPI>), grid, dim3(nt), 0, ctx->stream, s);
+        switch (cfg) {
+            case 1: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2, EPI>), grid, dim3(nt), 0, ctx->stream, s); break;
+            case 2: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 5, EPI>), grid, dim3(nt), 0, ctx->stream, s); break;
+            case 3: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 6, EPI>), grid, dim3(nt), 0, ctx->stream, s); break;
+            case 4: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2, EPI, 2, 5>), grid, dim3(nt), 0, ctx->stream, s); break;
+            case 5: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 4, EPI, 4, 4>), grid, dim3(nt), 0, ctx->stream, s); break;
+            case 6: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2, EPI, 4, 4>), grid, dim3(nt), 0, ctx->stream, s); break;
+            default: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 4, EPI>), grid, dim3(nt), 0, ctx->stream, s); break;
+        }
         CORR_LAUNCHED(ctx, "gemm_kernel");
     }
     return CORR_OK;
