@@ -6,6 +6,7 @@ the library is loaded so that both share one HIP runtime.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import threading
 from pathlib import Path
@@ -188,8 +189,20 @@ class Context:
             device = torch.cuda.current_device()
         ctx = cls._per_device.get(device)
         if ctx is None:
+            if not cls._per_device:
+                atexit.register(cls.destroy_all)
             ctx = cls._per_device[device] = cls(device)
         return ctx
+
+    @classmethod
+    def destroy_all(cls) -> None:
+        """Destroys every context (scratch, streams, events) while the HIP
+        runtime is still up; registered with atexit on first use."""
+        for dev, ctx in list(cls._per_device.items()):
+            if ctx.handle:
+                lib().corr_ctx_destroy(ctx.handle)
+                ctx.handle = P()
+            del cls._per_device[dev]
 
     def bind_stream(self) -> None:
         s = torch.cuda.current_stream(self.device).cuda_stream
