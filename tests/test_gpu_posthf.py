@@ -119,6 +119,38 @@ def test_gemm_fp64_staged_interior_tiles(corr, ta, tb):
         assert np.max(np.abs(got - ref)) < 1e-12 * np.sqrt(K) * 4, off
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("shape", ["128x160 of 2x5", "256x128 of 4x4"])
+def test_gemm_fp64_wide_wave_tiles(corr, ta, tb, shape):
+    """Outputs of >= 512 tiles whose padded areas select the FP64 tiles with
+    2 x 5 (N = 150: 128 x 160) and 4 x 4 (N = 7552 = 59 x 128: 256 x 128) MFMA
+    blocks per wave; ragged M, a K remainder below one slab (K = 2 x 16 + 10),
+    batched; 16-byte aligned operands with even leading dimensions (staged
+    interior tiles) and operands one double off alignment (register path),
+    both against numpy."""
+    from corr_amd import _abi
+    from corr_amd._abi import Context, ptr
+    M, N, K, nb = (66554, 150, 42, 1) if shape.startswith("128") else (2554, 7552, 42, 2)
+    _say(f"gemm fp64 {shape} ta={ta} tb={tb} M={M} N={N} K={K} batch={nb}")
+    rng = np.random.default_rng(41 + 2 * ta + tb)
+    ash, bsh = ((K, M) if ta else (M, K)), ((N, K) if tb else (K, N))
+    A = rng.standard_normal((nb,) + ash)
+    B = rng.standard_normal((nb,) + bsh)
+    ref = np.stack([(A[b].T if ta else A[b]) @ (B[b].T if tb else B[b]) for b in range(nb)])
+    ctx = Context.get(0)
+    for off in (0, 1):
+        dA = torch.zeros(A.size + 2, dtype=torch.float64, device="cuda")
+        dB = torch.zeros(B.size + 2, dtype=torch.float64, device="cuda")
+        dA[off:off + A.size] = dev(A.ravel())
+        dB[off:off + B.size] = dev(B.ravel())
+        C = torch.zeros(nb, M, N, dtype=torch.float64, device="cuda")
+        ctx.call("corr_dgemm", ta, tb, M, N, K, 1.0, ptr(dA[off:]), ash[1], A[0].size, ptr(dB[off:]), bsh[1],
+                 B[0].size, 0.0, ptr(C), N, M * N, nb, _abi.PRECISIONS["fp64"])
+        err = float(np.max(np.abs(C.cpu().numpy() - ref)))
+        _say(f"  offset {off}: max |diff| = {err:.2e}")
+        assert err < 1e-12 * np.sqrt(K) * 4, off
+
+
 @pytest.mark.parametrize("entry", ["ri_metric", "ri_bia_from_j"])
 def test_cholesky_metric_not_positive_definite(corr, entry):
     """A metric with a negative eigenvalue is refused with CORR_ENOTPD (the
