@@ -507,17 +507,20 @@ def run_suite(dev, seed: int, rows=(), main=None) -> dict:
         Cm, h, ion = r["C"], r["H"], r["mol"].nuclear_repulsion()
         # work of this implementation: the pp ladder over packed antisymmetric pairs
         # 2 Po Pv^2 (Po = o(o-1)/2, Pv = v(v-1)/2), ring 2 x 2 (ov)^3, Wmnij tau term and hh
-        # ladder 2 x 2 o^4 v^2; the reference's intermediates: 4 o^2 v^4 + 4 o^3 v^3 + 4 o^4 v^2
-        # (unpacked pp ladder, and Wabef's tau term, carried here by Wmnij)
+        # ladder over packed pairs 2 x 2 Po^2 Pv; the reference's intermediates:
+        # 4 o^2 v^4 + 4 o^3 v^3 + 4 o^4 v^2 (unpacked ladders, and Wabef's tau term, carried
+        # here by Wmnij)
         Po, Pv = o * (o - 1) // 2, v * (v - 1) // 2
-        W = 2.0 * Po * Pv * Pv + 4.0 * o ** 3 * v ** 3 + 4.0 * o ** 4 * v * v
+        W = 2.0 * Po * Pv * Pv + 4.0 * o ** 3 * v ** 3 + 4.0 * Po * Po * Pv
         W_ref = 4.0 * o * o * v ** 4 + 4.0 * o ** 3 * v ** 3 + 4.0 * o ** 4 * v * v
         # executed with spin-symmetric amplitudes: each ring product as one 2x2-class and two
-        # cross-class GEMMs (20 B4^3 each, B4 = o v / 4), the pp ladder on its three pair
-        # spin classes (alpha alpha, beta beta: h(h-1)/2 pairs; mixed: h^2 pairs)
+        # cross-class GEMMs (20 B4^3 each, B4 = o v / 4), the pp ladder, Wmnij's tau term and
+        # the hh ladder on their three pair spin classes (alpha alpha, beta beta: h(h-1)/2
+        # pairs; mixed: h^2 pairs)
         oh, vh, B4 = o // 2, v // 2, o * v // 4
         po_c, pv_c = [oh * (oh - 1) // 2] * 2 + [oh * oh], [vh * (vh - 1) // 2] * 2 + [vh * vh]
-        W_exec = 2 * 20.0 * B4 ** 3 + sum(2.0 * a * b * b for a, b in zip(po_c, pv_c)) + 4.0 * o ** 4 * v * v
+        W_exec = (2 * 20.0 * B4 ** 3 + sum(2.0 * a * b * b for a, b in zip(po_c, pv_c))
+                  + sum(4.0 * a * a * b for a, b in zip(po_c, pv_c)))
         res = {"shape": "(H2O)4 N=96 spin-orbital o=40 v=152, real integrals",
                "e_rhf": r["scf"]["e_total"], "W_CC_executed_TFLOP_per_iter": round(W_exec / 1e12, 3),
                "W_CC_dense_spin_orbital_TFLOP": round(W / 1e12, 3),

@@ -131,6 +131,15 @@ __global__ void pack_tau_kernel(const double* __restrict__ tau, int64_t o, int64
     }
 }
 
+// WP[p][q] = W[m][n][i][j], (m<n) = pair p and (i<j) = pair q of the occupied.
+__global__ void pack_oooo_kernel(const double* __restrict__ W, int64_t o, const int32_t* __restrict__ pi,
+                                 const int32_t* __restrict__ pj, int64_t Po, double* __restrict__ out) {
+    PAIR_LOOP(Po, Po) {
+        const int64_t m = pi[p], n = pj[p], i = pi[q], j = pj[q];
+        out[p * Po + q] = W[((m * o + n) * o + i) * o + j];
+    }
+}
+
 // WP[p][q] = VP[p][q] - XP[a][b][q] + XP[b][a][q], (a<b) = pair p:
 // <ab||ef> - P(ab) t_mb <am||ef> with XP[a][b][q] = sum_m t_mb <am||ef>.
 __global__ void wabef_packed_kernel(const double* __restrict__ VP, const double* __restrict__ XP, int64_t v,
@@ -384,6 +393,7 @@ struct CCSD {
     double *fo;   // f (n x n)
     // per-iteration work
     double *tau_s, *tau, *tauS_a, *Fae, *Faep, *Fmi, *Fmip, *Fme, *Yk, *Wmnij, *WabefP, *XabP, *tauP, *Ypp, *Wmbej;
+    double *VoovvP, *WmnijP;   // <mn||ef> over (m<n) x (e<f); Wmnij over (m<n) x (i<j)
     double *Zp, *W2, *Wp, *T2p, *T2a, *X1, *X2, *Zr, *U, *Q, *Tmp1;
     double* partial;
     // Spin-blocked ring terms (V from ints_so_kernel: spin = index % 2).  Exact
@@ -588,6 +598,13 @@ struct CCSD {
         CORR_TRY(alloc(&WabefP, "cc_WabefP", std::max<int64_t>(Pv * Pv, 1)));
         CORR_TRY(alloc(&XabP, "cc_XabP", std::max<int64_t>(v * v * Pv, 1)));
         CORR_TRY(alloc(&tauP, "cc_tauP", std::max<int64_t>(Po * Pv, 1)));
+        CORR_TRY(alloc(&VoovvP, "cc_VoovvP", std::max<int64_t>(Po * Pv, 1)));
+        CORR_TRY(alloc(&WmnijP, "cc_WmnijP", std::max<int64_t>(Po * Po, 1)));
+        if (Po > 0 && Pv > 0) {
+            hipLaunchKernelGGL(pack_tau_kernel, dim3((unsigned)cdiv(Pv, 256), (unsigned)std::min<int64_t>(Po, 65535)),
+                               dim3(256), 0, ctx->stream, Voovv, o, v, po_i, po_j, pv_a, pv_b, Po, Pv, VoovvP);
+            CORR_LAUNCHED(ctx, "pack_tau_kernel");
+        }
         CORR_TRY(alloc(&Ypp, "cc_Ypp", std::max<int64_t>(Po * Pv, 1)));
         CORR_TRY(alloc(&Wmbej, "cc_Wmbej", O2V2));
         CORR_TRY(alloc(&Zp, "cc_Zp", O2V2));
@@ -674,7 +691,29 @@ struct CCSD {
         CORR_TRY(perm(Wmnij, Voooo, {o, o, o, o}, {o * o2, o2, o, 1}));
         CORR_TRY(mm(0, 1, o * o2, o, v, 1.0, Vooov, v, 0, T1, v, 0, 1.0, Wmnij, o, 0));
         CORR_TRY(mm(0, 1, o, o, v, -1.0, T1, v, 0, Vooov, v, o * v, 1.0, Wmnij, o, o2, o2));
-        CORR_TRY(mm(0, 1, o2, o2, v2, 0.5, Voovv, v2, 0, tau, v2, 0, 1.0, Wmnij, o2, 0));
+        // ... and its tau term on the packed pairs: W_mnij is antisymmetric in
+        // (mn) and (ij), so WmnijP[(m<n)][(i<j)] += sum_{e<f} <mn||ef> tau_ijef
+        // (= 1/2 sum_ef), 1/8 of the unpacked product; with spin-symmetric
+        // amplitudes only the diagonal pair-class blocks are nonzero.
+        if (Po > 0 && Pv > 0) {
+            const dim3 gpp((unsigned)cdiv(Pv, 256), (unsigned)std::min<int64_t>(Po, 65535));
+            hipLaunchKernelGGL(pack_tau_kernel, gpp, dim3(256), 0, ctx->stream, tau, o, v, po_i, po_j, pv_a, pv_b, Po,
+                               Pv, tauP);
+            CORR_LAUNCHED(ctx, "pack_tau_kernel");
+            hipLaunchKernelGGL(pack_oooo_kernel, dim3((unsigned)cdiv(Po, 256), (unsigned)std::min<int64_t>(Po, 65535)),
+                               dim3(256), 0, ctx->stream, Wmnij, o, po_i, po_j, Po, WmnijP);
+            CORR_LAUNCHED(ctx, "pack_oooo_kernel");
+            if (spin) {
+                for (int c = 0; c < 3; ++c) {
+                    const int64_t r0 = po_cls[c], nr = po_cls[c + 1] - r0, k0 = pv_cls[c], nk = pv_cls[c + 1] - k0;
+                    if (nr > 0 && nk > 0)
+                        CORR_TRY(mm(0, 1, nr, nr, nk, 1.0, VoovvP + r0 * Pv + k0, Pv, 0, tauP + r0 * Pv + k0, Pv, 0,
+                                    1.0, WmnijP + r0 * Po + r0, Po, 0));
+                }
+            } else {
+                CORR_TRY(mm(0, 1, Po, Po, Pv, 1.0, VoovvP, Pv, 0, tauP, Pv, 0, 1.0, WmnijP, Po, 0));
+            }
+        }
         // ---- Wabef = <ab||ef> - P(ab) t_mb <am||ef>  (its tau term is in Wmnij),
         // antisymmetric in (ab) and (ef): kept only for a < b, e < f.
         // XabP[a][b][(e<f)] = sum_m t_mb <am||ef> (one GEMM per a), then one pass
@@ -731,27 +770,28 @@ struct CCSD {
         CORR_TRY(mm(1, 0, o, v2, o, 1.0, Fmip, o, 0, T2, v2, o * v2, 0.0, X2, v2, o * v2, o));
         CORR_TRY(perm(T2n, X2, {o, o, v, v}, {o * v2, v2, v, 1}, -1.0, 1.0));
         CORR_TRY(perm(T2n, X2, {o, o, v, v}, {v2, o * v2, v, 1}, 1.0, 1.0));
-        // hh ladder: 1/2 tau_mnab W_mnij
-        CORR_TRY(mm(1, 0, o2, v2, o2, 0.5, Wmnij, o2, 0, tau, v2, 0, 1.0, T2n, v2, 0));
-        // pp ladder: 1/2 sum_ef tau_ijef W_abef = sum_{e<f} tau_ijef W_abef, computed
-        // for i < j and a < b only (the product is symmetric under e <-> f, T2
-        // antisymmetric in (ij) and (ab)): one GEMM over the packed pairs, 1/8 of
-        // the unpacked work, then each result goes to its four T2 elements.
+        // pp ladder: 1/2 sum_ef tau_ijef W_abef = sum_{e<f} tau_ijef W_abef, and hh
+        // ladder: 1/2 sum_mn tau_mnab W_mnij = sum_{m<n} W_mnij tau_mnab, computed
+        // for i < j and a < b only (the products are symmetric under e <-> f and
+        // m <-> n, T2 antisymmetric in (ij) and (ab)): GEMMs over the packed pairs
+        // (tauP from the Wmnij step), 1/8 of the unpacked work, then each result
+        // goes to its four T2 elements.
         if (Po > 0 && Pv > 0) {
             const dim3 gp((unsigned)cdiv(Pv, 256), (unsigned)std::min<int64_t>(Po, 65535));
-            hipLaunchKernelGGL(pack_tau_kernel, gp, dim3(256), 0, ctx->stream, tau, o, v, po_i, po_j, pv_a, pv_b, Po, Pv,
-                               tauP);
-            CORR_LAUNCHED(ctx, "pack_tau_kernel");
             if (spin) {
                 // diagonal class blocks only (the others stay zero from setup)
                 for (int c = 0; c < 3; ++c) {
                     const int64_t r0 = po_cls[c], nr = po_cls[c + 1] - r0, k0 = pv_cls[c], nk = pv_cls[c + 1] - k0;
-                    if (nr > 0 && nk > 0)
+                    if (nr > 0 && nk > 0) {
                         CORR_TRY(mm(0, 1, nr, nk, nk, 1.0, tauP + r0 * Pv + k0, Pv, 0, WabefP + k0 * Pv + k0, Pv, 0,
                                     0.0, Ypp + r0 * Pv + k0, Pv, 0));
+                        CORR_TRY(mm(1, 0, nr, nk, nr, 1.0, WmnijP + r0 * Po + r0, Po, 0, tauP + r0 * Pv + k0, Pv, 0,
+                                    1.0, Ypp + r0 * Pv + k0, Pv, 0));
+                    }
                 }
             } else {
                 CORR_TRY(mm(0, 1, Po, Pv, Pv, 1.0, tauP, Pv, 0, WabefP, Pv, 0, 0.0, Ypp, Pv, 0));
+                CORR_TRY(mm(1, 0, Po, Pv, Po, 1.0, WmnijP, Po, 0, tauP, Pv, 0, 1.0, Ypp, Pv, 0));
             }
             hipLaunchKernelGGL(scatter_pp_kernel, gp, dim3(256), 0, ctx->stream, Ypp, o, v, po_i, po_j, pv_a, pv_b, Po,
                                Pv, T2n);
