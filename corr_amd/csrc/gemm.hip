@@ -410,8 +410,8 @@ template <int TA, int TB, typename T, int EPI = GEMM_EPI_PLAIN>
 static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
     // Tile shapes: 0: 128 x 128 (8 waves of 4 x 2 MFMA tiles), 1: 256 x 64 (8 x 4 x 2),
     // 2: 128 x 160 (10 x 4 x 2), 3: 128 x 192 (12 x 4 x 2), 4: 128 x 160 (8 waves of
-    // 2 x 5), 5: 128 x 256 (8 x 4 x 4), 6: 256 x 128 (8 x 4 x 4).
-    const int BMs[7] = {128, 256, 128, 128, 128, 128, 256}, BNs[7] = {128, 64, 160, 192, 160, 256, 128};
+    // 2 x 5), 5: 128 x 256 (8 x 4 x 4), 6: 256 x 128 (8 x 4 x 4), 7: 128 x 192 (8 x 2 x 6).
+    const int BMs[8] = {128, 256, 128, 128, 128, 128, 256, 128}, BNs[8] = {128, 64, 160, 192, 160, 256, 128, 192};
     int cfg = 0;
     int64_t best = -1;
     // The shape with the least weighted padded area of C wins; ties keep the
@@ -424,14 +424,17 @@ static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
     // 7.6 ms against 9.5 ms for 128 x 192 at N = o = 160) and 256 x 128 on 8
     // waves of 4 x 4 (1.01; 6.2 against 6.6 ms for 128 x 128 at
     // 389120 x 608 x 768); 128 x 256 of 4 x 4 is not faster than 128 x 192
-    // and stays out.  Outputs with fewer than 512 tiles of 128 x 128 keep
+    // and stays out.  For a single column of 192-wide tiles (FP64), 128 x 192
+    // on 8 waves of 2 x 6 (0.96; the (H2O)8 4-index quarters, N^3 x 192 x 192:
+    // MOtrans 29.3 against 30.4 ms).  Outputs with fewer than 512 tiles of 128 x 128 keep
     // unweighted areas: there the tile count (filling the 256 CUs) matters
     // more than per-tile speed.
     const bool big = cdiv(a.M, 128) * cdiv(a.N, 128) * batch >= 512;
     const bool f64 = std::is_same<T, double>::value;
-    const int64_t weight[7] = {big ? 106 : 100, big ? 111 : 100, big ? 120 : 100, 100,
-                               f64 ? (big ? 103 : 100) : 9999, 9999, f64 ? (big ? 101 : 100) : 9999};
-    for (int c : {0, 3, 2, 1, 4, 5, 6}) {
+    const int64_t weight[8] = {big ? 106 : 100, big ? 111 : 100, big ? 120 : 100, 100,
+                               f64 ? (big ? 103 : 100) : 9999, 9999, f64 ? (big ? 101 : 100) : 9999,
+                               f64 && big && a.N <= 192 ? 96 : 9999};
+    for (int c : {0, 3, 2, 1, 4, 5, 6, 7}) {
         const int64_t area = cdiv(a.M, BMs[c]) * BMs[c] * (cdiv(a.N, BNs[c]) * BNs[c]) / 100 * weight[c];
         if (best < 0 || area < best) { best = area; cfg = c; }
     }
@@ -460,6 +463,7 @@ static int launch_t(corr_ctx* ctx, GemmArgs& a, int64_t batch) {
             case 4: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2, EPI, 2, 5>), grid, dim3(nt), 0, ctx->stream, s); break;
             case 5: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 4, EPI, 4, 4>), grid, dim3(nt), 0, ctx->stream, s); break;
             case 6: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2, EPI, 4, 4>), grid, dim3(nt), 0, ctx->stream, s); break;
+            case 7: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 4, 2, EPI, 2, 6>), grid, dim3(nt), 0, ctx->stream, s); break;
             default: hipLaunchKernelGGL((gemm_kernel<TA, TB, T, 2, 4, EPI>), grid, dim3(nt), 0, ctx->stream, s); break;
         }
         CORR_LAUNCHED(ctx, "gemm_kernel");

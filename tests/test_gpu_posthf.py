@@ -120,17 +120,18 @@ def test_gemm_fp64_staged_interior_tiles(corr, ta, tb):
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("shape", ["128x160 of 2x5", "256x128 of 4x4"])
+@pytest.mark.parametrize("shape", ["128x160 of 2x5", "256x128 of 4x4", "128x192 of 2x6"])
 def test_gemm_fp64_wide_wave_tiles(corr, ta, tb, shape):
     """Outputs of >= 512 tiles whose padded areas select the FP64 tiles with
-    2 x 5 (N = 150: 128 x 160) and 4 x 4 (N = 7552 = 59 x 128: 256 x 128) MFMA
-    blocks per wave; ragged M, a K remainder below one slab (K = 2 x 16 + 10),
+    2 x 5 (N = 150: 128 x 160), 4 x 4 (N = 7552 = 59 x 128: 256 x 128) and
+    2 x 6 (N = 170, one column of 128 x 192) MFMA blocks per wave; ragged M, a K remainder below one slab (K = 2 x 16 + 10),
     batched; 16-byte aligned operands with even leading dimensions (staged
     interior tiles) and operands one double off alignment (register path),
     both against numpy."""
     from corr_amd import _abi
     from corr_amd._abi import Context, ptr
-    M, N, K, nb = (66554, 150, 42, 1) if shape.startswith("128") else (2554, 7552, 42, 2)
+    M, N, K, nb = {"128x160 of 2x5": (66554, 150, 42, 1), "256x128 of 4x4": (2554, 7552, 42, 2),
+                   "128x192 of 2x6": (66554, 170, 42, 1)}[shape]
     _say(f"gemm fp64 {shape} ta={ta} tb={tb} M={M} N={N} K={K} batch={nb}")
     rng = np.random.default_rng(41 + 2 * ta + tb)
     ash, bsh = ((K, M) if ta else (M, K)), ((N, K) if tb else (K, N))
