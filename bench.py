@@ -8,16 +8,20 @@ RI-MP2 pair-block energy.  Sizes are those of
 (H2O)32: N = 768 basis functions, o = 160 occupied, v = 608 virtual,
 naux = 2432 auxiliary functions.  Three paths are timed, K steps each:
 
-  mixed: metric and B formation FP64 (FP64 MFMA), energy BF16x3 MFMA with
-         FP32 accumulation and FP64 denominators/reduction
+  mixed: metric factorisation and AO->MO GEMMs FP64 (FP64 MFMA), the
+         Cholesky metric product as an FP64 GEMM emulated on the int8 matrix
+         cores (5 int8 slices per operand, exact int32 sums), energy BF16x3
+         MFMA with FP32 accumulation and FP64 denominators/reduction
   fp64:  everything FP64 (FP64 MFMA)
   fp32:  B-formation GEMMs and energy in FP32 MFMA (reported beside, with
          its error against fp64)
 
-B formation stays in FP64 on the mixed path: the AO->MO sums cancel (3e-6
-rms relative error in (Q|ia) from FP32 GEMMs) and the Cholesky metric
-(cond(J) = 1.1e6) amplifies that to 8e-5 in B, an energy error of about
-0.5 uEh per FP32 stage with a sign set by the summation order.
+FP32 has no place in the mixed path's B formation: the AO->MO sums cancel
+(3e-6 rms relative error in (Q|ia) from FP32 GEMMs) and the Cholesky metric
+(cond(J) = 1.1e6) amplifies that to ~1e-4 in B, an energy error of about
+0.3 uEh per FP32 stage with a sign set by the summation order; an FP32
+metric product alone costs 0.28 uEh.  The int8-emulated metric product keeps
+B within 3e-8 of FP64 (0.002 uEh).
 
 value = algorithmic TFLOP/s of the mixed path, (W_B_chol + W_E) / t_step, with
 W_E = 2 [o(o+1)/2] v^2 naux and W_B_chol = 2 naux o N^2 + 2 naux o v N + naux^2 o v
@@ -207,7 +211,7 @@ def main() -> int:
         return corr_amd.rimp2_energy(B, eo, ev, prec_e)
 
     results = {}
-    for name, pg, pe in (("fp64", "fp64", "fp64"), ("mixed", "fp64", "bf16x3"), ("fp32", "fp32", "fp32")):
+    for name, pg, pe in (("fp64", "fp64", "fp64"), ("mixed", "bf16x3", "bf16x3"), ("fp32", "fp32", "fp32")):
         for _ in range(args.warmup):
             step(pg, pe)
         torch.cuda.synchronize()
@@ -244,7 +248,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3 energy (fp32 accumulation; fp64 B formation, denominators and reduction)",
+        "dtype": "bf16x3 energy (fp32 accumulation; fp64 AO->MO GEMMs, int8-emulated fp64 metric product, "
+                 "fp64 denominators and reduction)",
         "data": ("real chemistry: seeded water cluster, def2-SVP/def2-SVP-RI integrals and RI-JK RHF computed on the "
                  "GPU by this library before the timed region" if args.data == "real" else
                  "synthetic (seeded tensors of the (H2O)32 shapes; no integral files)"),
@@ -462,7 +467,8 @@ def run_suite(dev, seed: int, rows=(), main=None) -> dict:
         _, WB, WBc = work(N, o, v, naux)
         res["W_B_cholesky_TFLOP"] = round(WBc / 1e12, 4)
         res["W_B_dense_metric_TFLOP"] = round(WB / 1e12, 4)
-        for prec in ("fp64", "fp32"):
+        # bf16x3: the mixed path's B formation (FP64 AO->MO, int8-emulated FP64 metric product)
+        for prec in ("fp64", "bf16x3", "fp32"):
             t = _timed(lambda: corr_amd.ri_form_bia(C, Pmn, M, o, "cholesky", prec))
             res[f"form_bia_{prec}_ms"] = round(t * 1e3, 2)
             # the triangular metric product is credited at its own (half) work

@@ -21,6 +21,9 @@ METRIC_EIGEN, METRIC_CHOLESKY = 0, 1
 CCSD_REFERENCE, CCSD_DIIS = 0, 1
 
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+# corr_dgemm alone also takes the int8-emulated FP64 GEMM (Ozaki splitting).
+PREC_INT8X5 = 3
+GEMM_PRECISIONS = {**PRECISIONS, "int8x5": PREC_INT8X5}
 METRICS = {"eigen": METRIC_EIGEN, "cholesky": METRIC_CHOLESKY}
 
 _STATUS = {

@@ -581,8 +581,13 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
     CORR_REQUIRE(ctx, transb == 0 || transb == 1, "gemm: transb must be 0 or 1");
     CORR_REQUIRE(ctx, lda >= (transa ? M : K) && ldb >= (transb ? K : N) && ldc >= N,
                  "gemm: leading dimension too small");
-    CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
+    CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3 ||
+                          prec == CORR_PREC_INT8X5,
                  "gemm: unknown precision");
+    if (prec == CORR_PREC_INT8X5) {
+        CORR_REQUIRE(ctx, transa == 0 && transb == 0 && batch == 1, "gemm int8x5: no transposes, batch 1");
+        return gemm_ozaki(ctx, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, a_lower);
+    }
     const int f64 = (prec == CORR_PREC_FP64) ? 1 : 0;
     if ((N == 1 || M == 1) && !a_lower && K > 0) {
         GemvArgs g;

@@ -1,0 +1,344 @@
+// FP64 GEMM emulated on the int8 matrix cores (Ozaki splitting).
+//
+// C = alpha A B + beta C for row-major A (M x K) and B (K x N).  Every row m of
+// A gets a power-of-two scale sig[m] >= max_k |A[m][k]| (127/128 headroom) and
+// every column n of B a scale tau[n]; the scaled values x in (-1, 1) are cut
+// into S = 5 int8 slices by round-to-nearest,
+//     r_0 = x,  q_s = rint(128 r_s),  r_{s+1} = 128 r_s - q_s,
+// so x = sum_s q_s 2^-7(s+1) + r_S 2^-7S with |r_S| <= 1/2 (|q_0| <= 127,
+// |q_s| <= 64 after).  Every product q_s(A) q_t(B) with s + t <= S - 1 is an
+// exact int32 sum on v_mfma_i32_32x32x32_i8 (15 of the 25 slice pairs; the
+// dropped ones and the remainders are below 2^-35 of the row and column
+// maxima), accumulated per diagonal d = s + t, and the epilogue combines the
+// five diagonals in FP64 by Horner's rule in 2^-7 (exact up to the last step)
+// and applies sig[m] tau[n] 2^-14.  Results are bit-reproducible: the integer
+// sums are exact and the FP64 steps run in a fixed order.
+//
+// Non-finite entries (and maxima of 2^1023 and above) make their row's (A) or
+// column's (B) scale NaN, so the whole row or column of C is NaN.  K is
+// limited to 75000 (int32 range of the widest diagonal:
+// K (2 127 64 + 3 64 64) < 2^31).
+//
+// Operands are sliced into the kernel's LDS stage order first (one pass over
+// A, two over B: column maxima, then slices), so each stage copy is a
+// contiguous 1 KiB run per wave: a stage holds 32 k of the S slices of 128 rows
+// of A and of 128 columns of B, [slice][row][32 bytes], the two 16-byte k
+// chunks of a row in swizzled slots (conflict-free ds_read_b128 fragments).
+#include "common.hpp"
+
+#include <cfloat>
+
+namespace corr {
+
+namespace {
+constexpr int OZ_S = 5;                          // slices per operand
+constexpr int OZ_T = 128;                        // rows (columns) of a tile
+constexpr int OZ_KS = 32;                        // k per stage
+constexpr int OZ_SLICE = OZ_T * OZ_KS;           // bytes of one slice of a stage (4 KiB)
+constexpr int OZ_PANEL = OZ_S * OZ_SLICE;        // one operand's stage (20 KiB)
+constexpr int OZ_THREADS = 512;                  // 8 waves, 2 per SIMD
+constexpr int OZ_NC = 2 * OZ_PANEL / 1024 / (OZ_THREADS / 64);   // 1 KiB copies per wave and stage (5)
+constexpr int64_t OZ_KMAX = 75000;
+
+typedef int oz_i4 __attribute__((ext_vector_type(4)));
+typedef int oz_i16 __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void* oz_lds_ptr;
+typedef __attribute__((address_space(1))) const void* oz_gbl_ptr;
+
+// Slot of 16-byte k chunk ch of row r in a 32-byte LDS row (an involution in ch).
+__device__ inline int oz_slot(int row, int ch) { return ch ^ ((row >> 3) & 1); }
+
+// Scale 2^e for a row (column) with maximum magnitude mx: |x| / 2^e < 127.5/128
+// for every |x| <= mx, so the first slice stays within [-127, 127].  1 for an
+// all-zero row, NaN when an entry is not finite.
+__device__ inline double oz_scale(double mx, bool bad) {
+    if (bad) return __builtin_nan("");
+    if (!(mx > 0.0)) return 1.0;
+    int e;
+    const double f = frexp(mx, &e);   // mx = f 2^e, f in [0.5, 1)
+    if (f * 128.0 >= 127.5) ++e;
+    if (e > 1023) return __builtin_nan("");   // 2^e would overflow
+    return ldexp(1.0, e);
+}
+
+// inv_scale = 0 marks a row (column) with a NaN scale: its slices are zero.
+__device__ inline void oz_slices(double x, double inv_scale, int8_t (&q)[OZ_S]) {
+    double r = inv_scale == 0.0 ? 0.0 : x * inv_scale;   // power-of-two scaling: exact
+#pragma unroll
+    for (int s = 0; s < OZ_S; ++s) {
+        r *= 128.0;
+        const double t = rint(r);
+        r -= t;
+        q[s] = (int8_t)(int)t;
+    }
+}
+
+// 1 / scale as an exact power of two (0 for a NaN scale: its slices are zero).
+__device__ inline double oz_inv(double scale) {
+    if (scale != scale) return 0.0;
+    int e;
+    (void)frexp(scale, &e);
+    return ldexp(1.0, 1 - e);
+}
+
+// sig[m] for the rows of A (one wave per row).
+__global__ __launch_bounds__(256) void oz_rowscale_kernel(const double* __restrict__ A, int64_t lda, int64_t M,
+                                                          int64_t K, double* __restrict__ sig) {
+    const int64_t m = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (m >= M) return;
+    double mx = 0.0;
+    bool bad = false;
+    for (int64_t k = lane; k < K; k += 64) {
+        const double a = fabs(A[m * lda + k]);
+        bad |= !(a <= DBL_MAX);
+        mx = a > mx ? a : mx;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+        bad |= __shfl_xor((int)bad, off, 64) != 0;
+    }
+    if (lane == 0) sig[m] = oz_scale(mx, bad);
+}
+
+// Slices of A into stage order: one block per (row tile, stage); thread t takes
+// row t / 2 and k chunk t % 2 (16 values).  Rows past M and k past K are zero.
+__global__ __launch_bounds__(256) void oz_split_a_kernel(const double* __restrict__ A, int64_t lda, int64_t M,
+                                                         int64_t K, int64_t Ks, const double* __restrict__ sig,
+                                                         int8_t* __restrict__ Ap) {
+    const int64_t st = blockIdx.x % Ks, mt = blockIdx.x / Ks;
+    const int r = threadIdx.x >> 1, ch = threadIdx.x & 1;
+    const int64_t m = mt * OZ_T + r, k0 = st * OZ_KS + ch * 16;
+    const double inv = m < M ? oz_inv(sig[m]) : 0.0;
+    int8_t q[16][OZ_S];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const double x = (m < M && k0 + j < K) ? A[m * lda + k0 + j] : 0.0;
+        oz_slices(x, inv, q[j]);
+    }
+    int8_t* out = Ap + (mt * Ks + st) * OZ_PANEL + r * 32 + oz_slot(r, ch) * 16;
+#pragma unroll
+    for (int s = 0; s < OZ_S; ++s) {
+        oz_i4 w;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            w[u] = (int)(uint8_t)q[4 * u][s] | ((int)(uint8_t)q[4 * u + 1][s] << 8) |
+                   ((int)(uint8_t)q[4 * u + 2][s] << 16) | ((int)(uint8_t)q[4 * u + 3][s] << 24);
+        *reinterpret_cast<oz_i4*>(out + s * OZ_SLICE) = w;
+    }
+}
+
+// Column scales and slices of B: one block per 128-column tile; thread t takes
+// column t / 2 and, in every stage, k chunk t % 2.  Pass 1 finds the column
+// maxima (the two chunk threads of a column are adjacent lanes), pass 2 writes
+// the slices of every stage.
+__global__ __launch_bounds__(256) void oz_split_b_kernel(const double* __restrict__ B, int64_t ldb, int64_t N,
+                                                         int64_t K, int64_t Ks, double* __restrict__ tau,
+                                                         int8_t* __restrict__ Bp) {
+    const int64_t nt = blockIdx.x;
+    const int c = threadIdx.x >> 1, ch = threadIdx.x & 1;
+    const int64_t n = nt * OZ_T + c;
+    const bool col_ok = n < N;
+    double mx = 0.0;
+    bool bad = false;
+    for (int64_t st = 0; st < Ks; ++st) {
+        const int64_t k0 = st * OZ_KS + ch * 16;
+        double x[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] = (col_ok && k0 + j < K) ? B[(k0 + j) * ldb + n] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const double a = fabs(x[j]);
+            bad |= !(a <= DBL_MAX);
+            mx = a > mx ? a : mx;
+        }
+    }
+    {
+        const double o = __shfl_xor(mx, 1, 64);
+        mx = o > mx ? o : mx;
+        bad |= __shfl_xor((int)bad, 1, 64) != 0;
+    }
+    const double scale = oz_scale(mx, bad);
+    if (col_ok && ch == 0) tau[n] = scale;
+    const double inv = col_ok ? oz_inv(scale) : 0.0;
+    for (int64_t st = 0; st < Ks; ++st) {
+        const int64_t k0 = st * OZ_KS + ch * 16;
+        int8_t q[16][OZ_S];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const double x = (col_ok && k0 + j < K) ? B[(k0 + j) * ldb + n] : 0.0;
+            oz_slices(x, inv, q[j]);
+        }
+        int8_t* out = Bp + (nt * Ks + st) * OZ_PANEL + c * 32 + oz_slot(c, ch) * 16;
+#pragma unroll
+        for (int s = 0; s < OZ_S; ++s) {
+            oz_i4 w;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                w[u] = (int)(uint8_t)q[4 * u][s] | ((int)(uint8_t)q[4 * u + 1][s] << 8) |
+                       ((int)(uint8_t)q[4 * u + 2][s] << 16) | ((int)(uint8_t)q[4 * u + 3][s] << 24);
+            *reinterpret_cast<oz_i4*>(out + s * OZ_SLICE) = w;
+        }
+    }
+}
+
+__device__ inline int64_t oz_xcd_logical(int64_t per) {
+    const int64_t w = blockIdx.x;
+    return (w % 8) * per + (w / 8);
+}
+
+// Waits until at most N of this wave's global->LDS copies are outstanding and its
+// LDS reads are back, then meets the workgroup barrier.
+#define OZ_STAGE_BARRIER(N)                                     \
+    {                                                           \
+        __builtin_amdgcn_s_waitcnt((N) | (0x7 << 4));           \
+        __asm__ volatile("" ::: "memory");                      \
+        __builtin_amdgcn_s_barrier();                           \
+        __asm__ volatile("" ::: "memory");                      \
+    }
+
+// One 128 x 128 tile of C per workgroup.  Eight waves as 4 (rows) x 2 (columns):
+// wave w owns rows 32 (w / 2) .. +32 and two 32-column blocks, with five int32
+// diagonal accumulators per block (160 registers).  Stage buffers: three of
+// 40 KiB (A slices, then B slices), copies two stages ahead.  With a_lower
+// the K range of row tile mt stops at 128 (mt + 1).  Tiles are ordered row
+// tiles fastest, deepest first, so that consecutive workgroups of one XCD
+// share the column tile.
+__global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(const int8_t* __restrict__ Ap,
+                                                             const int8_t* __restrict__ Bp,
+                                                             const double* __restrict__ sig,
+                                                             const double* __restrict__ tau, double* __restrict__ C,
+                                                             int64_t ldc, int64_t M, int64_t N, int64_t Ks, int nMT,
+                                                             int64_t nT, int a_lower, double alpha, double beta,
+                                                             int64_t per) {
+    __shared__ __attribute__((aligned(16))) int8_t L[3][2 * OZ_PANEL];
+    const int64_t id = oz_xcd_logical(per);
+    if (id >= nT) return;
+    const int mt = nMT - 1 - (int)(id % nMT);
+    const int64_t nt = id / nMT;
+    const int64_t Kc = a_lower ? (Ks < 4 * (int64_t)(mt + 1) ? Ks : 4 * (int64_t)(mt + 1)) : Ks;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r32 = lane & 31, kh = lane >> 5;
+    const int wr = wave >> 1, wc = wave & 1;
+
+    // copies: chunk c = OZ_NC wave + k of the stage's 40 (20 of A, then 20 of B)
+    const int8_t* src[OZ_NC];
+    int seg[OZ_NC];
+#pragma unroll
+    for (int k = 0; k < OZ_NC; ++k) {
+        const int c = OZ_NC * wave + k;
+        const int8_t* base = c < OZ_PANEL / 1024 ? Ap + (int64_t)mt * Ks * OZ_PANEL + c * 1024
+                                                 : Bp + nt * Ks * OZ_PANEL + (c - OZ_PANEL / 1024) * 1024;
+        src[k] = base + lane * 16;
+        seg[k] = c * 1024;
+    }
+    auto copy_stage = [&](int buf, int64_t st) {
+#pragma unroll
+        for (int k = 0; k < OZ_NC; ++k)
+            __builtin_amdgcn_global_load_lds((oz_gbl_ptr)(src[k] + st * OZ_PANEL), (oz_lds_ptr)(&L[buf][0] + seg[k]),
+                                             16, 0, 0);
+    };
+    const int rowA = wr * 32 + r32;
+    const int offA = rowA * 32 + oz_slot(rowA, kh) * 16;
+    int offB[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int col = (2 * wc + q) * 32 + r32;
+        offB[q] = OZ_PANEL + col * 32 + oz_slot(col, kh) * 16;
+    }
+    oz_i16 acc[OZ_S][2];
+#pragma unroll
+    for (int d = 0; d < OZ_S; ++d)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[d][q][r] = 0;
+
+    copy_stage(0, 0);
+    copy_stage(1, Kc > 1 ? 1 : Kc - 1);
+    OZ_STAGE_BARRIER(OZ_NC)
+    int buf = 0;
+    for (int64_t s = 0; s < Kc; ++s) {
+        copy_stage(buf == 0 ? 2 : buf - 1, s + 2 < Kc ? s + 2 : Kc - 1);
+        const int8_t* S = &L[buf][0];
+        oz_i4 a[OZ_S];
+#pragma unroll
+        for (int sa = 0; sa < OZ_S; ++sa) a[sa] = *reinterpret_cast<const oz_i4*>(S + sa * OZ_SLICE + offA);
+#pragma unroll
+        for (int t = 0; t < OZ_S; ++t) {
+            oz_i4 b[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) b[q] = *reinterpret_cast<const oz_i4*>(S + t * OZ_SLICE + offB[q]);
+#pragma unroll
+            for (int sa = 0; sa + t < OZ_S; ++sa)
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+                    acc[sa + t][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[sa], b[q], acc[sa + t][q], 0, 0, 0);
+        }
+        OZ_STAGE_BARRIER(OZ_NC)
+        buf = buf == 2 ? 0 : buf + 1;
+    }
+    __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xF << 8));
+
+    // element r of lane l: C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32] of its 32 x 32 block
+    constexpr double R7 = 1.0 / 128.0, R14 = 1.0 / 16384.0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int64_t n = nt * OZ_T + (2 * wc + q) * 32 + r32;
+        if (n >= N) continue;
+        const double tn = tau[n];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int64_t m = (int64_t)mt * OZ_T + wr * 32 + 8 * (r / 4) + 4 * kh + (r % 4);
+            if (m >= M) continue;
+            double h = (double)acc[4][q][r];
+            h = h * R7 + (double)acc[3][q][r];
+            h = h * R7 + (double)acc[2][q][r];
+            h = h * R7 + (double)acc[1][q][r];
+            h = h * R7 + (double)acc[0][q][r];
+            const double val = (h * R14) * sig[m] * tn;
+            double* c = C + m * ldc + n;
+            *c = beta == 0.0 ? alpha * val : alpha * val + beta * *c;
+        }
+    }
+}
+
+}  // namespace
+
+int gemm_ozaki(corr_ctx* ctx, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+               const double* B, int64_t ldb, double beta, double* C, int64_t ldc, bool a_lower) {
+    CORR_REQUIRE(ctx, M >= 0 && N >= 0 && K >= 0, "gemm int8x5: negative size");
+    CORR_REQUIRE(ctx, K <= OZ_KMAX, "gemm int8x5: K above 75000 (int32 range of the slice sums)");
+    if (M == 0 || N == 0) return CORR_OK;
+    CORR_REQUIRE(ctx, A && B && C && lda >= K && ldb >= N && ldc >= N, "gemm int8x5: bad operands");
+    const int64_t nMT = cdiv(M, OZ_T), nNT = cdiv(N, OZ_T), Ks = cdiv(K > 0 ? K : 1, OZ_KS);
+    CORR_REQUIRE(ctx, nMT < (1 << 30) && nMT * nNT < ((int64_t)1 << 40), "gemm int8x5: too many tiles");
+    int8_t *Ap = nullptr, *Bp = nullptr;
+    double *sig = nullptr, *tau = nullptr;
+    CORR_TRY(scratch(ctx, "oz_Ap", (size_t)nMT * Ks * OZ_PANEL, (void**)&Ap));
+    CORR_TRY(scratch(ctx, "oz_Bp", (size_t)nNT * Ks * OZ_PANEL, (void**)&Bp));
+    CORR_TRY(scratch(ctx, "oz_sig", sizeof(double) * nMT * OZ_T, (void**)&sig));
+    CORR_TRY(scratch(ctx, "oz_tau", sizeof(double) * nNT * OZ_T, (void**)&tau));
+    {
+        KernelTimer kt(ctx, "oz_split");
+        hipLaunchKernelGGL(oz_rowscale_kernel, dim3((unsigned)cdiv(M, 4)), dim3(256), 0, ctx->stream, A, lda, M,
+                           K, sig);
+        CORR_LAUNCHED(ctx, "oz_rowscale_kernel");
+        hipLaunchKernelGGL(oz_split_a_kernel, dim3((unsigned)(nMT * Ks)), dim3(256), 0, ctx->stream, A, lda, M, K,
+                           Ks, sig, Ap);
+        CORR_LAUNCHED(ctx, "oz_split_a_kernel");
+        hipLaunchKernelGGL(oz_split_b_kernel, dim3((unsigned)nNT), dim3(256), 0, ctx->stream, B, ldb, N, K, Ks,
+                           tau, Bp);
+        CORR_LAUNCHED(ctx, "oz_split_b_kernel");
+    }
+    const int64_t nT = nMT * nNT, per = cdiv(nT, 8);
+    KernelTimer kt(ctx, "oz_gemm");
+    hipLaunchKernelGGL(oz_gemm_kernel, dim3((unsigned)(8 * per)), dim3(OZ_THREADS), 0, ctx->stream, Ap, Bp, sig,
+                       tau, C, ldc, M, N, Ks, (int)nMT, nT, a_lower ? 1 : 0, alpha, beta, per);
+    CORR_LAUNCHED(ctx, "oz_gemm_kernel");
+    return CORR_OK;
+}
+
+}  // namespace corr

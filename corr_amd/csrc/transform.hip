@@ -199,14 +199,23 @@ __global__ __launch_bounds__(256) void canonical_mp2_kernel(const double* __rest
 
 int gemm_prec(int prec) { return prec == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_PREC_FP32; }
 
+// Precision of the AO->MO GEMMs of B formation.  Their sums cancel: FP32
+// leaves a 3e-6 rms relative error in (P|ia) at (H2O)32 / def2-SVP, which the
+// Cholesky metric amplifies to ~1e-4 in B and ~0.3 uEh in EMP2.  FP32 only
+// when asked for; FP64 for FP64 and for the mixed path (BF16X3).
+static int ao_prec(int prec) { return prec == CORR_PREC_FP32 ? CORR_PREC_FP32 : CORR_PREC_FP64; }
+
 // Precision of the metric product Y = M X.  The dense J^-1/2 of the eigen
 // metric has entries of order lambda_min^-1/2 whose products cancel in Y, so
-// FP32 rounding of that product is amplified (4.1 uEh in EMP2 on the
-// (H2O)32 / def2-SVP-RI inputs, against 0.02 uEh with the triangular
-// Cholesky factor): with the eigen metric it always runs in FP64, as the
-// factorisation does.
+// reduced-precision rounding of that product is amplified (FP32: 4.1 uEh in
+// EMP2 on the (H2O)32 / def2-SVP-RI inputs): with the eigen metric it always
+// runs in FP64, as the factorisation does.  With the Cholesky factor an FP32
+// product still costs 0.28 uEh there; the mixed path (BF16X3) runs it on the
+// int8 matrix cores with 5 slices per operand (max relative error of B
+// 2.8e-8, 0.002 uEh), at a quarter of the FP64 MFMA time.
 static int metric_gemm_prec(int mode, int prec) {
-    return mode == CORR_METRIC_EIGEN ? CORR_PREC_FP64 : gemm_prec(prec);
+    if (mode == CORR_METRIC_EIGEN || prec == CORR_PREC_FP64) return CORR_PREC_FP64;
+    return prec == CORR_PREC_BF16X3 ? CORR_PREC_INT8X5 : CORR_PREC_FP32;
 }
 
 }  // namespace
@@ -378,7 +387,7 @@ int transform_ri(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double
     CORR_REQUIRE(ctx, naux > 0 && N > 0 && o > 0 && o < N && C && BPQ && Bia, "transform_ri: bad arguments");
     double* X = nullptr;
     CORR_TRY(scratch(ctx, "ri_Xvo", sizeof(double) * naux * o * (N - o), (void**)&X));
-    CORR_TRY(ao2vo(ctx, naux, N, o, C, BPQ, X, gemm_prec(prec)));
+    CORR_TRY(ao2vo(ctx, naux, N, o, C, BPQ, X, ao_prec(prec)));
     return transpose_vo(ctx, naux, N - o, o, X, Bia);
 }
 
@@ -388,7 +397,7 @@ int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double*
                  "ri_form_bia: bad arguments");
     CORR_REQUIRE(ctx, mode == CORR_METRIC_EIGEN || mode == CORR_METRIC_CHOLESKY, "ri_form_bia: unknown metric mode");
     const int64_t v = N - o;
-    const int p = gemm_prec(prec);
+    const int p = ao_prec(prec);
     double* X = nullptr;
     CORR_TRY(scratch(ctx, "ri_Xov", sizeof(double) * naux * o * v, (void**)&X));
     CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, p));
@@ -439,7 +448,7 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
     CORR_REQUIRE(ctx, mode == CORR_METRIC_EIGEN || mode == CORR_METRIC_CHOLESKY, "ri_bia_from_j: unknown metric mode");
     CORR_REQUIRE(ctx, J != M, "ri_bia_from_j: J and M must not alias");
     const int64_t v = N - o;
-    const int p = gemm_prec(prec);
+    const int p = ao_prec(prec);
     double* X = nullptr;
     CORR_TRY(scratch(ctx, "ri_Xov", sizeof(double) * naux * o * v, (void**)&X));
     if (!M) CORR_TRY(scratch(ctx, "ri_M", sizeof(double) * naux * naux, (void**)&M));

@@ -23,6 +23,15 @@ def _prec(prec: str | int) -> int:
         raise ValueError(f"unknown precision {prec!r}; use one of {list(_abi.PRECISIONS)}") from None
 
 
+def _gemm_prec(prec: str | int) -> int:
+    if isinstance(prec, int):
+        return prec
+    try:
+        return _abi.GEMM_PRECISIONS[prec]
+    except KeyError:
+        raise ValueError(f"unknown GEMM precision {prec!r}; use one of {list(_abi.GEMM_PRECISIONS)}") from None
+
+
 def _dev(*ts: torch.Tensor) -> torch.Tensor:
     for t in ts:
         if not isinstance(t, torch.Tensor):
@@ -42,7 +51,8 @@ def _ctx(t: torch.Tensor) -> Context:
 
 def dgemm(A: torch.Tensor, B: torch.Tensor, transa: bool = False, transb: bool = False,
           alpha: float = 1.0, prec: str | int = "fp64") -> torch.Tensor:
-    """C = alpha op(A) op(B) on the MFMA GEMM (2-D, row-major)."""
+    """C = alpha op(A) op(B) on the MFMA GEMM (2-D, row-major).  prec "int8x5":
+    FP64 emulated on the int8 matrix cores (no transposes)."""
     _dev(A, B)
     M = A.shape[1] if transa else A.shape[0]
     K = A.shape[0] if transa else A.shape[1]
@@ -52,7 +62,7 @@ def dgemm(A: torch.Tensor, B: torch.Tensor, transa: bool = False, transb: bool =
         raise ValueError("inner dimensions differ")
     Cm = torch.empty(M, N, dtype=torch.float64, device=A.device)
     _ctx(A).call("corr_dgemm", int(transa), int(transb), M, N, K, alpha, ptr(A), A.shape[1], 0,
-                 ptr(B), B.shape[1], 0, 0.0, ptr(Cm), N, 0, 1, _prec(prec))
+                 ptr(B), B.shape[1], 0, 0.0, ptr(Cm), N, 0, 1, _gemm_prec(prec))
     return Cm
 
 

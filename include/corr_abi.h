@@ -58,6 +58,10 @@ extern "C" {
 #define CORR_PREC_FP64     0   /* FP64 MFMA, FP64 accumulate (plain method)  */
 #define CORR_PREC_FP32     1   /* FP32 MFMA, FP32 accumulate                 */
 #define CORR_PREC_BF16X3   2   /* hi/lo BF16 split, 3 products, FP32 acc.    */
+#define CORR_PREC_INT8X5   3   /* corr_dgemm only: FP64 emulated on int8      */
+                               /* MFMA, 5 slices per operand (Ozaki), exact   */
+                               /* int32 sums; error below 2^-35 of the row    */
+                               /* and column maxima                           */
 
 /* ---- RI metric factorisation ------------------------------------------- */
 #define CORR_METRIC_EIGEN     0 /* J^-1/2 = V L^-1/2 V^T, cutoff as ri.cc:56-66 */
@@ -118,6 +122,8 @@ int corr_d2h(corr_ctx* ctx, void* h_dst, const void* d_src, int64_t bytes);
  * op(A) is M x K: transa = 0 -> A is M x K (lda >= K), 1 -> A is K x M.
  * op(B) is K x N: transb = 0 -> B is K x N (ldb >= N), 1 -> B is N x K.
  * All operands FP64 in memory; prec selects the MFMA arithmetic.
+ * CORR_PREC_INT8X5 takes transa = transb = 0 and batch = 1 only, K <= 75000;
+ * a non-finite entry makes its row of C (in A) or column (in B) NaN.
  * Used for every GEMM-shaped contraction of the path. */
 int corr_dgemm(corr_ctx* ctx, int transa, int transb,
                int64_t M, int64_t N, int64_t K, double alpha,
@@ -168,7 +174,13 @@ int corr_ri_metric(corr_ctx* ctx, int64_t naux, const double* d_J,
  *                   Coulomb integrals are: nu is contracted with the occupied
  *                   and mu with the virtual block, and Bia is written by the
  *                   metric GEMM in place (no transpose).
- * GEMM steps requested in CORR_PREC_BF16X3 run in FP32 MFMA. */
+ * Precision: CORR_PREC_FP64 and CORR_PREC_FP32 run every GEMM in that
+ * precision (the metric product of EIGEN always in FP64).  CORR_PREC_BF16X3
+ * asks for the mixed path's B formation: the AO->MO GEMMs of
+ * corr_transform_ri, corr_ri_form_bia and corr_ri_bia_from_j in FP64 (their
+ * sums cancel: FP32 leaves ~0.3 uEh per GEMM in EMP2 at (H2O)32) and the
+ * CHOLESKY metric product on int8 MFMA as CORR_PREC_INT8X5 (B within 3e-8 of
+ * FP64, 0.002 uEh in EMP2); corr_ri_bpq runs it in FP32. */
 int corr_ri_bpq(corr_ctx* ctx, int64_t naux, int64_t nroao, const double* d_M,
                 const double* d_Pmn, double* d_BPQ, int prec);
 int corr_transform_ri(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,

@@ -93,12 +93,14 @@ def test_molecular_density_guess_converges_to_same_state():
 
 def test_h2o32_mixed_precision_error_real_chemistry():
     """(H2O)32 / def2-SVP / def2-SVP-RI (N = 768, o = 160, naux = 2432): the
-    mixed path (FP64 B formation, BF16x3 energy) stays within 1 uEh of FP64
-    (0.77 measured) and the FP32 energy kernel on the same B within 0.2 uEh.
-    B formed with FP32 GEMMs is off by up to a few uEh: the AO->MO sums cancel
-    (3e-6 rms relative error in (Q|ia)) and the Cholesky metric (cond(J) =
-    1.1e6) amplifies that to 8e-5 in B, an energy error of about 0.5 uEh per
-    FP32 stage whose sign follows the summation order."""
+    mixed path (FP64 AO->MO GEMMs, the Cholesky metric product emulated on
+    int8 MFMA, BF16x3 energy) stays within 1 uEh of FP64 (0.76 measured); its
+    B alone moves the FP64 energy by at most 0.02 uEh (0.002 measured); the
+    FP32 energy kernel on the FP64 B stays within 0.2 uEh.  B formed with
+    FP32 GEMMs is off by up to a few uEh: the AO->MO sums cancel (3e-6 rms
+    relative error in (Q|ia)) and the Cholesky metric (cond(J) = 1.1e6)
+    amplifies that to ~1e-4 in B, an energy error of about 0.3 uEh per FP32
+    stage whose sign follows the summation order."""
     say("chem (H2O)32 mixed-precision error")
     from corr_amd import chem
     import corr_amd
@@ -108,9 +110,15 @@ def test_h2o32_mixed_precision_error_real_chemistry():
     o, eo, ev = r["nocc"], r["eps_occ"], r["eps_vir"]
     B64, _ = corr_amd.ri_bia_from_j(r["C"], r["Pmn"], r["J"], o, "cholesky", "fp64")
     e64 = corr_amd.rimp2_energy(B64, eo, ev, "fp64")["EMP2"]
-    emx = corr_amd.rimp2_energy(B64, eo, ev, "bf16x3")["EMP2"]
+    Bmx, _ = corr_amd.ri_bia_from_j(r["C"], r["Pmn"], r["J"], o, "cholesky", "bf16x3")
+    rel = ((Bmx - B64).abs().max() / B64.abs().max()).item()
+    eb = corr_amd.rimp2_energy(Bmx, eo, ev, "fp64")["EMP2"]
+    emx = corr_amd.rimp2_energy(Bmx, eo, ev, "bf16x3")["EMP2"]
+    del Bmx
+    say(f"mixed B formation: max |dB| / max |B| = {rel:.3e}, FP64 EMP2 moves {abs(eb - e64) * 1e6:.4f} uEh")
     say(f"EMP2 fp64 {e64:.10f} mixed {emx:.10f} error {abs(emx - e64) * 1e6:.4f} uEh")
     assert -0.22 * 32 < e64 < -0.19 * 32
+    assert rel <= 1e-7 and abs(eb - e64) <= 2e-8
     assert abs(emx - e64) <= 1e-6
     e32 = corr_amd.rimp2_energy(B64, eo, ev, "fp32")["EMP2"]
     say(f"EMP2 FP32 energy kernel {e32:.10f} error {abs(e32 - e64) * 1e6:.4f} uEh")

@@ -1,0 +1,100 @@
+"""The int8-emulated FP64 GEMM (CORR_PREC_INT8X5) and the mixed path's B
+formation that uses it, through the C-ABI: bit-identical to the CPU
+restatement in tests/ozaki_emulation.py (exact integer slice products, FP64
+combination in the kernel's order), within the slice error bound of the FP64
+product, NaN rows and columns for non-finite inputs."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from tests import _synth as S
+from tests import ozaki_emulation as OZ
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+
+
+@pytest.fixture(scope="module")
+def corr():
+    import corr_amd
+    return corr_amd
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (37, 53, 29), (128, 128, 32), (130, 257, 70), (300, 64, 400),
+                                   (257, 384, 1000)])
+def test_gemm_int8x5_bit_exact(corr, M, N, K):
+    _say(f"gemm int8x5 M={M} N={N} K={K}")
+    rng = np.random.default_rng(M * 7 + N + K)
+    A = rng.standard_normal((M, K)) * 10.0 ** rng.uniform(-6, 0, (M, K))
+    B = rng.standard_normal((K, N)) * 10.0 ** rng.uniform(-6, 0, (K, N))
+    got = corr.dgemm(dev(A), dev(B), alpha=-0.75, prec="int8x5").cpu().numpy()
+    want = OZ.gemm_int8x5(A, B, alpha=-0.75)
+    assert np.array_equal(got, want), np.max(np.abs(got - want))
+    sig, tau = OZ.row_scales(A), OZ.row_scales(B.T)
+    err = np.max(np.abs(got + 0.75 * (A @ B)) / (sig[:, None] * tau[None, :]))
+    _say(f"  max |C - AB| / (sig tau) = {err:.3e} (bound {4 * K * 2.0 ** -35:.3e})")
+    assert err <= 0.75 * 4 * K * 2.0 ** -35
+
+
+def test_gemm_int8x5_non_finite(corr):
+    rng = np.random.default_rng(11)
+    A, B = rng.standard_normal((70, 90)), rng.standard_normal((90, 150))
+    A[5, 40] = np.nan
+    A[66, 0] = np.inf
+    B[17, 129] = -np.inf
+    got = corr.dgemm(dev(A), dev(B), prec="int8x5").cpu().numpy()
+    bad = np.zeros(got.shape, bool)
+    bad[[5, 66]] = True
+    bad[:, 129] = True
+    assert np.all(np.isnan(got[bad])) and np.all(np.isfinite(got[~bad]))
+    assert np.array_equal(got[~bad], OZ.gemm_int8x5(A, B)[~bad])
+
+
+def test_gemm_int8x5_rejects(corr):
+    from corr_amd._abi import CorrError
+    a = torch.zeros(2, 75001, dtype=torch.float64, device="cuda")
+    b = torch.zeros(75001, 2, dtype=torch.float64, device="cuda")
+    with pytest.raises(CorrError):
+        corr.dgemm(a, b, prec="int8x5")
+    c = torch.ones(3, 3, dtype=torch.float64, device="cuda")
+    with pytest.raises(CorrError):
+        corr.dgemm(c, c, transa=True, prec="int8x5")
+    assert torch.equal(corr.dgemm(c, c, prec="int8x5"), torch.full((3, 3), 3.0, dtype=torch.float64,
+                                                                   device="cuda"))
+
+
+@pytest.mark.parametrize("naux,N,o", [(152, 48, 10), (300, 70, 13)])
+def test_mixed_b_formation(corr, naux, N, o):
+    """B formation of the mixed path (prec bf16x3): FP64 AO->MO GEMMs (bitwise
+    the FP64 path's (P|ia)) and the Cholesky metric product on int8 MFMA,
+    bit-identical to the restatement applied to that (P|ia); within 1e-7 of
+    the FP64 B; the overlapped entry gives the same bits."""
+    _say(f"mixed B formation naux={naux} N={N} o={o}")
+    rng = np.random.default_rng(naux + N)
+    J = S.metric(naux, seed=naux, cond=1e5)
+    Pmn = rng.standard_normal((naux, N, N))
+    Pmn = Pmn + Pmn.transpose(0, 2, 1)
+    C = S.orthonormal(N, seed=N)
+    dJ, dP, dC = dev(J), dev(Pmn), dev(C)
+    M, _ = corr.ri_metric(dJ, "cholesky")
+    eye = torch.eye(naux, dtype=torch.float64, device="cuda")
+    X = corr.ri_form_bia(dC, dP, eye, o, "cholesky", "fp64").reshape(naux, -1).cpu().numpy()
+    B64 = corr.ri_form_bia(dC, dP, M, o, "cholesky", "fp64").cpu().numpy()
+    Bm = corr.ri_form_bia(dC, dP, M, o, "cholesky", "bf16x3").cpu().numpy()
+    want = OZ.gemm_int8x5(M.cpu().numpy(), X).reshape(Bm.shape)
+    assert np.array_equal(Bm, want)
+    rel = np.max(np.abs(Bm - B64)) / np.max(np.abs(B64))
+    _say(f"  max |B_mixed - B_fp64| / max |B| = {rel:.3e}")
+    assert rel <= 1e-7
+    got, _ = corr.ri_bia_from_j(dC, dP, dJ, o, "cholesky", "bf16x3")
+    assert np.array_equal(got.cpu().numpy(), Bm)
