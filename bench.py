@@ -8,10 +8,11 @@ RI-MP2 pair-block energy.  Sizes are those of
 (H2O)32: N = 768 basis functions, o = 160 occupied, v = 608 virtual,
 naux = 2432 auxiliary functions.  Three paths are timed, K steps each:
 
-  mixed: metric factorisation and AO->MO GEMMs FP64 (FP64 MFMA), the
-         Cholesky metric product as an FP64 GEMM emulated on the int8 matrix
-         cores (5 int8 slices per operand, exact int32 sums), energy BF16x3
-         MFMA with FP32 accumulation and FP64 denominators/reduction
+  mixed: metric factorisation and the first AO->MO GEMM FP64 (FP64 MFMA),
+         the second AO->MO GEMM and the Cholesky metric product as FP64
+         GEMMs emulated on the int8 matrix cores (5 int8 slices per operand,
+         exact int32 sums), energy BF16x3 MFMA with FP32 accumulation and FP64
+         denominators/reduction
   fp64:  everything FP64 (FP64 MFMA)
   fp32:  B-formation GEMMs and energy in FP32 MFMA (reported beside, with
          its error against fp64)
@@ -20,8 +21,8 @@ FP32 has no place in the mixed path's B formation: the AO->MO sums cancel
 (3e-6 rms relative error in (Q|ia) from FP32 GEMMs) and the Cholesky metric
 (cond(J) = 1.1e6) amplifies that to ~1e-4 in B, an energy error of about
 0.3 uEh per FP32 stage with a sign set by the summation order; an FP32
-metric product alone costs 0.28 uEh.  The int8-emulated metric product keeps
-B within 3e-8 of FP64 (0.002 uEh).
+metric product alone costs 0.28 uEh.  The int8-emulated GEMMs keep B within
+3e-7 of FP64 relative to its maximum (0.003 uEh).
 
 value = algorithmic TFLOP/s of the mixed path, (W_B_chol + W_E) / t_step, with
 W_E = 2 [o(o+1)/2] v^2 naux and W_B_chol = 2 naux o N^2 + 2 naux o v N + naux^2 o v
@@ -248,7 +249,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3 energy (fp32 accumulation; fp64 AO->MO GEMMs, int8-emulated fp64 metric product, "
+        "dtype": "bf16x3 energy (fp32 accumulation; B formation fp64 and int8-emulated fp64 GEMMs, "
                  "fp64 denominators and reduction)",
         "data": ("real chemistry: seeded water cluster, def2-SVP/def2-SVP-RI integrals and RI-JK RHF computed on the "
                  "GPU by this library before the timed region" if args.data == "real" else

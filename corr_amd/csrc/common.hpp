@@ -68,10 +68,11 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
          const double* B, int64_t ldb, int64_t sB, double beta, double* C,
          int64_t ldc, int64_t sC, int64_t batch, int prec, bool a_lower = false);
 
-// C = alpha A B + beta C (row-major, no transposes, K <= 75000) as an FP64
-// GEMM emulated on the int8 matrix cores with 5 slices per operand (ozaki.hip).
-int gemm_ozaki(corr_ctx* ctx, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
-               const double* B, int64_t ldb, double beta, double* C, int64_t ldc, bool a_lower);
+// C = alpha op(A) op(B) + beta C (row-major, K <= 75000) as an FP64 GEMM
+// emulated on the int8 matrix cores with 5 slices per operand (ozaki.hip).
+int gemm_ozaki(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K, double alpha,
+               const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+               bool a_lower);
 
 // op(A) op(B) (M x N, alpha 1) written through an output mapping of the
 // 4-index transform (gemm.hip): mapping 1 (PAIRS): row X n + i, column j ->

@@ -585,8 +585,8 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
                           prec == CORR_PREC_INT8X5,
                  "gemm: unknown precision");
     if (prec == CORR_PREC_INT8X5) {
-        CORR_REQUIRE(ctx, transa == 0 && transb == 0 && batch == 1, "gemm int8x5: no transposes, batch 1");
-        return gemm_ozaki(ctx, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, a_lower);
+        CORR_REQUIRE(ctx, batch == 1, "gemm int8x5: batch 1 only");
+        return gemm_ozaki(ctx, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, a_lower);
     }
     const int f64 = (prec == CORR_PREC_FP64) ? 1 : 0;
     if ((N == 1 || M == 1) && !a_lower && K > 0) {

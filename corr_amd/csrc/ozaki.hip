@@ -1,8 +1,9 @@
 // FP64 GEMM emulated on the int8 matrix cores (Ozaki splitting).
 //
-// C = alpha A B + beta C for row-major A (M x K) and B (K x N).  Every row m of
-// A gets a power-of-two scale sig[m] >= max_k |A[m][k]| (127/128 headroom) and
-// every column n of B a scale tau[n]; the scaled values x in (-1, 1) are cut
+// C = alpha op(A) op(B) + beta C for row-major FP64 operands (op = identity or
+// transpose).  Every row m of op(A) gets a power-of-two scale sig[m] >=
+// max_k |op(A)[m][k]| (127/128 headroom) and every column n of op(B) a scale
+// tau[n]; the scaled values x in (-1, 1) are cut
 // into S = 5 int8 slices by round-to-nearest,
 //     r_0 = x,  q_s = rint(128 r_s),  r_{s+1} = 128 r_s - q_s,
 // so x = sum_s q_s 2^-7(s+1) + r_S 2^-7S with |r_S| <= 1/2 (|q_0| <= 127,
@@ -19,14 +20,17 @@
 // limited to 75000 (int32 range of the widest diagonal:
 // K (2 127 64 + 3 64 64) < 2^31).
 //
-// Operands are sliced into the kernel's LDS stage order first (one pass over
-// A, two over B: column maxima, then slices), so each stage copy is a
-// contiguous 1 KiB run per wave: a stage holds 32 k of the S slices of 128 rows
+// Operands are sliced into the kernel's LDS stage order first (rows of op(A)
+// and columns of op(B) whose k run is contiguous in memory: a scale pass and a
+// slice pass over (row tile, stage) blocks; the others: one block per 128-row
+// tile, maxima then slices), so each stage copy is a contiguous 1 KiB run per
+// wave: a stage holds 32 k of the S slices of 128 rows
 // of A and of 128 columns of B, [slice][row][32 bytes], the two 16-byte k
 // chunks of a row in swizzled slots (conflict-free ds_read_b128 fragments).
 #include "common.hpp"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace corr {
 
@@ -36,8 +40,7 @@ constexpr int OZ_T = 128;                        // rows (columns) of a tile
 constexpr int OZ_KS = 32;                        // k per stage
 constexpr int OZ_SLICE = OZ_T * OZ_KS;           // bytes of one slice of a stage (4 KiB)
 constexpr int OZ_PANEL = OZ_S * OZ_SLICE;        // one operand's stage (20 KiB)
-constexpr int OZ_THREADS = 512;                  // 8 waves, 2 per SIMD
-constexpr int OZ_NC = 2 * OZ_PANEL / 1024 / (OZ_THREADS / 64);   // 1 KiB copies per wave and stage (5)
+constexpr int OZ_NQ = 2;                         // 32-column blocks per wave (8 waves)
 constexpr int64_t OZ_KMAX = 75000;
 
 typedef int oz_i4 __attribute__((ext_vector_type(4)));
@@ -81,9 +84,10 @@ __device__ inline double oz_inv(double scale) {
     return ldexp(1.0, 1 - e);
 }
 
-// sig[m] for the rows of A (one wave per row).
-__global__ __launch_bounds__(256) void oz_rowscale_kernel(const double* __restrict__ A, int64_t lda, int64_t M,
-                                                          int64_t K, double* __restrict__ sig) {
+// Operand rows r (of op(A), or columns of op(B)) whose k run is contiguous:
+// element (r, k) at X[r ld + k].  Scales, one wave per row.
+__global__ __launch_bounds__(256) void oz_scale_kmajor_kernel(const double* __restrict__ A, int64_t lda, int64_t M,
+                                                              int64_t K, double* __restrict__ sig) {
     const int64_t m = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (m >= M) return;
@@ -103,11 +107,12 @@ __global__ __launch_bounds__(256) void oz_rowscale_kernel(const double* __restri
     if (lane == 0) sig[m] = oz_scale(mx, bad);
 }
 
-// Slices of A into stage order: one block per (row tile, stage); thread t takes
-// row t / 2 and k chunk t % 2 (16 values).  Rows past M and k past K are zero.
-__global__ __launch_bounds__(256) void oz_split_a_kernel(const double* __restrict__ A, int64_t lda, int64_t M,
-                                                         int64_t K, int64_t Ks, const double* __restrict__ sig,
-                                                         int8_t* __restrict__ Ap) {
+// Slices of k-contiguous rows into stage order: one block per (row tile,
+// stage); thread t takes row t / 2 and k chunk t % 2 (16 values).  Rows past M
+// and k past K are zero.
+__global__ __launch_bounds__(256) void oz_split_kmajor_kernel(const double* __restrict__ A, int64_t lda, int64_t M,
+                                                              int64_t K, int64_t Ks, const double* __restrict__ sig,
+                                                              int8_t* __restrict__ Ap) {
     const int64_t st = blockIdx.x % Ks, mt = blockIdx.x / Ks;
     const int r = threadIdx.x >> 1, ch = threadIdx.x & 1;
     const int64_t m = mt * OZ_T + r, k0 = st * OZ_KS + ch * 16;
@@ -130,13 +135,14 @@ __global__ __launch_bounds__(256) void oz_split_a_kernel(const double* __restric
     }
 }
 
-// Column scales and slices of B: one block per 128-column tile; thread t takes
-// column t / 2 and, in every stage, k chunk t % 2.  Pass 1 finds the column
-// maxima (the two chunk threads of a column are adjacent lanes), pass 2 writes
-// the slices of every stage.
-__global__ __launch_bounds__(256) void oz_split_b_kernel(const double* __restrict__ B, int64_t ldb, int64_t N,
-                                                         int64_t K, int64_t Ks, double* __restrict__ tau,
-                                                         int8_t* __restrict__ Bp) {
+// Scales and slices of operand rows laid out along memory rows: element (r, k)
+// at X[k ld + r] (columns of op(B) = B, rows of op(A) = A^T).  One block per
+// 128-row tile; thread t takes row t / 2 and, in every stage, k chunk t % 2.
+// Pass 1 finds the maxima (the two chunk threads of a row are adjacent
+// lanes), pass 2 writes the slices of every stage.
+__global__ __launch_bounds__(256) void oz_split_rmajor_kernel(const double* __restrict__ B, int64_t ldb, int64_t N,
+                                                              int64_t K, int64_t Ks, double* __restrict__ tau,
+                                                              int8_t* __restrict__ Bp) {
     const int64_t nt = blockIdx.x;
     const int c = threadIdx.x >> 1, ch = threadIdx.x & 1;
     const int64_t n = nt * OZ_T + c;
@@ -199,20 +205,27 @@ __device__ inline int64_t oz_xcd_logical(int64_t per) {
         __asm__ volatile("" ::: "memory");                      \
     }
 
-// One 128 x 128 tile of C per workgroup.  Eight waves as 4 (rows) x 2 (columns):
-// wave w owns rows 32 (w / 2) .. +32 and two 32-column blocks, with five int32
-// diagonal accumulators per block (160 registers).  Stage buffers: three of
-// 40 KiB (A slices, then B slices), copies two stages ahead.  With a_lower
-// the K range of row tile mt stops at 128 (mt + 1).  Tiles are ordered row
-// tiles fastest, deepest first, so that consecutive workgroups of one XCD
-// share the column tile.
-__global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(const int8_t* __restrict__ Ap,
-                                                             const int8_t* __restrict__ Bp,
-                                                             const double* __restrict__ sig,
-                                                             const double* __restrict__ tau, double* __restrict__ C,
-                                                             int64_t ldc, int64_t M, int64_t N, int64_t Ks, int nMT,
-                                                             int64_t nT, int a_lower, double alpha, double beta,
-                                                             int64_t per) {
+// One 128 x 128 tile of C per workgroup, waves as 4 (rows) x 4 / NQ (columns):
+// wave w owns 32 rows and NQ 32-column blocks, with five int32 diagonal
+// accumulators per block (80 NQ registers).  NQ = 2 (8 waves of up to 256
+// registers) is used: with NQ = 1 (16 waves of 128) the accumulators and
+// fragments spill.  Stage buffers: three of 40 KiB
+// (A slices, then B slices), copies two stages ahead; the 40 1 KiB copies of
+// a stage are dealt out to the waves in turn.  With a_lower the K range of
+// row tile mt stops at 128 (mt + 1).  Tiles are ordered row tiles fastest,
+// deepest first, so that consecutive workgroups of one XCD share the column
+// tile.
+template <int NQ>
+__global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __restrict__ Ap,
+                                                               const int8_t* __restrict__ Bp,
+                                                               const double* __restrict__ sig,
+                                                               const double* __restrict__ tau,
+                                                               double* __restrict__ C, int64_t ldc, int64_t M,
+                                                               int64_t N, int64_t Ks, int nMT, int64_t nT,
+                                                               int a_lower, double alpha, double beta, int64_t per) {
+    constexpr int W = 16 / NQ, WC = 4 / NQ;                 // waves, wave columns
+    constexpr int NCH = 2 * OZ_PANEL / 1024;                 // 1 KiB copies per stage (40)
+    constexpr int NCMAX = (NCH + W - 1) / W;
     __shared__ __attribute__((aligned(16))) int8_t L[3][2 * OZ_PANEL];
     const int64_t id = oz_xcd_logical(per);
     if (id >= nT) return;
@@ -221,14 +234,15 @@ __global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(const int8_t* __res
     const int64_t Kc = a_lower ? (Ks < 4 * (int64_t)(mt + 1) ? Ks : 4 * (int64_t)(mt + 1)) : Ks;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r32 = lane & 31, kh = lane >> 5;
-    const int wr = wave >> 1, wc = wave & 1;
+    const int wr = wave / WC, wc = wave % WC;
 
-    // copies: chunk c = OZ_NC wave + k of the stage's 40 (20 of A, then 20 of B)
-    const int8_t* src[OZ_NC];
-    int seg[OZ_NC];
+    // copies: chunks c = wave + W k of the stage's 40 (20 of A, then 20 of B)
+    const int nc = (NCH - wave + W - 1) / W;
+    const int8_t* src[NCMAX];
+    int seg[NCMAX];
 #pragma unroll
-    for (int k = 0; k < OZ_NC; ++k) {
-        const int c = OZ_NC * wave + k;
+    for (int k = 0; k < NCMAX; ++k) {
+        const int c = wave + W * k < NCH ? wave + W * k : wave;
         const int8_t* base = c < OZ_PANEL / 1024 ? Ap + (int64_t)mt * Ks * OZ_PANEL + c * 1024
                                                  : Bp + nt * Ks * OZ_PANEL + (c - OZ_PANEL / 1024) * 1024;
         src[k] = base + lane * 16;
@@ -236,29 +250,36 @@ __global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(const int8_t* __res
     }
     auto copy_stage = [&](int buf, int64_t st) {
 #pragma unroll
-        for (int k = 0; k < OZ_NC; ++k)
-            __builtin_amdgcn_global_load_lds((oz_gbl_ptr)(src[k] + st * OZ_PANEL), (oz_lds_ptr)(&L[buf][0] + seg[k]),
-                                             16, 0, 0);
+        for (int k = 0; k < NCMAX; ++k)
+            if (k < nc)
+                __builtin_amdgcn_global_load_lds((oz_gbl_ptr)(src[k] + st * OZ_PANEL),
+                                                 (oz_lds_ptr)(&L[buf][0] + seg[k]), 16, 0, 0);
+    };
+    // waits for all but this wave's last nc copies (the stage copied two ahead)
+    auto stage_barrier = [&]() {
+        if (NCMAX == 3 && nc == 3) OZ_STAGE_BARRIER(3)
+        else if (nc == 2) OZ_STAGE_BARRIER(2)
+        else OZ_STAGE_BARRIER(NCMAX)
     };
     const int rowA = wr * 32 + r32;
     const int offA = rowA * 32 + oz_slot(rowA, kh) * 16;
-    int offB[2];
+    int offB[NQ];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int col = (2 * wc + q) * 32 + r32;
+    for (int q = 0; q < NQ; ++q) {
+        const int col = (NQ * wc + q) * 32 + r32;
         offB[q] = OZ_PANEL + col * 32 + oz_slot(col, kh) * 16;
     }
-    oz_i16 acc[OZ_S][2];
+    oz_i16 acc[OZ_S][NQ];
 #pragma unroll
     for (int d = 0; d < OZ_S; ++d)
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+        for (int q = 0; q < NQ; ++q)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[d][q][r] = 0;
 
     copy_stage(0, 0);
     copy_stage(1, Kc > 1 ? 1 : Kc - 1);
-    OZ_STAGE_BARRIER(OZ_NC)
+    stage_barrier();
     int buf = 0;
     for (int64_t s = 0; s < Kc; ++s) {
         copy_stage(buf == 0 ? 2 : buf - 1, s + 2 < Kc ? s + 2 : Kc - 1);
@@ -268,51 +289,89 @@ __global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(const int8_t* __res
         for (int sa = 0; sa < OZ_S; ++sa) a[sa] = *reinterpret_cast<const oz_i4*>(S + sa * OZ_SLICE + offA);
 #pragma unroll
         for (int t = 0; t < OZ_S; ++t) {
-            oz_i4 b[2];
+            oz_i4 b[NQ];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) b[q] = *reinterpret_cast<const oz_i4*>(S + t * OZ_SLICE + offB[q]);
+            for (int q = 0; q < NQ; ++q) b[q] = *reinterpret_cast<const oz_i4*>(S + t * OZ_SLICE + offB[q]);
 #pragma unroll
             for (int sa = 0; sa + t < OZ_S; ++sa)
 #pragma unroll
-                for (int q = 0; q < 2; ++q)
+                for (int q = 0; q < NQ; ++q)
                     acc[sa + t][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[sa], b[q], acc[sa + t][q], 0, 0, 0);
         }
-        OZ_STAGE_BARRIER(OZ_NC)
+        stage_barrier();
         buf = buf == 2 ? 0 : buf + 1;
     }
     __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xF << 8));
 
-    // element r of lane l: C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32] of its 32 x 32 block
+    // element r of lane l: C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32] of its 32 x 32 block;
+    // the 16 row scales and 2 column scales of a lane are loaded together
     constexpr double R7 = 1.0 / 128.0, R14 = 1.0 / 16384.0;
+    const int64_t m0 = (int64_t)mt * OZ_T + wr * 32 + 4 * kh;
+    double sg[16], tn[NQ];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int64_t n = nt * OZ_T + (2 * wc + q) * 32 + r32;
-        if (n >= N) continue;
-        const double tn = tau[n];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int64_t m = (int64_t)mt * OZ_T + wr * 32 + 8 * (r / 4) + 4 * kh + (r % 4);
-            if (m >= M) continue;
-            double h = (double)acc[4][q][r];
-            h = h * R7 + (double)acc[3][q][r];
-            h = h * R7 + (double)acc[2][q][r];
-            h = h * R7 + (double)acc[1][q][r];
-            h = h * R7 + (double)acc[0][q][r];
-            const double val = (h * R14) * sig[m] * tn;
-            double* c = C + m * ldc + n;
-            *c = beta == 0.0 ? alpha * val : alpha * val + beta * *c;
-        }
+    for (int r = 0; r < 16; ++r) {
+        const int64_t m = m0 + 8 * (r / 4) + (r % 4);
+        sg[r] = m < M ? sig[m] : 0.0;
     }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int64_t n = nt * OZ_T + (NQ * wc + q) * 32 + r32;
+        tn[q] = n < N ? tau[n] : 0.0;
+    }
+    auto store = [&](auto with_beta) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int64_t n = nt * OZ_T + (NQ * wc + q) * 32 + r32;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t m = m0 + 8 * (r / 4) + (r % 4);
+                double h = (double)acc[4][q][r];
+                h = h * R7 + (double)acc[3][q][r];
+                h = h * R7 + (double)acc[2][q][r];
+                h = h * R7 + (double)acc[1][q][r];
+                h = h * R7 + (double)acc[0][q][r];
+                const double val = alpha * ((h * R14) * sg[r] * tn[q]);
+                if (m < M && n < N) {
+                    double* c = C + m * ldc + n;
+                    if constexpr (decltype(with_beta)::value) *c = val + beta * *c;
+                    else *c = val;
+                }
+            }
+        }
+    };
+    if (beta == 0.0) store(std::false_type{});
+    else store(std::true_type{});
 }
 
 }  // namespace
 
-int gemm_ozaki(corr_ctx* ctx, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
-               const double* B, int64_t ldb, double beta, double* C, int64_t ldc, bool a_lower) {
+// Scales and stage-order slices of the R x K operand X (kmajor: (r, k) at
+// X[r ld + k], else at X[k ld + r]).
+static int oz_split(corr_ctx* ctx, const double* X, int64_t ld, int64_t R, int64_t K, int64_t Ks, bool kmajor,
+                    double* scale, int8_t* P) {
+    if (kmajor) {
+        hipLaunchKernelGGL(oz_scale_kmajor_kernel, dim3((unsigned)cdiv(R, 4)), dim3(256), 0, ctx->stream, X, ld, R,
+                           K, scale);
+        CORR_LAUNCHED(ctx, "oz_scale_kmajor_kernel");
+        hipLaunchKernelGGL(oz_split_kmajor_kernel, dim3((unsigned)(cdiv(R, OZ_T) * Ks)), dim3(256), 0, ctx->stream,
+                           X, ld, R, K, Ks, scale, P);
+        CORR_LAUNCHED(ctx, "oz_split_kmajor_kernel");
+    } else {
+        hipLaunchKernelGGL(oz_split_rmajor_kernel, dim3((unsigned)cdiv(R, OZ_T)), dim3(256), 0, ctx->stream, X, ld,
+                           R, K, Ks, scale, P);
+        CORR_LAUNCHED(ctx, "oz_split_rmajor_kernel");
+    }
+    return CORR_OK;
+}
+
+int gemm_ozaki(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K, double alpha,
+               const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+               bool a_lower) {
     CORR_REQUIRE(ctx, M >= 0 && N >= 0 && K >= 0, "gemm int8x5: negative size");
     CORR_REQUIRE(ctx, K <= OZ_KMAX, "gemm int8x5: K above 75000 (int32 range of the slice sums)");
     if (M == 0 || N == 0) return CORR_OK;
-    CORR_REQUIRE(ctx, A && B && C && lda >= K && ldb >= N && ldc >= N, "gemm int8x5: bad operands");
+    CORR_REQUIRE(ctx, A && B && C && lda >= (transa ? M : K) && ldb >= (transb ? K : N) && ldc >= N,
+                 "gemm int8x5: bad operands");
     const int64_t nMT = cdiv(M, OZ_T), nNT = cdiv(N, OZ_T), Ks = cdiv(K > 0 ? K : 1, OZ_KS);
     CORR_REQUIRE(ctx, nMT < (1 << 30) && nMT * nNT < ((int64_t)1 << 40), "gemm int8x5: too many tiles");
     int8_t *Ap = nullptr, *Bp = nullptr;
@@ -323,20 +382,13 @@ int gemm_ozaki(corr_ctx* ctx, int64_t M, int64_t N, int64_t K, double alpha, con
     CORR_TRY(scratch(ctx, "oz_tau", sizeof(double) * nNT * OZ_T, (void**)&tau));
     {
         KernelTimer kt(ctx, "oz_split");
-        hipLaunchKernelGGL(oz_rowscale_kernel, dim3((unsigned)cdiv(M, 4)), dim3(256), 0, ctx->stream, A, lda, M,
-                           K, sig);
-        CORR_LAUNCHED(ctx, "oz_rowscale_kernel");
-        hipLaunchKernelGGL(oz_split_a_kernel, dim3((unsigned)(nMT * Ks)), dim3(256), 0, ctx->stream, A, lda, M, K,
-                           Ks, sig, Ap);
-        CORR_LAUNCHED(ctx, "oz_split_a_kernel");
-        hipLaunchKernelGGL(oz_split_b_kernel, dim3((unsigned)nNT), dim3(256), 0, ctx->stream, B, ldb, N, K, Ks,
-                           tau, Bp);
-        CORR_LAUNCHED(ctx, "oz_split_b_kernel");
+        CORR_TRY(oz_split(ctx, A, lda, M, K, Ks, !transa, sig, Ap));
+        CORR_TRY(oz_split(ctx, B, ldb, N, K, Ks, transb != 0, tau, Bp));
     }
     const int64_t nT = nMT * nNT, per = cdiv(nT, 8);
     KernelTimer kt(ctx, "oz_gemm");
-    hipLaunchKernelGGL(oz_gemm_kernel, dim3((unsigned)(8 * per)), dim3(OZ_THREADS), 0, ctx->stream, Ap, Bp, sig,
-                       tau, C, ldc, M, N, Ks, (int)nMT, nT, a_lower ? 1 : 0, alpha, beta, per);
+    hipLaunchKernelGGL(oz_gemm_kernel<OZ_NQ>, dim3((unsigned)(8 * per)), dim3(64 * 16 / OZ_NQ), 0, ctx->stream, Ap,
+                       Bp, sig, tau, C, ldc, M, N, Ks, (int)nMT, nT, a_lower ? 1 : 0, alpha, beta, per);
     CORR_LAUNCHED(ctx, "oz_gemm_kernel");
     return CORR_OK;
 }

@@ -202,7 +202,10 @@ int gemm_prec(int prec) { return prec == CORR_PREC_FP64 ? CORR_PREC_FP64 : CORR_
 // Precision of the AO->MO GEMMs of B formation.  Their sums cancel: FP32
 // leaves a 3e-6 rms relative error in (P|ia) at (H2O)32 / def2-SVP, which the
 // Cholesky metric amplifies to ~1e-4 in B and ~0.3 uEh in EMP2.  FP32 only
-// when asked for; FP64 for FP64 and for the mixed path (BF16X3).
+// when asked for; FP64 for FP64 and for the mixed path (BF16X3), whose second
+// GEMM in ao2ov_sym runs on int8 MFMA with 5 slices per operand ((P|ia)
+// within 1.1e-8 of FP64 relative to its maximum; 4 slices would give 1e-6
+// and 0.02 uEh).
 static int ao_prec(int prec) { return prec == CORR_PREC_FP32 ? CORR_PREC_FP32 : CORR_PREC_FP64; }
 
 // Precision of the metric product Y = M X.  The dense J^-1/2 of the eigen
@@ -212,7 +215,8 @@ static int ao_prec(int prec) { return prec == CORR_PREC_FP32 ? CORR_PREC_FP32 : 
 // runs in FP64, as the factorisation does.  With the Cholesky factor an FP32
 // product still costs 0.28 uEh there; the mixed path (BF16X3) runs it on the
 // int8 matrix cores with 5 slices per operand (max relative error of B
-// 2.8e-8, 0.002 uEh), at a quarter of the FP64 MFMA time.
+// 2.8e-8 from this product, 2.6e-7 with the int8 AO->MO GEMM before it;
+// 0.003 uEh), at a quarter of the FP64 MFMA time.
 static int metric_gemm_prec(int mode, int prec) {
     if (mode == CORR_METRIC_EIGEN || prec == CORR_PREC_FP64) return CORR_PREC_FP64;
     return prec == CORR_PREC_BF16X3 ? CORR_PREC_INT8X5 : CORR_PREC_FP32;
@@ -340,13 +344,16 @@ static int ao2vo(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double
 //   Zt[mu][Q][i] = sum_nu Y[Q][mu][nu] C[i][nu]    (batched over mu: rows Q
 //                  at stride N^2, K-contiguous; naux x o x N per batch)
 //   X[(Q i)][a]  = sum_mu Zt[mu][(Q i)] C[o+a][mu]  (one GEMM, naux o x v x N)
+// prec is the caller's request: FP64 or FP32 for both GEMMs, and for the mixed
+// path (BF16X3) the first in FP64 and the second emulated on int8 MFMA.
 static int ao2ov_sym(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Y,
-                     double* X, int p) {
+                     double* X, int prec) {
     const int64_t v = N - o;
+    const int p1 = ao_prec(prec), p2 = prec == CORR_PREC_BF16X3 ? CORR_PREC_INT8X5 : ao_prec(prec);
     double* Zt = nullptr;
     CORR_TRY(scratch(ctx, "ri_Znui", sizeof(double) * naux * N * o, (void**)&Zt));
-    CORR_TRY(gemm(ctx, 0, 1, naux, o, N, 1.0, Y, N * N, N, C, N, 0, 0.0, Zt, o, naux * o, N, p));
-    CORR_TRY(gemm(ctx, 1, 1, naux * o, v, N, 1.0, Zt, naux * o, 0, C + o * N, N, 0, 0.0, X, v, 0, 1, p));
+    CORR_TRY(gemm(ctx, 0, 1, naux, o, N, 1.0, Y, N * N, N, C, N, 0, 0.0, Zt, o, naux * o, N, p1));
+    CORR_TRY(gemm(ctx, 1, 1, naux * o, v, N, 1.0, Zt, naux * o, 0, C + o * N, N, 0, 0.0, X, v, 0, 1, p2));
     return CORR_OK;
 }
 
@@ -397,10 +404,9 @@ int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double*
                  "ri_form_bia: bad arguments");
     CORR_REQUIRE(ctx, mode == CORR_METRIC_EIGEN || mode == CORR_METRIC_CHOLESKY, "ri_form_bia: unknown metric mode");
     const int64_t v = N - o;
-    const int p = ao_prec(prec);
     double* X = nullptr;
     CORR_TRY(scratch(ctx, "ri_Xov", sizeof(double) * naux * o * v, (void**)&X));
-    CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, p));
+    CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, prec));
     // Bia[Q][(i a)] = sum_P M[Q][P] X[P][(i a)]   (M lower triangular for CHOLESKY)
     return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1,
                 metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY);
@@ -448,7 +454,6 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
     CORR_REQUIRE(ctx, mode == CORR_METRIC_EIGEN || mode == CORR_METRIC_CHOLESKY, "ri_bia_from_j: unknown metric mode");
     CORR_REQUIRE(ctx, J != M, "ri_bia_from_j: J and M must not alias");
     const int64_t v = N - o;
-    const int p = ao_prec(prec);
     double* X = nullptr;
     CORR_TRY(scratch(ctx, "ri_Xov", sizeof(double) * naux * o * v, (void**)&X));
     if (!M) CORR_TRY(scratch(ctx, "ri_M", sizeof(double) * naux * naux, (void**)&M));
@@ -459,7 +464,7 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
     hipStream_t main = ctx->stream;
     if (mode == CORR_METRIC_EIGEN && ctx->aux_masked) {
         // the eigensolver wants the whole device: no overlap
-        CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, p));
+        CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, prec));
         CORR_TRY(ri_metric(ctx, naux, J, mode, M, nkept));
         return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1,
                     metric_gemm_prec(mode, prec), false);
@@ -469,7 +474,7 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
     if (ctx->aux_masked) {
         CORR_HIP(ctx, hipStreamWaitEvent(ctx->part_stream, ctx->ev_fork, 0));
         ctx->stream = ctx->part_stream;
-        rc = ao2ov_sym(ctx, naux, N, o, C, Pmn, X, p);
+        rc = ao2ov_sym(ctx, naux, N, o, C, Pmn, X, prec);
         if (rc == CORR_OK && hipEventRecord(ctx->ev_part, ctx->part_stream) != hipSuccess)
             rc = set_error(ctx, CORR_EHIP, "ri_bia_from_j: event record failed");
         ctx->stream = main;
@@ -478,7 +483,7 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
             return rc;
         }
     } else {
-        CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, p));
+        CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, prec));
     }
     CORR_HIP(ctx, hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
     ctx->stream = ctx->aux_stream;

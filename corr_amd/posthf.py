@@ -52,7 +52,7 @@ def _ctx(t: torch.Tensor) -> Context:
 def dgemm(A: torch.Tensor, B: torch.Tensor, transa: bool = False, transb: bool = False,
           alpha: float = 1.0, prec: str | int = "fp64") -> torch.Tensor:
     """C = alpha op(A) op(B) on the MFMA GEMM (2-D, row-major).  prec "int8x5":
-    FP64 emulated on the int8 matrix cores (no transposes)."""
+    FP64 emulated on the int8 matrix cores."""
     _dev(A, B)
     M = A.shape[1] if transa else A.shape[0]
     K = A.shape[0] if transa else A.shape[1]

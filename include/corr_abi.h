@@ -122,7 +122,7 @@ int corr_d2h(corr_ctx* ctx, void* h_dst, const void* d_src, int64_t bytes);
  * op(A) is M x K: transa = 0 -> A is M x K (lda >= K), 1 -> A is K x M.
  * op(B) is K x N: transb = 0 -> B is K x N (ldb >= N), 1 -> B is N x K.
  * All operands FP64 in memory; prec selects the MFMA arithmetic.
- * CORR_PREC_INT8X5 takes transa = transb = 0 and batch = 1 only, K <= 75000;
+ * CORR_PREC_INT8X5 takes batch = 1 only and K <= 75000;
  * a non-finite entry makes its row of C (in A) or column (in B) NaN.
  * Used for every GEMM-shaped contraction of the path. */
 int corr_dgemm(corr_ctx* ctx, int transa, int transb,
@@ -176,11 +176,12 @@ int corr_ri_metric(corr_ctx* ctx, int64_t naux, const double* d_J,
  *                   metric GEMM in place (no transpose).
  * Precision: CORR_PREC_FP64 and CORR_PREC_FP32 run every GEMM in that
  * precision (the metric product of EIGEN always in FP64).  CORR_PREC_BF16X3
- * asks for the mixed path's B formation: the AO->MO GEMMs of
- * corr_transform_ri, corr_ri_form_bia and corr_ri_bia_from_j in FP64 (their
- * sums cancel: FP32 leaves ~0.3 uEh per GEMM in EMP2 at (H2O)32) and the
- * CHOLESKY metric product on int8 MFMA as CORR_PREC_INT8X5 (B within 3e-8 of
- * FP64, 0.002 uEh in EMP2); corr_ri_bpq runs it in FP32. */
+ * asks for the mixed path's B formation, with no FP32 rounding (the AO->MO
+ * sums cancel: FP32 leaves ~0.3 uEh per GEMM in EMP2 at (H2O)32): in
+ * corr_ri_form_bia and corr_ri_bia_from_j the first AO->MO GEMM in FP64, the
+ * second and the CHOLESKY metric product on int8 MFMA as CORR_PREC_INT8X5 (B
+ * within 3e-7 of FP64 relative to its maximum, 0.003 uEh in EMP2);
+ * corr_transform_ri runs its GEMMs in FP64 and corr_ri_bpq in FP32. */
 int corr_ri_bpq(corr_ctx* ctx, int64_t naux, int64_t nroao, const double* d_M,
                 const double* d_Pmn, double* d_BPQ, int prec);
 int corr_transform_ri(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,

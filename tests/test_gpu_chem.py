@@ -93,9 +93,11 @@ def test_molecular_density_guess_converges_to_same_state():
 
 def test_h2o32_mixed_precision_error_real_chemistry():
     """(H2O)32 / def2-SVP / def2-SVP-RI (N = 768, o = 160, naux = 2432): the
-    mixed path (FP64 AO->MO GEMMs, the Cholesky metric product emulated on
-    int8 MFMA, BF16x3 energy) stays within 1 uEh of FP64 (0.76 measured); its
-    B alone moves the FP64 energy by at most 0.02 uEh (0.002 measured); the
+    mixed path (first AO->MO GEMM FP64, the second and the Cholesky metric
+    product emulated on int8 MFMA, BF16x3 energy) stays within 1 uEh of FP64
+    (0.76 measured); its B is within 1e-6 of FP64 relative to max |B|
+    (2.6e-7 measured) and moves the FP64 energy by at most 0.02 uEh (0.003
+    measured); the
     FP32 energy kernel on the FP64 B stays within 0.2 uEh.  B formed with
     FP32 GEMMs is off by up to a few uEh: the AO->MO sums cancel (3e-6 rms
     relative error in (Q|ia)) and the Cholesky metric (cond(J) = 1.1e6)
@@ -118,7 +120,7 @@ def test_h2o32_mixed_precision_error_real_chemistry():
     say(f"mixed B formation: max |dB| / max |B| = {rel:.3e}, FP64 EMP2 moves {abs(eb - e64) * 1e6:.4f} uEh")
     say(f"EMP2 fp64 {e64:.10f} mixed {emx:.10f} error {abs(emx - e64) * 1e6:.4f} uEh")
     assert -0.22 * 32 < e64 < -0.19 * 32
-    assert rel <= 1e-7 and abs(eb - e64) <= 2e-8
+    assert rel <= 1e-6 and abs(eb - e64) <= 2e-8
     assert abs(emx - e64) <= 1e-6
     e32 = corr_amd.rimp2_energy(B64, eo, ev, "fp32")["EMP2"]
     say(f"EMP2 FP32 energy kernel {e32:.10f} error {abs(e32 - e64) * 1e6:.4f} uEh")
