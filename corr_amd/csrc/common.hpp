@@ -60,13 +60,22 @@ struct KernelTimer {
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline int64_t rup(int64_t a, int64_t b) { return cdiv(a, b) * b; }
 
-// GEMM launcher shared by all modules (row-major, see corr_dgemm).
-// a_lower: op(A) is lower triangular (entries with k > m are zero and
-// skipped); only used for the Cholesky-metric application.
+// Triangular structure a GEMM may skip (flags of gemm's tri argument):
+//   A_LOWER: op(A) is lower triangular (entries with k > m are zero): the K
+//            range of a row tile ends after its last row;
+//   B_LOWER: op(B) is lower triangular (entries with k < n are zero): the K
+//            range of a column tile starts at its first column;
+//   C_LOWER: only C on and below the diagonal is wanted: tiles wholly above
+//            it are skipped (and left untouched; tiles across it are written
+//            whole).
+enum { GEMM_TRI_A_LOWER = 1, GEMM_TRI_B_LOWER = 2, GEMM_TRI_C_LOWER = 4 };
+
+// GEMM launcher shared by all modules (row-major, see corr_dgemm); tri takes
+// GEMM_TRI_* flags (0 or 1 = A_LOWER for the Cholesky-metric application).
 int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
          double alpha, const double* A, int64_t lda, int64_t sA,
          const double* B, int64_t ldb, int64_t sB, double beta, double* C,
-         int64_t ldc, int64_t sC, int64_t batch, int prec, bool a_lower = false);
+         int64_t ldc, int64_t sC, int64_t batch, int prec, int tri = 0);
 
 // C = alpha op(A) op(B) + beta C (row-major, K <= 75000) as an FP64 GEMM
 // emulated on the int8 matrix cores with 5 slices per operand (ozaki.hip).

@@ -32,7 +32,7 @@ struct GemmArgs {
     double* C; int64_t ldc, sC;
     double alpha, beta;
     int64_t tilesM;
-    int a_lower;
+    int tri;          // GEMM_TRI_* flags
     int64_t ksplit;   // > 0: blockIdx.y is a K chunk of this depth (C = partial slice), not a batch index
     int dma;          // FP64 operands 16-byte aligned (rows, batch and K-chunk offsets): global_load_lds staging
     int epi;          // output mapping (GEMM_EPI_*), beta = 0 unless PLAIN
@@ -289,7 +289,17 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs g) {
         B = g.B + b * g.sB;
     }
     double* C = g.C + b * g.sC;
-    if (g.a_lower) kEnd = (m0 + BM < kEnd) ? (m0 + BM) : kEnd;
+    if (g.tri & GEMM_TRI_C_LOWER) {
+        if (n0 >= m0 + BM) return;   // the tile lies above the diagonal
+    }
+    if (g.tri & GEMM_TRI_A_LOWER) kEnd = (m0 + BM < kEnd) ? (m0 + BM) : kEnd;
+    if (g.tri & GEMM_TRI_B_LOWER) {
+        // rows k < n0 of op(B) are zero: start the K range at n0
+        const int64_t kb = n0 < kEnd ? n0 : kEnd;
+        A += TA ? kb * g.lda : kb;
+        B += TB ? kb : kb * g.ldb;
+        kEnd -= kb;
+    }
     const int64_t kTiles = (kEnd + BK - 1) / BK;
 
     acc_t acc[BI][BJ];
@@ -573,7 +583,7 @@ static int dma_ok(int f64, const double* A, int64_t lda, int64_t sA, const doubl
 int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
          double alpha, const double* A, int64_t lda, int64_t sA,
          const double* B, int64_t ldb, int64_t sB, double beta, double* C,
-         int64_t ldc, int64_t sC, int64_t batch, int prec, bool a_lower) {
+         int64_t ldc, int64_t sC, int64_t batch, int prec, int tri) {
     CORR_REQUIRE(ctx, M >= 0 && N >= 0 && K >= 0 && batch >= 0, "gemm: negative size");
     if (M == 0 || N == 0 || batch == 0) return CORR_OK;
     CORR_REQUIRE(ctx, A && B && C, "gemm: null operand");
@@ -586,10 +596,11 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
                  "gemm: unknown precision");
     if (prec == CORR_PREC_INT8X5) {
         CORR_REQUIRE(ctx, batch == 1, "gemm int8x5: batch 1 only");
-        return gemm_ozaki(ctx, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, a_lower);
+        CORR_REQUIRE(ctx, (tri & ~GEMM_TRI_A_LOWER) == 0, "gemm int8x5: only a lower-triangular A");
+        return gemm_ozaki(ctx, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tri != 0);
     }
     const int f64 = (prec == CORR_PREC_FP64) ? 1 : 0;
-    if ((N == 1 || M == 1) && !a_lower && K > 0) {
+    if ((N == 1 || M == 1) && tri == 0 && K > 0) {
         GemvArgs g;
         if (N == 1) {   // y[m] = sum_k op(A)(m, k) op(B)(k, 0), y at C[m ldc]
             g = GemvArgs{M, K, A, transa ? 1 : lda, transa ? lda : 1, sA, B, transb ? 1 : ldb, sB, C, ldc, sC,
@@ -605,7 +616,7 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
     // partial products to a workspace, then a fixed-order sum applies alpha
     // and beta (deterministic).
     const int64_t tiles_est = cdiv(M, 128) * cdiv(N, 128) * batch;
-    if (batch == 1 && !a_lower && tiles_est < 256 && K >= 2048) {
+    if (batch == 1 && tri == 0 && tiles_est < 256 && K >= 2048) {
         int64_t S = cdiv(512, tiles_est);
         if (S > K / 512) S = K / 512;
         const int64_t cap = (int64_t)(256 << 20) / (8 * M * N);   // workspace at most 256 MiB
@@ -629,7 +640,7 @@ int gemm(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K,
             return CORR_OK;
         }
     }
-    GemmArgs a{M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, alpha, beta, 0, a_lower ? 1 : 0, 0};
+    GemmArgs a{M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, alpha, beta, 0, tri, 0};
     a.dma = dma_ok(f64, A, lda, sA, B, ldb, sB);
     a.epi = GEMM_EPI_PLAIN;
     a.en = 0;

@@ -209,9 +209,9 @@ static int trinv_rec(corr_ctx* ctx, const double* L, int64_t n, int64_t r0, int6
     CORR_TRY(trinv_rec(ctx, L, n, r0 + n1, nb - nb1, n2, X, Linv, T));
     // T (n2 x n1) = L21 X11 ;  X21 = -X22 T  (X22 lower triangular)
     CORR_TRY(gemm(ctx, 0, 0, n2, n1, n1, 1.0, L + (r0 + n1) * n + r0, n, 0, X + r0 * n + r0, n, 0,
-                  0.0, T, n1, 0, 1, CORR_PREC_FP64));
+                  0.0, T, n1, 0, 1, CORR_PREC_FP64, GEMM_TRI_B_LOWER));
     CORR_TRY(gemm(ctx, 0, 0, n2, n1, n2, -1.0, X + (r0 + n1) * n + (r0 + n1), n, 0, T, n1, 0, 0.0,
-                  X + (r0 + n1) * n + r0, n, 0, 1, CORR_PREC_FP64, true));
+                  X + (r0 + n1) * n + r0, n, 0, 1, CORR_PREC_FP64, GEMM_TRI_A_LOWER));
     return CORR_OK;
 }
 
@@ -239,9 +239,11 @@ static int cholesky_inverse(corr_ctx* ctx, int64_t n, const double* J, double* M
         hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for(m * kb)), dim3(256), 0, ctx->stream, P,
                            (int64_t)NB, Ap, n, m, (int64_t)kb);
         CORR_LAUNCHED(ctx, "copy2d_kernel");
-        // Trailing update: A22 -= P P^T
+        // Trailing update: A22 -= P P^T on and below the diagonal (the upper
+        // triangle is never read and is zeroed at the end)
         double* A22 = A + (k0 + kb) * n + (k0 + kb);
-        CORR_TRY(gemm(ctx, 0, 1, m, m, kb, -1.0, P, NB, 0, P, NB, 0, 1.0, A22, n, 0, 1, CORR_PREC_FP64));
+        CORR_TRY(gemm(ctx, 0, 1, m, m, kb, -1.0, P, NB, 0, P, NB, 0, 1.0, A22, n, 0, 1, CORR_PREC_FP64,
+                      GEMM_TRI_C_LOWER));
     }
     hipLaunchKernelGGL(zero_upper_kernel, dim3(grid_for(n * n)), dim3(256), 0, ctx->stream, A, n);
     CORR_LAUNCHED(ctx, "zero_upper_kernel");

@@ -409,7 +409,7 @@ int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double*
     CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, prec));
     // Bia[Q][(i a)] = sum_P M[Q][P] X[P][(i a)]   (M lower triangular for CHOLESKY)
     return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1,
-                metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY);
+                metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY ? GEMM_TRI_A_LOWER : 0);
 }
 
 // The context's two overlap streams: aux_stream for the latency-bound metric
@@ -499,7 +499,7 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
     if (ctx->aux_masked) CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_part, 0));
     CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_join, 0));
     return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1,
-                metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY);
+                metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY ? GEMM_TRI_A_LOWER : 0);
 }
 
 }  // namespace corr

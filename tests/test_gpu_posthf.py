@@ -330,8 +330,11 @@ def test_rimp2_separated_waters_size_extensive(corr, prec, bound_uEh):
     assert err <= bound_uEh
 
 
-@pytest.mark.parametrize("naux", [7, 64, 65, 200])
+@pytest.mark.parametrize("naux", [7, 64, 65, 200, 700])
 def test_ri_metric(corr, naux):
+    """Both factors against the oracle; at 700 the Cholesky trailing updates
+    skip tiles above the diagonal and the inverse's L21 X11 products start
+    their K range at the tile's first column, over several 128-row tiles."""
     _say(f"ri_metric naux={naux}")
     J = S.metric(naux, seed=naux, cond=1e5)
     Me, kept = corr.ri_metric(dev(J), "eigen")
