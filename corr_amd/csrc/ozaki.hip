@@ -29,6 +29,7 @@
 // chunks of a row in swizzled slots (conflict-free ds_read_b128 fragments).
 #include "common.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <type_traits>
 
@@ -41,6 +42,11 @@ constexpr int OZ_KS = 32;                        // k per stage
 constexpr int OZ_SLICE = OZ_T * OZ_KS;           // bytes of one slice of a stage (4 KiB)
 constexpr int OZ_PANEL = OZ_S * OZ_SLICE;        // one operand's stage (20 KiB)
 constexpr int OZ_NQ = 2;                         // 32-column blocks per wave (8 waves)
+// Stage buffers of 40 KiB.  Four (all 160 KiB of LDS, copies three stages
+// ahead) measured no faster than three on the (H2O)32 B formation.
+#ifndef OZ_NBUF
+#define OZ_NBUF 3
+#endif
 constexpr int64_t OZ_KMAX = 75000;
 
 typedef int oz_i4 __attribute__((ext_vector_type(4)));
@@ -209,40 +215,44 @@ __device__ inline int64_t oz_xcd_logical(int64_t per) {
 // wave w owns 32 rows and NQ 32-column blocks, with five int32 diagonal
 // accumulators per block (80 NQ registers).  NQ = 2 (8 waves of up to 256
 // registers) is used: with NQ = 1 (16 waves of 128) the accumulators and
-// fragments spill.  Stage buffers: three of 40 KiB
-// (A slices, then B slices), copies two stages ahead; the 40 1 KiB copies of
-// a stage are dealt out to the waves in turn.  With a_lower the K range of
-// row tile mt stops at 128 (mt + 1).  Tiles are ordered row tiles fastest,
-// deepest first, so that consecutive workgroups of one XCD share the column
-// tile.
+// fragments spill.  Stage buffers: OZ_NBUF of 40 KiB (A slices, then B
+// slices), copies OZ_NBUF - 1 stages ahead; the 40 1 KiB copies of a stage
+// are dealt out to the waves in turn.  With a_lower the K range of
+// row tile mt stops at 128 (mt + 1).  Tiles come in groups of GM x GN (about
+// the 32 workgroups one XCD runs at once; row tiles fastest within a group,
+// groups of one column block consecutive, deepest row tiles first), so that
+// concurrent workgroups share both operands' panels in L2.
 template <int NQ>
 __global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __restrict__ Ap,
                                                                const int8_t* __restrict__ Bp,
                                                                const double* __restrict__ sig,
                                                                const double* __restrict__ tau,
                                                                double* __restrict__ C, int64_t ldc, int64_t M,
-                                                               int64_t N, int64_t Ks, int nMT, int64_t nT,
-                                                               int a_lower, double alpha, double beta, int64_t per) {
+                                                               int64_t N, int64_t Ks, int nMT, int64_t nNT,
+                                                               int GM, int GN, int64_t nT, int a_lower, double alpha,
+                                                               double beta, int64_t per) {
     constexpr int W = 16 / NQ, WC = 4 / NQ;                 // waves, wave columns
     constexpr int NCH = 2 * OZ_PANEL / 1024;                 // 1 KiB copies per stage (40)
-    constexpr int NCMAX = (NCH + W - 1) / W;
-    __shared__ __attribute__((aligned(16))) int8_t L[3][2 * OZ_PANEL];
+    constexpr int NC = NCH / W;                              // copies per wave and stage
+    static_assert(NCH % W == 0, "oz_gemm: copies must divide among the waves");
+    __shared__ __attribute__((aligned(16))) int8_t L[OZ_NBUF][2 * OZ_PANEL];
     const int64_t id = oz_xcd_logical(per);
     if (id >= nT) return;
-    const int mt = nMT - 1 - (int)(id % nMT);
-    const int64_t nt = id / nMT;
+    const int64_t nGM = (nMT + GM - 1) / GM, g = id / (GM * GN), r = id % (GM * GN);
+    const int mt = nMT - 1 - (int)((g % nGM) * GM + r % GM);
+    const int64_t nt = (g / nGM) * GN + r / GM;
+    if (mt < 0 || nt >= nNT) return;   // past the edge of a partial group
     const int64_t Kc = a_lower ? (Ks < 4 * (int64_t)(mt + 1) ? Ks : 4 * (int64_t)(mt + 1)) : Ks;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r32 = lane & 31, kh = lane >> 5;
     const int wr = wave / WC, wc = wave % WC;
 
     // copies: chunks c = wave + W k of the stage's 40 (20 of A, then 20 of B)
-    const int nc = (NCH - wave + W - 1) / W;
-    const int8_t* src[NCMAX];
-    int seg[NCMAX];
+    const int8_t* src[NC];
+    int seg[NC];
 #pragma unroll
-    for (int k = 0; k < NCMAX; ++k) {
-        const int c = wave + W * k < NCH ? wave + W * k : wave;
+    for (int k = 0; k < NC; ++k) {
+        const int c = wave + W * k;
         const int8_t* base = c < OZ_PANEL / 1024 ? Ap + (int64_t)mt * Ks * OZ_PANEL + c * 1024
                                                  : Bp + nt * Ks * OZ_PANEL + (c - OZ_PANEL / 1024) * 1024;
         src[k] = base + lane * 16;
@@ -250,16 +260,9 @@ __global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __r
     }
     auto copy_stage = [&](int buf, int64_t st) {
 #pragma unroll
-        for (int k = 0; k < NCMAX; ++k)
-            if (k < nc)
-                __builtin_amdgcn_global_load_lds((oz_gbl_ptr)(src[k] + st * OZ_PANEL),
-                                                 (oz_lds_ptr)(&L[buf][0] + seg[k]), 16, 0, 0);
-    };
-    // waits for all but this wave's last nc copies (the stage copied two ahead)
-    auto stage_barrier = [&]() {
-        if (NCMAX == 3 && nc == 3) OZ_STAGE_BARRIER(3)
-        else if (nc == 2) OZ_STAGE_BARRIER(2)
-        else OZ_STAGE_BARRIER(NCMAX)
+        for (int k = 0; k < NC; ++k)
+            __builtin_amdgcn_global_load_lds((oz_gbl_ptr)(src[k] + st * OZ_PANEL), (oz_lds_ptr)(&L[buf][0] + seg[k]),
+                                             16, 0, 0);
     };
     const int rowA = wr * 32 + r32;
     const int offA = rowA * 32 + oz_slot(rowA, kh) * 16;
@@ -277,12 +280,15 @@ __global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __r
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[d][q][r] = 0;
 
-    copy_stage(0, 0);
-    copy_stage(1, Kc > 1 ? 1 : Kc - 1);
-    stage_barrier();
+    // stage s lives in buffer s % OZ_NBUF; copies run OZ_NBUF - 1 stages ahead
+    // (past the end they repeat stage Kc - 1), and the barrier closing stage s
+    // waits until this wave's copies of stage s + 1 have landed.
+#pragma unroll
+    for (int p = 0; p < OZ_NBUF - 1; ++p) copy_stage(p, p < Kc ? p : Kc - 1);
+    OZ_STAGE_BARRIER(NC * (OZ_NBUF - 2))
     int buf = 0;
     for (int64_t s = 0; s < Kc; ++s) {
-        copy_stage(buf == 0 ? 2 : buf - 1, s + 2 < Kc ? s + 2 : Kc - 1);
+        copy_stage(buf == 0 ? OZ_NBUF - 1 : buf - 1, s + OZ_NBUF - 1 < Kc ? s + OZ_NBUF - 1 : Kc - 1);
         const int8_t* S = &L[buf][0];
         oz_i4 a[OZ_S];
 #pragma unroll
@@ -298,8 +304,8 @@ __global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __r
                 for (int q = 0; q < NQ; ++q)
                     acc[sa + t][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[sa], b[q], acc[sa + t][q], 0, 0, 0);
         }
-        stage_barrier();
-        buf = buf == 2 ? 0 : buf + 1;
+        OZ_STAGE_BARRIER(NC * (OZ_NBUF - 2))
+        buf = buf == OZ_NBUF - 1 ? 0 : buf + 1;
     }
     __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xF << 8));
 
@@ -385,10 +391,12 @@ int gemm_ozaki(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int6
         CORR_TRY(oz_split(ctx, A, lda, M, K, Ks, !transa, sig, Ap));
         CORR_TRY(oz_split(ctx, B, ldb, N, K, Ks, transb != 0, tau, Bp));
     }
-    const int64_t nT = nMT * nNT, per = cdiv(nT, 8);
+    // groups of about 32 tiles: up to 8 column tiles, the rest in rows
+    const int GN = (int)std::min<int64_t>(nNT, 8), GM = std::max(1, 32 / GN);
+    const int64_t nT = cdiv(nMT, GM) * GM * cdiv(nNT, GN) * GN, per = cdiv(nT, 8);
     KernelTimer kt(ctx, "oz_gemm");
     hipLaunchKernelGGL(oz_gemm_kernel<OZ_NQ>, dim3((unsigned)(8 * per)), dim3(64 * 16 / OZ_NQ), 0, ctx->stream, Ap,
-                       Bp, sig, tau, C, ldc, M, N, Ks, (int)nMT, nT, a_lower ? 1 : 0, alpha, beta, per);
+                       Bp, sig, tau, C, ldc, M, N, Ks, (int)nMT, nNT, GM, GN, nT, a_lower ? 1 : 0, alpha, beta, per);
     CORR_LAUNCHED(ctx, "oz_gemm_kernel");
     return CORR_OK;
 }
