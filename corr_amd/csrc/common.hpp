@@ -83,6 +83,22 @@ int gemm_ozaki(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int6
                const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
                bool a_lower);
 
+// The same in two steps, so that each operand can be sliced on its own
+// stream as soon as it exists: ozaki_prepare cuts the R x K operand X (rows
+// of op(A), or columns of op(B); kmajor: element (r, k) at X[r ld + k], else
+// at X[k ld + r]) into the context scratch buffers named after `name`;
+// ozaki_gemm_prepared multiplies a prepared A (M = A.R rows) and B (N = B.R
+// columns) on ctx->stream, after both preparations.
+struct OzOperand {
+    int8_t* slices = nullptr;
+    double* scales = nullptr;
+    int64_t R = 0, K = 0, Ks = 0;
+};
+int ozaki_prepare(corr_ctx* ctx, const char* name, const double* X, int64_t ld, int64_t R, int64_t K, bool kmajor,
+                  OzOperand* out);
+int ozaki_gemm_prepared(corr_ctx* ctx, const OzOperand& A, const OzOperand& B, double alpha, double beta, double* C,
+                        int64_t ldc, bool a_lower);
+
 // op(A) op(B) (M x N, alpha 1) written through an output mapping of the
 // 4-index transform (gemm.hip): mapping 1 (PAIRS): row X n + i, column j ->
 // C[X P + tri(i, j)] for j >= i, P = n (n + 1) / 2; mapping 2 (MIRROR): row

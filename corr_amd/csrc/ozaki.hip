@@ -370,6 +370,40 @@ static int oz_split(corr_ctx* ctx, const double* X, int64_t ld, int64_t R, int64
     return CORR_OK;
 }
 
+int ozaki_prepare(corr_ctx* ctx, const char* name, const double* X, int64_t ld, int64_t R, int64_t K,
+                  bool kmajor, OzOperand* out) {
+    CORR_REQUIRE(ctx, R > 0 && K >= 0 && X && out && ld >= (kmajor ? K : R), "int8x5 operand: bad arguments");
+    CORR_REQUIRE(ctx, K <= OZ_KMAX, "gemm int8x5: K above 75000 (int32 range of the slice sums)");
+    const int64_t nRT = cdiv(R, OZ_T), Ks = cdiv(K > 0 ? K : 1, OZ_KS);
+    CORR_REQUIRE(ctx, nRT < (1 << 30), "gemm int8x5: too many tiles");
+    const std::string n(name);
+    int8_t* P = nullptr;
+    double* sc = nullptr;
+    CORR_TRY(scratch(ctx, (n + "_slices").c_str(), (size_t)nRT * Ks * OZ_PANEL, (void**)&P));
+    CORR_TRY(scratch(ctx, (n + "_scales").c_str(), sizeof(double) * nRT * OZ_T, (void**)&sc));
+    KernelTimer kt(ctx, "oz_split");
+    CORR_TRY(oz_split(ctx, X, ld, R, K, Ks, kmajor, sc, P));
+    *out = OzOperand{P, sc, R, K, Ks};
+    return CORR_OK;
+}
+
+int ozaki_gemm_prepared(corr_ctx* ctx, const OzOperand& A, const OzOperand& B, double alpha, double beta, double* C,
+                        int64_t ldc, bool a_lower) {
+    CORR_REQUIRE(ctx, A.slices && B.slices && C && A.K == B.K && ldc >= B.R, "gemm int8x5: operands do not match");
+    const int64_t M = A.R, N = B.R, Ks = A.Ks;
+    const int64_t nMT = cdiv(M, OZ_T), nNT = cdiv(N, OZ_T);
+    CORR_REQUIRE(ctx, nMT * nNT < ((int64_t)1 << 40), "gemm int8x5: too many tiles");
+    // groups of about 32 tiles: up to 8 column tiles, the rest in rows
+    const int GN = (int)std::min<int64_t>(nNT, 8), GM = std::max(1, 32 / GN);
+    const int64_t nT = cdiv(nMT, GM) * GM * cdiv(nNT, GN) * GN, per = cdiv(nT, 8);
+    KernelTimer kt(ctx, "oz_gemm");
+    hipLaunchKernelGGL(oz_gemm_kernel<OZ_NQ>, dim3((unsigned)(8 * per)), dim3(64 * 16 / OZ_NQ), 0, ctx->stream,
+                       A.slices, B.slices, A.scales, B.scales, C, ldc, M, N, Ks, (int)nMT, nNT, GM, GN, nT,
+                       a_lower ? 1 : 0, alpha, beta, per);
+    CORR_LAUNCHED(ctx, "oz_gemm_kernel");
+    return CORR_OK;
+}
+
 int gemm_ozaki(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int64_t K, double alpha,
                const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
                bool a_lower) {
@@ -378,27 +412,10 @@ int gemm_ozaki(corr_ctx* ctx, int transa, int transb, int64_t M, int64_t N, int6
     if (M == 0 || N == 0) return CORR_OK;
     CORR_REQUIRE(ctx, A && B && C && lda >= (transa ? M : K) && ldb >= (transb ? K : N) && ldc >= N,
                  "gemm int8x5: bad operands");
-    const int64_t nMT = cdiv(M, OZ_T), nNT = cdiv(N, OZ_T), Ks = cdiv(K > 0 ? K : 1, OZ_KS);
-    CORR_REQUIRE(ctx, nMT < (1 << 30) && nMT * nNT < ((int64_t)1 << 40), "gemm int8x5: too many tiles");
-    int8_t *Ap = nullptr, *Bp = nullptr;
-    double *sig = nullptr, *tau = nullptr;
-    CORR_TRY(scratch(ctx, "oz_Ap", (size_t)nMT * Ks * OZ_PANEL, (void**)&Ap));
-    CORR_TRY(scratch(ctx, "oz_Bp", (size_t)nNT * Ks * OZ_PANEL, (void**)&Bp));
-    CORR_TRY(scratch(ctx, "oz_sig", sizeof(double) * nMT * OZ_T, (void**)&sig));
-    CORR_TRY(scratch(ctx, "oz_tau", sizeof(double) * nNT * OZ_T, (void**)&tau));
-    {
-        KernelTimer kt(ctx, "oz_split");
-        CORR_TRY(oz_split(ctx, A, lda, M, K, Ks, !transa, sig, Ap));
-        CORR_TRY(oz_split(ctx, B, ldb, N, K, Ks, transb != 0, tau, Bp));
-    }
-    // groups of about 32 tiles: up to 8 column tiles, the rest in rows
-    const int GN = (int)std::min<int64_t>(nNT, 8), GM = std::max(1, 32 / GN);
-    const int64_t nT = cdiv(nMT, GM) * GM * cdiv(nNT, GN) * GN, per = cdiv(nT, 8);
-    KernelTimer kt(ctx, "oz_gemm");
-    hipLaunchKernelGGL(oz_gemm_kernel<OZ_NQ>, dim3((unsigned)(8 * per)), dim3(64 * 16 / OZ_NQ), 0, ctx->stream, Ap,
-                       Bp, sig, tau, C, ldc, M, N, Ks, (int)nMT, nNT, GM, GN, nT, a_lower ? 1 : 0, alpha, beta, per);
-    CORR_LAUNCHED(ctx, "oz_gemm_kernel");
-    return CORR_OK;
+    OzOperand oa, ob;
+    CORR_TRY(ozaki_prepare(ctx, "oz_A", A, lda, M, K, !transa, &oa));
+    CORR_TRY(ozaki_prepare(ctx, "oz_B", B, ldb, N, K, transb != 0, &ob));
+    return ozaki_gemm_prepared(ctx, oa, ob, alpha, beta, C, ldc, a_lower);
 }
 
 }  // namespace corr

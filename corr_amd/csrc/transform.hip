@@ -446,7 +446,10 @@ static int overlap_streams(corr_ctx* ctx) {
 // metric runs on the context's auxiliary stream and the AO->MO GEMMs beside
 // it (on the CUs the auxiliary stream leaves free, when the CUs are split),
 // both ordered after the caller's prior work by an event; the metric GEMM
-// runs on the caller's stream once both have finished.
+// runs on the caller's stream once both have finished.  When that GEMM runs
+// on int8 MFMA (the mixed path), each stream slices its own operand first
+// ((P|ia) after the AO->MO GEMMs, L^-1 after the chain), so the slicing of
+// (P|ia) overlaps the end of the chain.
 int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Pmn,
                   const double* J, int mode, double* M, double* Bia, int prec, int64_t* nkept) {
     CORR_REQUIRE(ctx, naux > 0 && N > 0 && o > 0 && o < N && C && Pmn && J && Bia,
@@ -462,6 +465,8 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
     if (!ctx->ev_join) CORR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     if (!ctx->ev_part) CORR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_part, hipEventDisableTiming));
     hipStream_t main = ctx->stream;
+    const bool int8m = metric_gemm_prec(mode, prec) == CORR_PREC_INT8X5;
+    OzOperand opA, opB;
     if (mode == CORR_METRIC_EIGEN && ctx->aux_masked) {
         // the eigensolver wants the whole device: no overlap
         CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, prec));
@@ -475,6 +480,7 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
         CORR_HIP(ctx, hipStreamWaitEvent(ctx->part_stream, ctx->ev_fork, 0));
         ctx->stream = ctx->part_stream;
         rc = ao2ov_sym(ctx, naux, N, o, C, Pmn, X, prec);
+        if (rc == CORR_OK && int8m) rc = ozaki_prepare(ctx, "oz_metric_X", X, o * v, o * v, naux, false, &opB);
         if (rc == CORR_OK && hipEventRecord(ctx->ev_part, ctx->part_stream) != hipSuccess)
             rc = set_error(ctx, CORR_EHIP, "ri_bia_from_j: event record failed");
         ctx->stream = main;
@@ -484,10 +490,12 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
         }
     } else {
         CORR_TRY(ao2ov_sym(ctx, naux, N, o, C, Pmn, X, prec));
+        if (int8m) CORR_TRY(ozaki_prepare(ctx, "oz_metric_X", X, o * v, o * v, naux, false, &opB));
     }
     CORR_HIP(ctx, hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
     ctx->stream = ctx->aux_stream;
     rc = ri_metric(ctx, naux, J, mode, M, nkept);
+    if (rc == CORR_OK && int8m) rc = ozaki_prepare(ctx, "oz_metric_L", M, naux, naux, naux, true, &opA);
     if (rc == CORR_OK && hipEventRecord(ctx->ev_join, ctx->aux_stream) != hipSuccess)
         rc = set_error(ctx, CORR_EHIP, "ri_bia_from_j: event record failed");
     ctx->stream = main;
@@ -498,6 +506,7 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
     }
     if (ctx->aux_masked) CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_part, 0));
     CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_join, 0));
+    if (int8m) return ozaki_gemm_prepared(ctx, opA, opB, 1.0, 0.0, Bia, o * v, true);
     return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1,
                 metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY ? GEMM_TRI_A_LOWER : 0);
 }
