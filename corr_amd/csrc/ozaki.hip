@@ -42,10 +42,17 @@ constexpr int OZ_KS = 32;                        // k per stage
 constexpr int OZ_SLICE = OZ_T * OZ_KS;           // bytes of one slice of a stage (4 KiB)
 constexpr int OZ_PANEL = OZ_S * OZ_SLICE;        // one operand's stage (20 KiB)
 constexpr int OZ_NQ = 2;                         // 32-column blocks per wave (8 waves)
-// Stage buffers of 40 KiB.  Four (all 160 KiB of LDS, copies three stages
-// ahead) measured no faster than three on the (H2O)32 B formation.
+// Stage buffers of OZ_KD x 40 KiB (OZ_KD 32-deep sub-stages per barrier).
+// Measured on the (H2O)32 B formation against three buffers of one
+// sub-stage: four buffers (all 160 KiB of LDS, copies three stages ahead) no
+// faster, two of two sub-stages (half the barriers, copies one stage ahead)
+// 1% slower; fragments read one slice group ahead of their MFMAs (pinned by
+// scheduling barriers) no faster either.
 #ifndef OZ_NBUF
 #define OZ_NBUF 3
+#endif
+#ifndef OZ_KD
+#define OZ_KD 1
 #endif
 constexpr int64_t OZ_KMAX = 75000;
 
@@ -235,7 +242,7 @@ __global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __r
     constexpr int NCH = 2 * OZ_PANEL / 1024;                 // 1 KiB copies per stage (40)
     constexpr int NC = NCH / W;                              // copies per wave and stage
     static_assert(NCH % W == 0, "oz_gemm: copies must divide among the waves");
-    __shared__ __attribute__((aligned(16))) int8_t L[OZ_NBUF][2 * OZ_PANEL];
+    __shared__ __attribute__((aligned(16))) int8_t L[OZ_NBUF][OZ_KD][2 * OZ_PANEL];
     const int64_t id = oz_xcd_logical(per);
     if (id >= nT) return;
     const int64_t nGM = (nMT + GM - 1) / GM, g = id / (GM * GN), r = id % (GM * GN);
@@ -258,12 +265,6 @@ __global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __r
         src[k] = base + lane * 16;
         seg[k] = c * 1024;
     }
-    auto copy_stage = [&](int buf, int64_t st) {
-#pragma unroll
-        for (int k = 0; k < NC; ++k)
-            __builtin_amdgcn_global_load_lds((oz_gbl_ptr)(src[k] + st * OZ_PANEL), (oz_lds_ptr)(&L[buf][0] + seg[k]),
-                                             16, 0, 0);
-    };
     const int rowA = wr * 32 + r32;
     const int offA = rowA * 32 + oz_slot(rowA, kh) * 16;
     int offB[NQ];
@@ -280,31 +281,48 @@ __global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __r
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[d][q][r] = 0;
 
-    // stage s lives in buffer s % OZ_NBUF; copies run OZ_NBUF - 1 stages ahead
-    // (past the end they repeat stage Kc - 1), and the barrier closing stage s
-    // waits until this wave's copies of stage s + 1 have landed.
+    // A stage is OZ_KD 32-deep sub-stages; stage j lives in buffer j % OZ_NBUF,
+    // copies run OZ_NBUF - 1 stages ahead (sub-stages past the end repeat
+    // sub-stage Kc - 1 and are not computed), and the barrier closing stage j
+    // waits until this wave's copies of stage j + 1 have landed.
+    const int64_t Sc = (Kc + OZ_KD - 1) / OZ_KD;
+    auto copy_stage = [&](int b, int64_t j) {
 #pragma unroll
-    for (int p = 0; p < OZ_NBUF - 1; ++p) copy_stage(p, p < Kc ? p : Kc - 1);
-    OZ_STAGE_BARRIER(NC * (OZ_NBUF - 2))
-    int buf = 0;
-    for (int64_t s = 0; s < Kc; ++s) {
-        copy_stage(buf == 0 ? OZ_NBUF - 1 : buf - 1, s + OZ_NBUF - 1 < Kc ? s + OZ_NBUF - 1 : Kc - 1);
-        const int8_t* S = &L[buf][0];
-        oz_i4 a[OZ_S];
+        for (int h = 0; h < OZ_KD; ++h) {
+            const int64_t st = j * OZ_KD + h < Kc ? j * OZ_KD + h : Kc - 1;
 #pragma unroll
-        for (int sa = 0; sa < OZ_S; ++sa) a[sa] = *reinterpret_cast<const oz_i4*>(S + sa * OZ_SLICE + offA);
-#pragma unroll
-        for (int t = 0; t < OZ_S; ++t) {
-            oz_i4 b[NQ];
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) b[q] = *reinterpret_cast<const oz_i4*>(S + t * OZ_SLICE + offB[q]);
-#pragma unroll
-            for (int sa = 0; sa + t < OZ_S; ++sa)
-#pragma unroll
-                for (int q = 0; q < NQ; ++q)
-                    acc[sa + t][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[sa], b[q], acc[sa + t][q], 0, 0, 0);
+            for (int k = 0; k < NC; ++k)
+                __builtin_amdgcn_global_load_lds((oz_gbl_ptr)(src[k] + st * OZ_PANEL),
+                                                 (oz_lds_ptr)(&L[b][h][0] + seg[k]), 16, 0, 0);
         }
-        OZ_STAGE_BARRIER(NC * (OZ_NBUF - 2))
+    };
+#pragma unroll
+    for (int p = 0; p < OZ_NBUF - 1; ++p) copy_stage(p, p < Sc ? p : Sc - 1);
+    OZ_STAGE_BARRIER(NC * OZ_KD * (OZ_NBUF - 2))
+    int buf = 0;
+    for (int64_t j = 0; j < Sc; ++j) {
+        copy_stage(buf == 0 ? OZ_NBUF - 1 : buf - 1, j + OZ_NBUF - 1 < Sc ? j + OZ_NBUF - 1 : Sc - 1);
+#pragma unroll
+        for (int h = 0; h < OZ_KD; ++h) {
+            if (OZ_KD > 1 && j * OZ_KD + h >= Kc) break;
+            const int8_t* S = &L[buf][h][0];
+            oz_i4 a[OZ_S];
+#pragma unroll
+            for (int sa = 0; sa < OZ_S; ++sa) a[sa] = *reinterpret_cast<const oz_i4*>(S + sa * OZ_SLICE + offA);
+#pragma unroll
+            for (int t = 0; t < OZ_S; ++t) {
+                oz_i4 b[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) b[q] = *reinterpret_cast<const oz_i4*>(S + t * OZ_SLICE + offB[q]);
+#pragma unroll
+                for (int sa = 0; sa + t < OZ_S; ++sa)
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q)
+                        acc[sa + t][q] =
+                            __builtin_amdgcn_mfma_i32_32x32x32_i8(a[sa], b[q], acc[sa + t][q], 0, 0, 0);
+            }
+        }
+        OZ_STAGE_BARRIER(NC * OZ_KD * (OZ_NBUF - 2))
         buf = buf == OZ_NBUF - 1 ? 0 : buf + 1;
     }
     __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xF << 8));
