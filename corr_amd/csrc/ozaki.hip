@@ -393,7 +393,7 @@ int ozaki_prepare(corr_ctx* ctx, const char* name, const double* X, int64_t ld, 
     CORR_REQUIRE(ctx, R > 0 && K >= 0 && X && out && ld >= (kmajor ? K : R), "int8x5 operand: bad arguments");
     CORR_REQUIRE(ctx, K <= OZ_KMAX, "gemm int8x5: K above 75000 (int32 range of the slice sums)");
     const int64_t nRT = cdiv(R, OZ_T), Ks = cdiv(K > 0 ? K : 1, OZ_KS);
-    CORR_REQUIRE(ctx, nRT < (1 << 30), "gemm int8x5: too many tiles");
+    CORR_REQUIRE(ctx, nRT * Ks < ((int64_t)1 << 31), "gemm int8x5: operand too large for one slicing launch");
     const std::string n(name);
     int8_t* P = nullptr;
     double* sc = nullptr;
@@ -410,10 +410,10 @@ int ozaki_gemm_prepared(corr_ctx* ctx, const OzOperand& A, const OzOperand& B, d
     CORR_REQUIRE(ctx, A.slices && B.slices && C && A.K == B.K && ldc >= B.R, "gemm int8x5: operands do not match");
     const int64_t M = A.R, N = B.R, Ks = A.Ks;
     const int64_t nMT = cdiv(M, OZ_T), nNT = cdiv(N, OZ_T);
-    CORR_REQUIRE(ctx, nMT * nNT < ((int64_t)1 << 40), "gemm int8x5: too many tiles");
     // groups of about 32 tiles: up to 8 column tiles, the rest in rows
     const int GN = (int)std::min<int64_t>(nNT, 8), GM = std::max(1, 32 / GN);
     const int64_t nT = cdiv(nMT, GM) * GM * cdiv(nNT, GN) * GN, per = cdiv(nT, 8);
+    CORR_REQUIRE(ctx, nMT < (1 << 30) && 8 * per < ((int64_t)1 << 31), "gemm int8x5: too many tiles for one launch");
     KernelTimer kt(ctx, "oz_gemm");
     hipLaunchKernelGGL(oz_gemm_kernel<OZ_NQ>, dim3((unsigned)(8 * per)), dim3(64 * 16 / OZ_NQ), 0, ctx->stream,
                        A.slices, B.slices, A.scales, B.scales, C, ldc, M, N, Ks, (int)nMT, nNT, GM, GN, nT,
