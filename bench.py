@@ -4,7 +4,9 @@
 Workload (one "step"): B formation from the metric J and the 3-index
 integrals (P|mu nu) already resident in HBM (Cholesky metric, overlapped on a
 second stream with the AO->MO GEMMs, then the metric GEMM), then the fused
-RI-MP2 pair-block energy.  Sizes are those of
+RI-MP2 pair-block energy, in one call (corr_ri_mp2_energy_from_j; on the
+mixed path the metric GEMM writes B as the energy kernel's BF16 hi/lo
+panels, so no FP64 B is stored or repacked).  Sizes are those of
 (H2O)32: N = 768 basis functions, o = 160 occupied, v = 608 virtual,
 naux = 2432 auxiliary functions.  Three paths are timed, K steps each:
 
@@ -206,21 +208,22 @@ def main() -> int:
                    "emp2_fp64_cholesky": e_chol, "eigen_minus_cholesky_Eh": e_eig - e_chol}
     log(f"metric: eigen keeps {kept} of {naux}; FP64 EMP2 eigen - cholesky = {e_eig - e_chol:.3e} Eh")
 
-    def step(prec_gemm: str, prec_e: str):
-        # metric factorisation overlapped with the AO->MO GEMMs, then the energy
-        B, _ = corr_amd.ri_bia_from_j(C, Pmn, J, o, "cholesky", prec_gemm)
-        return corr_amd.rimp2_energy(B, eo, ev, prec_e)
+    def step(prec: str):
+        # metric factorisation overlapped with the AO->MO GEMMs, then the
+        # energy; at bf16x3 the metric product writes the energy kernel's
+        # BF16 hi/lo panels directly
+        return corr_amd.ri_mp2_energy_from_j(C, Pmn, J, o, eo, ev, "cholesky", prec)[0]
 
     results = {}
-    for name, pg, pe in (("fp64", "fp64", "fp64"), ("mixed", "bf16x3", "bf16x3"), ("fp32", "fp32", "fp32")):
+    for name, prec in (("fp64", "fp64"), ("mixed", "bf16x3"), ("fp32", "fp32")):
         for _ in range(args.warmup):
-            step(pg, pe)
+            step(prec)
         torch.cuda.synchronize()
         ctx.kernel_time(None)
         ctx.enable_timing(True)
         t0 = time.perf_counter()
         for k in range(args.steps):
-            e = step(pg, pe)
+            e = step(prec)
         torch.cuda.synchronize()
         t = (time.perf_counter() - t0) / args.steps
         ms, n = ctx.kernel_time("rimp2_energy")

@@ -6,6 +6,7 @@ the C-ABI of include/corr_abi.h); this package is the PyTorch-ROCm front-end.
 from ._abi import (CorrError, CorrLibraryError, LIB_PATH, PREC_BF16X3, PREC_FP32, PREC_FP64,  # noqa: F401
                    lib)
 from .posthf import (MOtrans, ao_unfold, build_FMo, calculate_ri, dgemm, ri_bia_from_j, ri_form_bia,  # noqa: F401
+                     ri_mp2_energy_from_j,
                      ri_metric, rimp2, rimp2_energy, rimp2_gradient, run_canonical_mp2, run_canonical_mp2_ri, run_non_canonical_mp2,
                      transform_ri, RIMP2Energy)
 

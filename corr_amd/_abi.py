@@ -123,6 +123,7 @@ SIGNATURES = {
     "corr_transform_ri": [P, I64, I64, I64, P, P, P, I],
     "corr_ri_form_bia": [P, I64, I64, I64, P, P, P, I, P, I],
     "corr_ri_bia_from_j": [P, I64, I64, I64, P, P, P, I, P, P, I, P],
+    "corr_ri_mp2_energy_from_j": [P, I64, I64, I64, P, P, P, I, P, P, I, C.POINTER(MP2Energy), P],
     "corr_rimp2_energy": [P, I64, I64, I64, P, P, P, I, C.POINTER(MP2Energy)],
     "corr_rimp2_energy_noalloc": [P, I64, I64, I64, P, P, P, I, C.POINTER(MP2Energy)],
     "corr_rimp2_gradient": [P, I64, I64, I64, P, P, P, D, D, I, P, P, P, C.POINTER(MP2Energy)],

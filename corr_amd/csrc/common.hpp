@@ -96,8 +96,20 @@ struct OzOperand {
 };
 int ozaki_prepare(corr_ctx* ctx, const char* name, const double* X, int64_t ld, int64_t R, int64_t K, bool kmajor,
                   OzOperand* out);
+
+// The BF16x3 RI-MP2 energy kernel's panels (rimp2.hip) as a GEMM output:
+// element (Q, n = i v + a) goes, as hi = bf16(x) and lo = bf16(x - hi), to
+// panel (i / 4, a / 32), 16-deep stage Q / 16, row (i % 4) 32 + a % 32, in the
+// energy kernels' LDS order; Kp rows per panel (Ks = Kp / 16 stages).
+struct Bf16Panels {
+    uint16_t* p = nullptr;
+    int o = 0, v = 0, nAC = 0;
+    int64_t Kp = 0, Ks = 0;
+};
+// pan != nullptr: C is not written; the product goes to the panels (alpha 1,
+// beta 0, rows up to Kp, the pad rows Q >= M as zeros).
 int ozaki_gemm_prepared(corr_ctx* ctx, const OzOperand& A, const OzOperand& B, double alpha, double beta, double* C,
-                        int64_t ldc, bool a_lower);
+                        int64_t ldc, bool a_lower, const Bf16Panels* pan = nullptr);
 
 // op(A) op(B) (M x N, alpha 1) written through an output mapping of the
 // 4-index transform (gemm.hip): mapping 1 (PAIRS): row X n + i, column j ->
@@ -145,6 +157,15 @@ int64_t rimp2_workspace(int64_t naux, int64_t nocc, int64_t nvir, int prec, void
 // The energy on a caller-provided workspace (no allocation, no kernel timing).
 int rimp2_energy_with(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
                       const double* eo, const double* ev, int prec, corr_mp2_energy* out, const RimpWs* ws);
+
+// The BF16x3 energy's panels in the context scratch as a producer's target
+// (pad rows past naux and, for ragged o or v, the empty panel rows zeroed),
+// and the energy on them as they stand (rimp2.hip).
+int rimp2_bf16_panels(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, Bf16Panels* out);
+int rimp2_energy_prepacked(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* eo,
+                           const double* ev, corr_mp2_energy* out);
+int rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia, const double* eo,
+                 const double* ev, int prec, corr_mp2_energy* out);
 
 // Pair-tile part of the RI-MP2 gradient (rimp2.hip): the symmetric matrix R
 // (ov x ov doubles at R, every element written), dE/deps and the energies.

@@ -229,7 +229,19 @@ __device__ inline int64_t oz_xcd_logical(int64_t per) {
 // the 32 workgroups one XCD runs at once; row tiles fastest within a group,
 // groups of one column block consecutive, deepest row tiles first), so that
 // concurrent workgroups share both operands' panels in L2.
-template <int NQ>
+// BF16 hi/lo of a double for the energy kernel's panels: the conversion of the
+// RI-MP2 repack (round-to-nearest-even of the FP32 value; lo from x - hi).
+__device__ inline uint16_t oz_f32_to_bf16_rn(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ inline void oz_bf16_split(double x, uint16_t& hi, uint16_t& lo) {
+    hi = oz_f32_to_bf16_rn((float)x);
+    lo = oz_f32_to_bf16_rn((float)(x - (double)__uint_as_float(((uint32_t)hi) << 16)));
+}
+
+template <int NQ, bool PAN>
 __global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __restrict__ Ap,
                                                                const int8_t* __restrict__ Bp,
                                                                const double* __restrict__ sig,
@@ -237,7 +249,7 @@ __global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __r
                                                                double* __restrict__ C, int64_t ldc, int64_t M,
                                                                int64_t N, int64_t Ks, int nMT, int64_t nNT,
                                                                int GM, int GN, int64_t nT, int a_lower, double alpha,
-                                                               double beta, int64_t per) {
+                                                               double beta, int64_t per, Bf16Panels pan) {
     constexpr int W = 16 / NQ, WC = 4 / NQ;                 // waves, wave columns
     constexpr int NCH = 2 * OZ_PANEL / 1024;                 // 1 KiB copies per stage (40)
     constexpr int NC = NCH / W;                              // copies per wave and stage
@@ -363,8 +375,47 @@ __global__ __launch_bounds__(64 * 16 / NQ) void oz_gemm_kernel(const int8_t* __r
             }
         }
     };
-    if (beta == 0.0) store(std::false_type{});
-    else store(std::true_type{});
+    if constexpr (PAN) {
+        // groups of four rows (4 k of one 16-byte chunk) per column: an
+        // 8-byte hi store and an 8-byte lo store
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int64_t n = nt * OZ_T + (NQ * wc + q) * 32 + r32;
+            if (n >= N) continue;
+            const int i = (int)(n / pan.v), a = (int)(n % pan.v);
+            const int c = (i % 4) * 32 + a % 32;
+            const int64_t blk = (int64_t)(i / 4) * pan.nAC + a / 32;
+#pragma unroll
+            for (int gq = 0; gq < 4; ++gq) {
+                const int64_t Q0 = m0 + 8 * gq;
+                if (Q0 >= pan.Kp) continue;
+                uint16_t hv[4], lv[4];
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int r = 4 * gq + rr;
+                    double h = (double)acc[4][q][r];
+                    h = h * R7 + (double)acc[3][q][r];
+                    h = h * R7 + (double)acc[2][q][r];
+                    h = h * R7 + (double)acc[1][q][r];
+                    h = h * R7 + (double)acc[0][q][r];
+                    const double val = Q0 + rr < M ? (h * R14) * sg[r] * tn[q] : 0.0;
+                    oz_bf16_split(val, hv[rr], lv[rr]);
+                }
+                const int ch = (int)((Q0 % 16) / 8), slot = ch ^ ((c >> 3) & 1);
+                uint16_t* dst = pan.p + (blk * pan.Ks + Q0 / 16) * (2 * OZ_T * 16) + c * 16 + slot * 8 + Q0 % 8;
+                uint2 wh, wl;
+                wh.x = hv[0] | ((uint32_t)hv[1] << 16);
+                wh.y = hv[2] | ((uint32_t)hv[3] << 16);
+                wl.x = lv[0] | ((uint32_t)lv[1] << 16);
+                wl.y = lv[2] | ((uint32_t)lv[3] << 16);
+                *reinterpret_cast<uint2*>(dst) = wh;
+                *reinterpret_cast<uint2*>(dst + OZ_T * 16) = wl;
+            }
+        }
+    } else {
+        if (beta == 0.0) store(std::false_type{});
+        else store(std::true_type{});
+    }
 }
 
 }  // namespace
@@ -406,8 +457,12 @@ int ozaki_prepare(corr_ctx* ctx, const char* name, const double* X, int64_t ld, 
 }
 
 int ozaki_gemm_prepared(corr_ctx* ctx, const OzOperand& A, const OzOperand& B, double alpha, double beta, double* C,
-                        int64_t ldc, bool a_lower) {
-    CORR_REQUIRE(ctx, A.slices && B.slices && C && A.K == B.K && ldc >= B.R, "gemm int8x5: operands do not match");
+                        int64_t ldc, bool a_lower, const Bf16Panels* pan) {
+    CORR_REQUIRE(ctx, A.slices && B.slices && A.K == B.K, "gemm int8x5: operands do not match");
+    CORR_REQUIRE(ctx, pan ? (pan->p && alpha == 1.0 && beta == 0.0 && (int64_t)pan->o * pan->v == B.R &&
+                            pan->Kp % 16 == 0 && pan->Kp >= A.R && pan->Ks == pan->Kp / 16)
+                          : (C && ldc >= B.R),
+                 "gemm int8x5: bad output");
     const int64_t M = A.R, N = B.R, Ks = A.Ks;
     const int64_t nMT = cdiv(M, OZ_T), nNT = cdiv(N, OZ_T);
     // groups of about 32 tiles: up to 8 column tiles, the rest in rows
@@ -415,9 +470,14 @@ int ozaki_gemm_prepared(corr_ctx* ctx, const OzOperand& A, const OzOperand& B, d
     const int64_t nT = cdiv(nMT, GM) * GM * cdiv(nNT, GN) * GN, per = cdiv(nT, 8);
     CORR_REQUIRE(ctx, nMT < (1 << 30) && 8 * per < ((int64_t)1 << 31), "gemm int8x5: too many tiles for one launch");
     KernelTimer kt(ctx, "oz_gemm");
-    hipLaunchKernelGGL(oz_gemm_kernel<OZ_NQ>, dim3((unsigned)(8 * per)), dim3(64 * 16 / OZ_NQ), 0, ctx->stream,
-                       A.slices, B.slices, A.scales, B.scales, C, ldc, M, N, Ks, (int)nMT, nNT, GM, GN, nT,
-                       a_lower ? 1 : 0, alpha, beta, per);
+    if (pan)
+        hipLaunchKernelGGL((oz_gemm_kernel<OZ_NQ, true>), dim3((unsigned)(8 * per)), dim3(64 * 16 / OZ_NQ), 0,
+                           ctx->stream, A.slices, B.slices, A.scales, B.scales, C, ldc, M, N, Ks, (int)nMT, nNT, GM,
+                           GN, nT, a_lower ? 1 : 0, alpha, beta, per, *pan);
+    else
+        hipLaunchKernelGGL((oz_gemm_kernel<OZ_NQ, false>), dim3((unsigned)(8 * per)), dim3(64 * 16 / OZ_NQ), 0,
+                           ctx->stream, A.slices, B.slices, A.scales, B.scales, C, ldc, M, N, Ks, (int)nMT, nNT, GM,
+                           GN, nT, a_lower ? 1 : 0, alpha, beta, per, Bf16Panels{});
     CORR_LAUNCHED(ctx, "oz_gemm_kernel");
     return CORR_OK;
 }

@@ -941,15 +941,25 @@ static int ri_repack(corr_ctx* ctx, const RiLayout& l, int64_t naux, int o, int 
     return CORR_OK;
 }
 
+// Zeroes uint4 runs: run b (of nrun) covers [b * stride + off, + len) uint4.
+__global__ void zero_runs_kernel(uint4* __restrict__ p, int64_t nrun, int64_t stride, int64_t off, int64_t len) {
+    const int64_t total = nrun * len;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
+        p[(e / len) * stride + off + e % len] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // The energy.  ws == nullptr: workspace from the context's cached scratch;
 // otherwise the caller's (sized by rimp2_workspace_bytes).  timed: bracket
-// the energy kernel with events when the context's timing is on.
+// the energy kernel with events when the context's timing is on.  prepacked:
+// the BF16x3 panels in the context scratch already hold B (filled through
+// rimp2_bf16_panels); Bia is not read.
 static int rimp2_energy_impl(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
                              const double* eo, const double* ev, int prec, corr_mp2_energy* out, const RimpWs* ws,
-                             bool timed) {
+                             bool timed, bool prepacked = false) {
     CORR_REQUIRE(ctx, naux > 0 && nocc > 0 && nvir > 0, "rimp2: sizes must be positive");
     CORR_REQUIRE(ctx, nocc < (1 << 20) && nvir < (1 << 20), "rimp2: nocc/nvir too large");
-    CORR_REQUIRE(ctx, Bia && eo && ev && out, "rimp2: null pointer");
+    CORR_REQUIRE(ctx, (Bia || prepacked) && eo && ev && out, "rimp2: null pointer");
+    CORR_REQUIRE(ctx, !prepacked || (prec == CORR_PREC_BF16X3 && !ws), "rimp2: prepacked panels are BF16x3 scratch");
     CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
                  "rimp2: unknown precision");
     const RimpPlan p = rimp2_plan(naux, nocc, nvir, prec);
@@ -968,7 +978,7 @@ static int rimp2_energy_impl(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t 
         CORR_TRY(scratch(ctx, "reduce_out", sizeof(double) * 16, (void**)&w.red_out));
         CORR_TRY(scratch(ctx, "reduce_mid", sizeof(double) * 16 * REDUCE_NCHUNK, (void**)&w.red_mid));
     }
-    CORR_TRY(ri_repack(ctx, l, naux, (int)nocc, (int)nvir, Bia, prec, w.panels));
+    if (!prepacked) CORR_TRY(ri_repack(ctx, l, naux, (int)nocc, (int)nvir, Bia, prec, w.panels));
     if (bf) {
         hipLaunchKernelGGL(wide_tile_list, dim3(1), dim3(1024), 0, ctx->stream, w.tiles, l.nIB, l.nAC, p.wt.nJG,
                            p.wt.nSP, p.wt.NS);
@@ -1002,6 +1012,35 @@ static int rimp2_energy_impl(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t 
 int rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* Bia,
                  const double* eo, const double* ev, int prec, corr_mp2_energy* out) {
     return rimp2_energy_impl(ctx, naux, nocc, nvir, Bia, eo, ev, prec, out, nullptr, true);
+}
+
+int rimp2_bf16_panels(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, Bf16Panels* out) {
+    CORR_REQUIRE(ctx, naux > 0 && nocc > 0 && nvir > 0 && nocc < (1 << 20) && nvir < (1 << 20) && out,
+                 "rimp2 panels: bad arguments");
+    const RimpPlan p = rimp2_plan(naux, nocc, nvir, CORR_PREC_BF16X3);
+    const RiLayout& l = p.l;
+    void* panels = nullptr;
+    CORR_TRY(scratch(ctx, "rimp2_panels", p.b_panels, &panels));
+    const int64_t Ks = l.Kp / 16, nblk = (int64_t)l.nIB * l.nAC, stage = 2 * PW * 16 / 8;   // uint4 per stage
+    // a producer writes rows Q < Kp of the o x v columns; zero the rest: the
+    // stages from naux / 16 on (rows past naux) and, when o or v leave part of
+    // a panel empty, every panel whole
+    const bool ragged = nocc % PI != 0 || nvir % PA != 0;
+    const int64_t ks0 = ragged ? 0 : naux / 16;
+    if (ks0 < Ks) {
+        const int64_t len = (Ks - ks0) * stage;
+        const int64_t grid = std::min<int64_t>(cdiv(nblk * len, 256), 65536);
+        hipLaunchKernelGGL(zero_runs_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, (uint4*)panels, nblk,
+                           Ks * stage, ks0 * stage, len);
+        CORR_LAUNCHED(ctx, "zero_runs_kernel");
+    }
+    *out = Bf16Panels{(uint16_t*)panels, (int)nocc, (int)nvir, l.nAC, l.Kp, Ks};
+    return CORR_OK;
+}
+
+int rimp2_energy_prepacked(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir, const double* eo,
+                           const double* ev, corr_mp2_energy* out) {
+    return rimp2_energy_impl(ctx, naux, nocc, nvir, nullptr, eo, ev, CORR_PREC_BF16X3, out, nullptr, true, true);
 }
 
 // dE/deps from the per-tile partials, in a fixed order: block t < o sums,

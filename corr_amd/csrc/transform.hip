@@ -450,10 +450,15 @@ static int overlap_streams(corr_ctx* ctx) {
 // on int8 MFMA (the mixed path), each stream slices its own operand first
 // ((P|ia) after the AO->MO GEMMs, L^-1 after the chain), so the slicing of
 // (P|ia) overlaps the end of the chain.
-int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Pmn,
-                  const double* J, int mode, double* M, double* Bia, int prec, int64_t* nkept) {
-    CORR_REQUIRE(ctx, naux > 0 && N > 0 && o > 0 && o < N && C && Pmn && J && Bia,
+// pan != nullptr (Cholesky metric, mixed path only): the metric product writes
+// the BF16x3 energy's panels instead of Bia (which may then be null).
+static int ri_bia_from_j_impl(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C,
+                              const double* Pmn, const double* J, int mode, double* M, double* Bia, int prec,
+                              int64_t* nkept, const Bf16Panels* pan) {
+    CORR_REQUIRE(ctx, naux > 0 && N > 0 && o > 0 && o < N && C && Pmn && J && (Bia || pan),
                  "ri_bia_from_j: bad arguments");
+    CORR_REQUIRE(ctx, !pan || metric_gemm_prec(mode, prec) == CORR_PREC_INT8X5,
+                 "ri_bia_from_j: panel output needs the int8 metric product");
     CORR_REQUIRE(ctx, mode == CORR_METRIC_EIGEN || mode == CORR_METRIC_CHOLESKY, "ri_bia_from_j: unknown metric mode");
     CORR_REQUIRE(ctx, J != M, "ri_bia_from_j: J and M must not alias");
     const int64_t v = N - o;
@@ -506,9 +511,35 @@ int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const doubl
     }
     if (ctx->aux_masked) CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_part, 0));
     CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_join, 0));
-    if (int8m) return ozaki_gemm_prepared(ctx, opA, opB, 1.0, 0.0, Bia, o * v, true);
+    if (int8m) return ozaki_gemm_prepared(ctx, opA, opB, 1.0, 0.0, Bia, o * v, true, pan);
     return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1,
                 metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY ? GEMM_TRI_A_LOWER : 0);
+}
+
+int ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Pmn,
+                  const double* J, int mode, double* M, double* Bia, int prec, int64_t* nkept) {
+    return ri_bia_from_j_impl(ctx, naux, N, o, C, Pmn, J, mode, M, Bia, prec, nkept, nullptr);
+}
+
+// RI-MP2 from J and (P|mu nu): B formation as ri_bia_from_j, then the energy.
+// The mixed path with the Cholesky metric never writes B: its int8 metric
+// product fills the BF16x3 energy's panels (the repack and 8 bytes per B
+// element of HBM writes saved); every other combination forms B in scratch.
+int ri_mp2_from_j(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C, const double* Pmn,
+                  const double* J, int mode, const double* eo, const double* ev, int prec, corr_mp2_energy* out,
+                  int64_t* nkept) {
+    CORR_REQUIRE(ctx, naux > 0 && N > 0 && o > 0 && o < N && eo && ev && out, "ri_mp2_from_j: bad arguments");
+    const int64_t v = N - o;
+    if (mode == CORR_METRIC_CHOLESKY && prec == CORR_PREC_BF16X3) {
+        Bf16Panels pan;
+        CORR_TRY(rimp2_bf16_panels(ctx, naux, o, v, &pan));
+        CORR_TRY(ri_bia_from_j_impl(ctx, naux, N, o, C, Pmn, J, mode, nullptr, nullptr, prec, nkept, &pan));
+        return rimp2_energy_prepacked(ctx, naux, o, v, eo, ev, out);
+    }
+    double* B = nullptr;
+    CORR_TRY(scratch(ctx, "ri_mp2_B", sizeof(double) * naux * o * v, (void**)&B));
+    CORR_TRY(ri_bia_from_j(ctx, naux, N, o, C, Pmn, J, mode, nullptr, B, prec, nkept));
+    return rimp2_energy(ctx, naux, o, v, B, eo, ev, prec, out);
 }
 
 }  // namespace corr
@@ -520,6 +551,14 @@ int corr_ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,
                        int prec, int64_t* h_nkept) {
     CORR_CHECK_CTX(ctx);
     return corr::ri_bia_from_j(ctx, naux, nroao, nocc, d_MOs, d_Pmn, d_J, metric_mode, d_M, d_Bia, prec, h_nkept);
+}
+
+int corr_ri_mp2_energy_from_j(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc, const double* d_MOs,
+                              const double* d_Pmn, const double* d_J, int metric_mode, const double* d_eps_occ,
+                              const double* d_eps_vir, int prec, corr_mp2_energy* h_out, int64_t* h_nkept) {
+    CORR_CHECK_CTX(ctx);
+    return corr::ri_mp2_from_j(ctx, naux, nroao, nocc, d_MOs, d_Pmn, d_J, metric_mode, d_eps_occ, d_eps_vir, prec,
+                               h_out, h_nkept);
 }
 
 int corr_build_fmo(corr_ctx* ctx, int64_t nroao, const double* d_MOs, const double* d_Fmat,

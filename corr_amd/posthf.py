@@ -180,6 +180,27 @@ def ri_bia_from_j(MOs: torch.Tensor, Pmn: torch.Tensor, J: torch.Tensor, nocc: i
     return Bia, kept.value
 
 
+def ri_mp2_energy_from_j(MOs: torch.Tensor, Pmn: torch.Tensor, J: torch.Tensor, nocc: int,
+                         eps_occ: torch.Tensor, eps_vir: torch.Tensor, mode: str = "cholesky",
+                         prec: str | int = "fp64") -> tuple[dict, int]:
+    """(energies, kept): B^Q_ia formation from J and (P|mu nu) followed by the
+    RI-MP2 energy, without handing B back (calculate_ri + transform_ri,
+    ri.cc:50-69, 102-103, 147-172, then run_canonical_mp2_ri,
+    can_mp2.cc:53-132).  Same numbers as ri_bia_from_j(..., prec) followed by
+    rimp2_energy(Bia, ..., prec); with mode "cholesky" and prec "bf16x3" the
+    metric product writes B straight into the energy kernel's BF16 hi/lo
+    panels instead of an FP64 B."""
+    _dev(MOs, Pmn, J, eps_occ, eps_vir)
+    naux, N = Pmn.shape[0], Pmn.shape[1]
+    if eps_occ.numel() != nocc or eps_vir.numel() != N - nocc:
+        raise ValueError("orbital energy lengths do not match nocc and the basis size")
+    e = MP2Energy()
+    kept = C.c_int64(0)
+    _ctx(Pmn).call("corr_ri_mp2_energy_from_j", naux, N, nocc, ptr(MOs), ptr(Pmn), ptr(J), _abi.METRICS[mode],
+                   ptr(eps_occ), ptr(eps_vir), _prec(prec), C.byref(e), C.byref(kept))
+    return e.as_dict(), kept.value
+
+
 def rimp2_energy(Bia: torch.Tensor, eps_occ: torch.Tensor, eps_vir: torch.Tensor,
                  prec: str | int = "fp64") -> dict:
     """RI-MP2 energies (run_canonical_mp2_ri, can_mp2.cc:53-132)."""

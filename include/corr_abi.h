@@ -207,6 +207,20 @@ int corr_rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir,
                       const double* d_Bia, const double* d_eps_occ,
                       const double* d_eps_vir, int prec, corr_mp2_energy* h_out);
 
+/* corr_ri_mp2_energy_from_j: corr_ri_bia_from_j followed by corr_rimp2_energy
+ * (calculate_ri + transform_ri + run_canonical_mp2_ri, ri.cc:50-172,
+ * can_mp2.cc:53-132) without handing B back.  With the Cholesky metric and
+ * CORR_PREC_BF16X3 (the mixed path) B is never written out: the int8 metric
+ * product stores it as the BF16 hi/lo panels the energy kernel reads (same
+ * energies, bit for bit, as the two calls).  Other metric modes and
+ * precisions form B in context scratch and call the energy.  h_nkept as in
+ * corr_ri_metric (may be NULL). */
+int corr_ri_mp2_energy_from_j(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,
+                              const double* d_MOs, const double* d_Pmn, const double* d_J,
+                              int metric_mode, const double* d_eps_occ,
+                              const double* d_eps_vir, int prec, corr_mp2_energy* h_out,
+                              int64_t* h_nkept);
+
 /* corr_rimp2_energy_noalloc: corr_rimp2_energy without any device allocation.
  * No counterpart in the reference: it is for callers that cannot allocate
  * (code under HIP graph capture setup, tight serving loops, harnesses that

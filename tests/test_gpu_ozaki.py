@@ -132,3 +132,43 @@ def test_mixed_b_formation(corr, naux, N, o):
     assert rel <= 1e-7
     got, _ = corr.ri_bia_from_j(dC, dP, dJ, o, "cholesky", "bf16x3")
     assert np.array_equal(got.cpu().numpy(), Bm)
+
+
+@pytest.mark.parametrize("naux,N,o,mode,prec", [
+    (160, 72, 8, "cholesky", "bf16x3"),     # full panels: o % 4 == 0, v % 32 == 0, naux % 16 == 0
+    (150, 61, 7, "cholesky", "bf16x3"),     # ragged o, v and naux
+    (300, 90, 13, "cholesky", "bf16x3"),    # several row and column tiles
+    (150, 61, 7, "cholesky", "fp64"),
+    (150, 61, 7, "eigen", "bf16x3"),
+])
+def test_ri_mp2_energy_from_j(corr, naux, N, o, mode, prec):
+    """B formation + energy in one call gives the bits of ri_bia_from_j
+    followed by rimp2_energy at the same precision; on the mixed path
+    (cholesky, bf16x3) the metric product writes the energy kernel's BF16
+    hi/lo panels itself."""
+    _say(f"ri_mp2_energy_from_j naux={naux} N={N} o={o} {mode} {prec}")
+    v = N - o
+    rng = np.random.default_rng(naux * N + o)
+    J = S.metric(naux, seed=naux + 1, cond=1e5)
+    Pmn = rng.standard_normal((naux, N, N))
+    Pmn = Pmn + Pmn.transpose(0, 2, 1)
+    C = S.orthonormal(N, seed=N + 1)
+    eo = -1.0 - rng.random(o)
+    ev = 0.5 + rng.random(v)
+    dJ, dP, dC, deo, dev_ = dev(J), dev(Pmn), dev(C), dev(eo), dev(ev)
+    Bia, kept = corr.ri_bia_from_j(dC, dP, dJ, o, mode, prec)
+    want = corr.rimp2_energy(Bia, deo, dev_, prec)
+    got, kept2 = corr.ri_mp2_energy_from_j(dC, dP, dJ, o, deo, dev_, mode, prec)
+    assert kept2 == kept
+    _say(f"  E = {got['EMP2']:.15f}")
+    assert got == want
+    # a second call on the same context reuses the panels it zeroed
+    again, _ = corr.ri_mp2_energy_from_j(dC, dP, dJ, o, deo, dev_, mode, prec)
+    assert again == want
+
+
+def test_ri_mp2_energy_from_j_rejects(corr):
+    dJ = dev(S.metric(40, seed=1, cond=10.0))
+    dP, dC = dev(np.zeros((40, 12, 12))), dev(np.eye(12))
+    with pytest.raises(ValueError):
+        corr.ri_mp2_energy_from_j(dC, dP, dJ, 4, dev(np.zeros(3)), dev(np.ones(8)))
