@@ -262,7 +262,12 @@ typedef __attribute__((address_space(1))) const void* corr_gbl_ptr;
 // on one FP32 level.  Pairs (i, j) carry their own weight: 2 for i < j, 1 for
 // i = j, 0 for i > j (the 4 x 8 occupied tiles straddle the diagonal).
 // Three stage buffers, copies two stages ahead.  The tile of each workgroup
-// comes from a list built on the host (ushort4: I, Jg, A, B).
+// comes from a list built on the device (ushort4: I, Jg, z, w): z < w is the
+// virtual pair (A, B) = (z, w); z > w holds the two diagonal blocks A = w and
+// A' = z (the T1 waves compute M^{ij} on A x A, the T2 waves on A' x A', from
+// the same six-panel stages, and each wave finds M_ba in its own block);
+// z = w is the diagonal block A alone (T1 and T2 then cover the same
+// elements, each at half weight).
 constexpr int WIDE_THREADS = 1024;
 typedef uint16_t Bf16Stage6[6][2][PW][16];   // [panel][hi/lo][row][k slot]
 
@@ -313,8 +318,8 @@ __device__ inline void wide_stage_compute(const Bf16Stage6& S, int pa, int pb, c
 // into buffer (s+2) % 3, which held stage s-1 (read before the barrier closing
 // iteration s-1), computes stage s, then waits until this wave's copies of
 // stage s+1 have landed (the NC of s+2 may still run) and its reads are back,
-// and meets the barrier.  Copies past the end re-copy stage Kc-1.  Kc is a
-// multiple of 3.  Ends with no copy in flight.
+// and meets the barrier.  Copies past the end re-copy stage Kc-1 (into a
+// buffer no later stage reads).  Ends with no copy in flight.
 template <int NC, int NA, int NB>
 __device__ inline void wide_k_loop(Bf16Stage6& L0, Bf16Stage6& L1, Bf16Stage6& L2, const uint16_t* const (&src)[NC],
                                    const int (&seg)[NC], int64_t Kc, int pa, int pb, const int (&rA)[NA],
@@ -328,11 +333,17 @@ __device__ inline void wide_k_loop(Bf16Stage6& L0, Bf16Stage6& L1, Bf16Stage6& L
         wide_stage_compute<NA, NB>(CUR, pa, pb, rA, rB, ko, acc);                             \
         CORR_STAGE_BARRIER(NC)                                                                \
     }
-    for (int64_t s = 0; s < Kc; s += 3) {
+    // whole groups of three stages in the loop, the one or two left after it: a
+    // branch inside the loop would make the compiler wait for every copy in
+    // flight before the next LDS read
+    const int64_t K3 = Kc - Kc % 3;
+    for (int64_t s = 0; s < K3; s += 3) {
         CORR_WIDE_STAGE(L0, L2, s)
         CORR_WIDE_STAGE(L1, L0, s + 1)
         CORR_WIDE_STAGE(L2, L1, s + 2)
     }
+    if (K3 < Kc) CORR_WIDE_STAGE(L0, L2, K3)
+    if (K3 + 1 < Kc) CORR_WIDE_STAGE(L1, L0, K3 + 1)
 #undef CORR_WIDE_STAGE
     CORR_WAIT_VMCNT(0);
 }
@@ -354,37 +365,37 @@ __device__ inline void wide_energy_term(double m1, double m2, double ea, double 
     sx += m1 * m2 * dinv;
 }
 
-__global__ __launch_bounds__(WIDE_THREADS) void rimp2_energy_bf16x3_wide(
-    const uint16_t* __restrict__ Bp, const ushort4* __restrict__ tiles, const double* __restrict__ eo,
-    const double* __restrict__ ev, int o, int v, int nIB, int nAC, int64_t Kc, int64_t nT, int64_t per,
-    double* __restrict__ partial) {
+// One tile of the wide kernel; PAIR2: the tile holds two diagonal blocks.
+template <bool PAIR2>
+__device__ __forceinline__ void wide_tile(Bf16Stage6& L0, Bf16Stage6& L1, Bf16Stage6& L2, double* sh, double* evAB,
+                                          const uint16_t* Bp,
+                                          ushort4 tl, const double* __restrict__ eo, const double* __restrict__ ev,
+                                          int o, int v, int nIB, int nAC, int64_t Kc, int64_t id,
+                                          double* __restrict__ partial) {
     constexpr int SUB = 2 * PW * 16;
-    __shared__ __attribute__((aligned(16))) Bf16Stage6 L0;
-    __shared__ __attribute__((aligned(16))) Bf16Stage6 L1;
-    __shared__ __attribute__((aligned(16))) Bf16Stage6 L2;
-    __shared__ double sh[WIDE_THREADS / 64];
-    __shared__ double evA[PA];   // eps_vir of the virtual block A (read after the exchange barrier)
-
-    const int64_t id = xcd_logical(per);
-    if (id >= nT) return;
-    const ushort4 tl = tiles[id];
-    const int IB = tl.x, JG = tl.y, AC = tl.z, BC = tl.w;
-    // loaded now, stored into evA after the K loop (no wait in the prologue)
+    const int IB = tl.x, JG = tl.y, AC = PAIR2 ? tl.w : tl.z, BC = PAIR2 ? tl.z : tl.w;
+    // stored into evAB after the K loop; loaded before it (no wait in the
+    // prologue), except for the rarer two-block tiles, where the register is
+    // worth more during the loop
     const int t32 = threadIdx.x & 31;
-    const double ev_own = ev[AC * PA + t32 < v ? AC * PA + t32 : v - 1];
+    const int evb = ((threadIdx.x & 63) < PA ? AC : BC) * PA + t32;
+    double ev_own = 0.0;
+    if constexpr (!PAIR2) ev_own = ev[evb < v ? evb : v - 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r32 = lane & 31, kh = lane >> 5;
     const int ko = bf16_slot(r32, kh) * 8;
 
-    // panel pointers: 0 (I,A), 1 (J0,B), 2 (J1,B), 3 (J0,A), 4 (J1,A), 5 (I,B); a J1
-    // past the last occupied block reads J0's panel (its pairs have j >= o).
+    // panel pointers: 0 (I,A), 1 (J0,B), 2 (J1,B), 3 (J0,A), 4 (J1,A), 5 (I,B), or for
+    // two diagonal blocks 0 (I,A), 1 (J0,A), 2 (J1,A), 3 (J0,A'), 4 (J1,A'), 5 (I,A');
+    // a J1 past the last occupied block reads J0's panel (its pairs have j >= o).
     const int J0 = 2 * JG, J1 = (2 * JG + 1 < nIB) ? 2 * JG + 1 : 2 * JG;
-    auto panel = [&](int p) -> const uint16_t* {
+    auto panel = [=](int p) -> const uint16_t* {
         const int ob = (p == 0 || p == 5) ? IB : (p == 1 || p == 3) ? J0 : J1;
-        const int vb = (p == 0 || p == 3 || p == 4) ? AC : BC;
+        const bool first = PAIR2 ? p < 3 : (p == 0 || p == 3 || p == 4);
+        const int vb = first ? AC : BC;
         return Bp + ((int64_t)ob * nAC + vb) * Kc * SUB;
     };
-    const double wab = AC < BC ? 1.0 : 0.5;
+    const double wab = AC < BC && !PAIR2 ? 1.0 : 0.5;
     double sos = 0.0, sx = 0.0;
 
     {
@@ -414,40 +425,82 @@ __global__ __launch_bounds__(WIDE_THREADS) void rimp2_energy_bf16x3_wide(
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[p][q][r] = 0.0f;
         wide_k_loop<3, 2, 2>(L0, L1, L2, src, seg, Kc, pa, pb, rA, rB, ko, acc);
-        if (threadIdx.x < PA) evA[threadIdx.x] = ev_own;
-
-        // The energy of block (x, y) of wave pair t8 (i = 2ip+x, j = 2jp+y) is
-        // taken by the T1 wave for x = 0 and by the T2 wave for x = 1, so the two
-        // swap one block row through LDS: the T1 wave stores its x = 1 blocks,
-        // the T2 wave its x = 0 blocks (M_ba of T1's x = 0), 16 KiB per wave
-        // pair, three pairs per 48 KiB buffer.
-        __syncthreads();
-        float* X = reinterpret_cast<float*>(t8 < 3 ? &L0[0][0][0][0] : t8 < 6 ? &L1[0][0][0][0] : &L2[0][0][0][0]) +
-                   (t8 % 3) * 4096;
+        if constexpr (PAIR2) ev_own = ev[evb < v ? evb : v - 1];
+        if (threadIdx.x < 2 * PA) evAB[threadIdx.x] = ev_own;
+        if constexpr (PAIR2) {
+            // Two diagonal blocks: each wave holds whole 32 x 32 blocks, M^{ij} on
+            // A x A (T1 waves) or A' x A' (T2 waves), and M_ba of an element is the
+            // transposed element of its own block.  Block by block a wave stores
+            // the block (element (r, c) at r * 33 + c: 4.1 KiB, six waves per 48 KiB
+            // buffer; the odd row stride keeps the transposed reads conflict-free)
+            // and reads M_ab and M_ba back.  Each (a, b) and (b, a) is visited, at
+            // half weight.
+            constexpr int XB = 32 * 33;
+            float* X = reinterpret_cast<float*>(wave < 6 ? &L0[0][0][0][0] : wave < 12 ? &L1[0][0][0][0]
+                                                                                      : &L2[0][0][0][0]) +
+                       (wave % 6) * XB;
+            const int vb = (half ? BC : AC) * PA;
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
+            for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                if (half) X[2048 + (y * 16 + r) * 64 + lane] = acc[y][0][r];
-                else X[(y * 16 + r) * 64 + lane] = acc[1][y][r];
-            }
-        __syncthreads();
-        const int b = BC * PA + r32;
-        const int i = IB * PI + 2 * ip + half;
+                for (int q = 0; q < 2; ++q) {
+                    __syncthreads();
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
-            const int j = JG * 2 * PI + 2 * jp + y;
-            const double w = (i < j ? 2.0 : (i == j ? 1.0 : 0.0)) * wab;
-            if (w > 0.0 && i < o && j < o && b < v) {
-                const double eij = ev[b] - eo[i] - eo[j];
-                float mo[16];   // T1 wave: M_ba from the T2 wave; T2 wave: M_ab from the T1 wave
+                    for (int r = 0; r < 16; ++r) {
+                        const int al = 8 * (r / 4) + 4 * kh + (r % 4);
+                        X[al * 33 + r32] = acc[p][q][r];
+                    }
+                    __syncthreads();
+                    // T1 wave: rows i = 2ip+p, columns j = 2jp+q; T2 wave: rows j = 2jp+p, columns i = 2ip+q
+                    const int i = IB * PI + 2 * ip + (half ? q : p);
+                    const int j = JG * 2 * PI + 2 * jp + (half ? p : q);
+                    const double w = (i < j ? 2.0 : (i == j ? 1.0 : 0.0)) * 0.5;
+                    if (w > 0.0 && i < o && j < o && vb + r32 < v) {
+                        const double eij = ev[vb + r32] - eo[i] - eo[j];
+#pragma unroll 4
+                        for (int r = 0; r < 16; ++r) {
+                            const int al = 8 * (r / 4) + 4 * kh + (r % 4);
+                            const float m = X[al * 33 + r32], mt = X[r32 * 33 + al];
+                            wide_energy_term((double)m, (double)mt, evAB[half * PA + al], eij, w, vb + al < v,
+                                             sos, sx);
+                        }
+                    }
+                }
+        } else {
+            // The energy of block (x, y) of wave pair t8 (i = 2ip+x, j = 2jp+y) is
+            // taken by the T1 wave for x = 0 and by the T2 wave for x = 1, so the two
+            // swap one block row through LDS: the T1 wave stores its x = 1 blocks,
+            // the T2 wave its x = 0 blocks (M_ba of T1's x = 0), 16 KiB per wave
+            // pair, three pairs per 48 KiB buffer.
+            __syncthreads();
+            float* X = reinterpret_cast<float*>(t8 < 3 ? &L0[0][0][0][0] : t8 < 6 ? &L1[0][0][0][0]
+                                                                                 : &L2[0][0][0][0]) +
+                       (t8 % 3) * 4096;
 #pragma unroll
-                for (int r = 0; r < 16; ++r) mo[r] = X[(half ? 0 : 2048) + (y * 16 + r) * 64 + lane];
+            for (int y = 0; y < 2; ++y)
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const double mr = half ? (double)acc[y][1][r] : (double)acc[0][y][r];
-                    const int al = 8 * (r / 4) + 4 * kh + (r % 4);
-                    wide_energy_term(mr, (double)mo[r], evA[al], eij, w, AC * PA + al < v, sos, sx);
+                    if (half) X[2048 + (y * 16 + r) * 64 + lane] = acc[y][0][r];
+                    else X[(y * 16 + r) * 64 + lane] = acc[1][y][r];
+                }
+            __syncthreads();
+            const int b = BC * PA + r32;
+            const int i = IB * PI + 2 * ip + half;
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int j = JG * 2 * PI + 2 * jp + y;
+                const double w = (i < j ? 2.0 : (i == j ? 1.0 : 0.0)) * wab;
+                if (w > 0.0 && i < o && j < o && b < v) {
+                    const double eij = ev[b] - eo[i] - eo[j];
+                    float mo[16];   // T1 wave: M_ba from the T2 wave; T2 wave: M_ab from the T1 wave
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) mo[r] = X[(half ? 0 : 2048) + (y * 16 + r) * 64 + lane];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const double mr = half ? (double)acc[y][1][r] : (double)acc[0][y][r];
+                        const int al = 8 * (r / 4) + 4 * kh + (r % 4);
+                        wide_energy_term(mr, (double)mo[r], evAB[al], eij, w, AC * PA + al < v, sos, sx);
+                    }
                 }
             }
         }
@@ -473,6 +526,26 @@ __global__ __launch_bounds__(WIDE_THREADS) void rimp2_energy_bf16x3_wide(
         partial[id * 2 + 0] = -tos;
         partial[id * 2 + 1] = 2.0 * tx;
     }
+}
+
+__global__ __launch_bounds__(WIDE_THREADS) void rimp2_energy_bf16x3_wide(
+    const uint16_t* __restrict__ Bp, const ushort4* __restrict__ tiles, const double* __restrict__ eo,
+    const double* __restrict__ ev, int o, int v, int nIB, int nAC, int64_t Kc, int64_t nT, int64_t per,
+    double* __restrict__ partial) {
+    // three separate buffers: the compiler can then tell that reads of one do
+    // not overlap the copies into another, and needs no wait for them
+    __shared__ __attribute__((aligned(16))) Bf16Stage6 L0;
+    __shared__ __attribute__((aligned(16))) Bf16Stage6 L1;
+    __shared__ __attribute__((aligned(16))) Bf16Stage6 L2;
+    __shared__ double sh[WIDE_THREADS / 64];
+    __shared__ double evAB[2 * PA];   // eps_vir of the blocks A and B (or A'), read after the K loop
+
+    const int64_t id = xcd_logical(per);
+    if (id >= nT) return;
+    const ushort4 tl = tiles[id];
+    // the two kinds of tile are separate code: nothing is live across this branch
+    if (tl.z > tl.w) wide_tile<true>(L0, L1, L2, sh, evAB, Bp, tl, eo, ev, o, v, nIB, nAC, Kc, id, partial);
+    else wide_tile<false>(L0, L1, L2, sh, evAB, Bp, tl, eo, ev, o, v, nIB, nAC, Kc, id, partial);
 }
 
 // ---- FP64 / FP32 kernel: stages -----------------------------------------------
@@ -804,8 +877,8 @@ struct RiLayout {
     int64_t Kp, nWG, per;
 };
 
-// kstage: K padding of the panels (64 for the 4-buffer kernels, 48 for the
-// 3-buffer wide kernel: whole groups of 16-deep stages).
+// kstage: K padding of the panels (64 for the 4-buffer kernels, 16 for the
+// wide kernel: whole 16-deep stages).
 static RiLayout ri_layout(int64_t naux, int64_t o, int64_t v, int64_t kstage = 64) {
     RiLayout l;
     l.nIB = (int)cdiv(o, PI);
@@ -824,10 +897,13 @@ static RiLayout ri_layout(int64_t naux, int64_t o, int64_t v, int64_t kstage = 6
 // 62.5 ms against 66.8 ms for plain (I, Jg, A, B) order at (H2O)32).
 constexpr int WT_GI = 4, WT_GA = 4;
 
+// Tiles of one occupied (I, Jg) in the virtual group pair (gA, gB): every
+// (A, B) with A < B, and on a diagonal group the diagonal blocks two at a
+// time (one tile holds (A, A) and (A + 1, A + 1); an odd last block alone).
 __host__ __device__ inline int wt_npairs(int gA, int gB, int nAC) {
     const int nA = std::min(nAC, (gA + 1) * WT_GA) - gA * WT_GA;
     const int nB = std::min(nAC, (gB + 1) * WT_GA) - gB * WT_GA;
-    return gA < gB ? nA * nB : nA * (nA + 1) / 2;
+    return gA < gB ? nA * nB : nA * (nA - 1) / 2 + (nA + 1) / 2;
 }
 
 __host__ __device__ inline int wt_count(int64_t st, int nIB, int nAC, int nJG, int nSP, int& iG, int& G, int& gA, int& gB) {
@@ -870,10 +946,18 @@ __global__ __launch_bounds__(1024) void wide_tile_list(ushort4* __restrict__ til
         wt_count(st, nIB, nAC, nJG, nSP, iG, G, gA, gB);
         for (int I = iG * WT_GI; I < std::min(nIB, (iG + 1) * WT_GI); ++I) {
             if (8 * G + 7 < 4 * I) continue;   // every pair has i > j
-            for (int A = gA * WT_GA; A < std::min(nAC, (gA + 1) * WT_GA); ++A)
-                for (int B = std::max(A, gB * WT_GA); B < std::min(nAC, (gB + 1) * WT_GA); ++B)
+            const int aEnd = std::min(nAC, (gA + 1) * WT_GA), bEnd = std::min(nAC, (gB + 1) * WT_GA);
+            for (int A = gA * WT_GA; A < aEnd; ++A) {
+                if (gA == gB && (A - gA * WT_GA) % 2 == 0) {
+                    // (z, w) = (A + 1, A): the diagonal blocks A and A + 1; (A, A): A alone
+                    const int A2 = A + 1 < aEnd ? A + 1 : A;
+                    tiles[k++] = make_ushort4((unsigned short)I, (unsigned short)G, (unsigned short)A2,
+                                              (unsigned short)A);
+                }
+                for (int B = std::max(A + 1, gB * WT_GA); B < bEnd; ++B)
                     tiles[k++] = make_ushort4((unsigned short)I, (unsigned short)G, (unsigned short)A,
                                               (unsigned short)B);
+            }
         }
     }
 }
@@ -904,7 +988,7 @@ struct RimpPlan {
 static RimpPlan rimp2_plan(int64_t naux, int64_t o, int64_t v, int prec) {
     RimpPlan p;
     const bool bf = prec == CORR_PREC_BF16X3;
-    p.l = ri_layout(naux, o, v, bf ? 48 : 64);
+    p.l = ri_layout(naux, o, v, bf ? 16 : 64);
     const size_t esz = prec == CORR_PREC_FP64 ? 8 : 4;   // bf16 hi+lo = 4 B
     p.b_panels = (size_t)p.l.nIB * p.l.nAC * p.l.Kp * PW * esz;
     if (bf) {

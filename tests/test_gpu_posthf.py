@@ -271,6 +271,25 @@ def test_rimp2_reduced_precision(corr, prec, rtol):
         assert abs(got[k] - ref[k]) <= rtol * abs(ref[k]), (k, got[k], ref[k])
 
 
+@pytest.mark.parametrize("naux,o,v", [(20, 13, 31), (33, 5, 64), (48, 9, 150), (40, 16, 224), (64, 21, 608)])
+def test_rimp2_bf16x3_exact_products(corr, naux, o, v):
+    """BF16x3 tile coverage and weights, sharply: B holds small integers
+    (exact in BF16, every M^{ij}_ab sum exact in FP32), so the BF16x3 energy
+    equals the FP64 oracle's to rounding of the FP64 epilogue.  The virtual
+    block counts 1, 2, 5, 7 and 19 give a diagonal block alone, one pair of
+    diagonal blocks, pairs with an odd block left over (in a group of four and
+    in a group of three), and the (H2O)32 layout; odd occupied block counts
+    leave the last 8-block half empty."""
+    _say(f"rimp2 bf16x3 exact products naux={naux} o={o} v={v}")
+    rng = np.random.default_rng(naux * o + v)
+    B = rng.integers(-8, 9, (naux, o, v)).astype(float)
+    eo, ev = S.orbital_energies(o, v, seed=o + v)
+    ref = O.rimp2(B, eo, ev)
+    got = corr.rimp2_energy(dev(B), dev(eo), dev(ev), prec="bf16x3")
+    for k in ref:
+        assert abs(got[k] - ref[k]) <= 1e-12 * abs(ref[k]), (k, got[k], ref[k])
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 def test_rimp2_deterministic(corr, prec):
     """Fixed-order reductions: repeated calls give bit-identical energies in
