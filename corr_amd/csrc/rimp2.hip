@@ -26,6 +26,7 @@
 // workgroup stages into LDS is one contiguous, fully coalesced read, moved by
 // direct global->LDS copies.
 #include "common.hpp"
+#include "wide_tiles.hpp"
 
 #include <algorithm>
 #include <type_traits>
@@ -320,13 +321,14 @@ __device__ inline void wide_stage_compute(const Bf16Stage6& S, int pa, int pb, c
 // stage s+1 have landed (the NC of s+2 may still run) and its reads are back,
 // and meets the barrier.  Copies past the end re-copy stage Kc-1 (into a
 // buffer no later stage reads).  Ends with no copy in flight.
-template <int NC, int NA, int NB>
+template <int NC, int NA, int NB, typename Pro>
 __device__ inline void wide_k_loop(Bf16Stage6& L0, Bf16Stage6& L1, Bf16Stage6& L2, const uint16_t* const (&src)[NC],
                                    const int (&seg)[NC], int64_t Kc, int pa, int pb, const int (&rA)[NA],
-                                   const int (&rB)[NB], int ko, corr_f16v (&acc)[NA][NB]) {
+                                   const int (&rB)[NB], int ko, corr_f16v (&acc)[NA][NB], Pro pro) {
     wide_stage_copy<NC>(L0, src, seg, 0);
     wide_stage_copy<NC>(L1, src, seg, Kc > 1 ? 1 : Kc - 1);
     CORR_STAGE_BARRIER(NC)
+    pro();   // the caller's prologue work (older loads have landed: no wait for the copies)
 #define CORR_WIDE_STAGE(CUR, COPY, s_)                                                        \
     {                                                                                         \
         wide_stage_copy<NC>(COPY, src, seg, ((s_) + 2 < Kc) ? (s_) + 2 : Kc - 1);             \
@@ -373,22 +375,20 @@ __device__ __forceinline__ void wide_tile(Bf16Stage6& L0, Bf16Stage6& L1, Bf16St
                                           int o, int v, int nIB, int nAC, int64_t Kc, int64_t id,
                                           double* __restrict__ partial) {
     constexpr int SUB = 2 * PW * 16;
-    const int IB = tl.x, JG = tl.y, AC = PAIR2 ? tl.w : tl.z, BC = PAIR2 ? tl.z : tl.w;
-    // stored into evAB after the K loop; loaded before it (no wait in the
-    // prologue), except for the rarer two-block tiles, where the register is
-    // worth more during the loop
+    const int IB = tl.x & WT_IMASK, J0 = tl.y, AC = PAIR2 ? tl.w : tl.z, BC = PAIR2 ? tl.z : tl.w;
+    const bool j1ok = !(tl.x & WT_NOJ1);
+    const int J1 = j1ok ? J0 + ((tl.x & WT_STEP2) ? 2 : 1) : J0;   // no J1: J0's panel again, weight 0
+    // loaded first, stored into evAB once the first stage copies are issued
+    // (the wait then covers this load only) and read after the K loop
     const int t32 = threadIdx.x & 31;
     const int evb = ((threadIdx.x & 63) < PA ? AC : BC) * PA + t32;
-    double ev_own = 0.0;
-    if constexpr (!PAIR2) ev_own = ev[evb < v ? evb : v - 1];
+    const double ev_own = ev[evb < v ? evb : v - 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r32 = lane & 31, kh = lane >> 5;
     const int ko = bf16_slot(r32, kh) * 8;
 
     // panel pointers: 0 (I,A), 1 (J0,B), 2 (J1,B), 3 (J0,A), 4 (J1,A), 5 (I,B), or for
     // two diagonal blocks 0 (I,A), 1 (J0,A), 2 (J1,A), 3 (J0,A'), 4 (J1,A'), 5 (I,A');
-    // a J1 past the last occupied block reads J0's panel (its pairs have j >= o).
-    const int J0 = 2 * JG, J1 = (2 * JG + 1 < nIB) ? 2 * JG + 1 : 2 * JG;
     auto panel = [=](int p) -> const uint16_t* {
         const int ob = (p == 0 || p == 5) ? IB : (p == 1 || p == 3) ? J0 : J1;
         const bool first = PAIR2 ? p < 3 : (p == 0 || p == 3 || p == 4);
@@ -424,9 +424,9 @@ __device__ __forceinline__ void wide_tile(Bf16Stage6& L0, Bf16Stage6& L1, Bf16St
             for (int q = 0; q < 2; ++q)
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[p][q][r] = 0.0f;
-        wide_k_loop<3, 2, 2>(L0, L1, L2, src, seg, Kc, pa, pb, rA, rB, ko, acc);
-        if constexpr (PAIR2) ev_own = ev[evb < v ? evb : v - 1];
-        if (threadIdx.x < 2 * PA) evAB[threadIdx.x] = ev_own;
+        wide_k_loop<3, 2, 2>(L0, L1, L2, src, seg, Kc, pa, pb, rA, rB, ko, acc, [&]() {
+            if (threadIdx.x < 2 * PA) evAB[threadIdx.x] = ev_own;
+        });
         if constexpr (PAIR2) {
             // Two diagonal blocks: each wave holds whole 32 x 32 blocks, M^{ij} on
             // A x A (T1 waves) or A' x A' (T2 waves), and M_ba of an element is the
@@ -453,8 +453,8 @@ __device__ __forceinline__ void wide_tile(Bf16Stage6& L0, Bf16Stage6& L1, Bf16St
                     __syncthreads();
                     // T1 wave: rows i = 2ip+p, columns j = 2jp+q; T2 wave: rows j = 2jp+p, columns i = 2ip+q
                     const int i = IB * PI + 2 * ip + (half ? q : p);
-                    const int j = JG * 2 * PI + 2 * jp + (half ? p : q);
-                    const double w = (i < j ? 2.0 : (i == j ? 1.0 : 0.0)) * 0.5;
+                    const int jb = jp < 2 ? J0 : J1, j = jb * PI + ((2 * jp + (half ? p : q)) & 3);
+                    const double w = wt_pair_weight(IB, jb, i, j, jp < 2 || j1ok) * 0.5;
                     if (w > 0.0 && i < o && j < o && vb + r32 < v) {
                         const double eij = ev[vb + r32] - eo[i] - eo[j];
 #pragma unroll 4
@@ -488,8 +488,8 @@ __device__ __forceinline__ void wide_tile(Bf16Stage6& L0, Bf16Stage6& L1, Bf16St
             const int i = IB * PI + 2 * ip + half;
 #pragma unroll
             for (int y = 0; y < 2; ++y) {
-                const int j = JG * 2 * PI + 2 * jp + y;
-                const double w = (i < j ? 2.0 : (i == j ? 1.0 : 0.0)) * wab;
+                const int jb = jp < 2 ? J0 : J1, j = jb * PI + ((2 * jp + y) & 3);
+                const double w = wt_pair_weight(IB, jb, i, j, jp < 2 || j1ok) * wab;
                 if (w > 0.0 && i < o && j < o && b < v) {
                     const double eij = ev[b] - eo[i] - eo[j];
                     float mo[16];   // T1 wave: M_ba from the T2 wave; T2 wave: M_ab from the T1 wave
@@ -889,65 +889,19 @@ static RiLayout ri_layout(int64_t naux, int64_t o, int64_t v, int64_t kstage = 6
     return l;
 }
 
-// Tile list of the wide kernel: for every (I, Jg) with some pair i <= j, the
-// virtual pairs A < B and the diagonal blocks (two per tile), in super-tiles
-// of WT_GI I-blocks x one Jg x (WT_GA x WT_GA virtual blocks, A-group <=
-// B-group), super-tiles row-major (I-group, Jg, group pair), tiles inside
-// row-major (I, A, B).  Consecutive tiles, which run together on one XCD, then
-// share most of their panels in its L2 (measured: 62.5 ms against 66.8 ms for
-// plain (I, Jg, A, B) order at (H2O)32).  The diagonal tiles come after all
-// the others, in the same order: placed among them they took the L2 hit rate
-// of a (H2O)32 launch from 81% to 50% (2.5 times the HBM traffic); at the end
-// it is 78% and the launch 2.8% faster than with one diagonal block per tile.
-constexpr int WT_GI = 4, WT_GA = 4;
-
-// Tiles of one occupied (I, Jg) in the virtual group pair (gA, gB): every
-// (A, B) with A < B, and on a diagonal group the diagonal blocks two at a
-// time (one tile holds (A, A) and (A + 1, A + 1); an odd last block alone).
-__host__ __device__ inline int wt_npairs(int gA, int gB, int nAC) {
-    const int nA = std::min(nAC, (gA + 1) * WT_GA) - gA * WT_GA;
-    const int nB = std::min(nAC, (gB + 1) * WT_GA) - gB * WT_GA;
-    return gA < gB ? nA * nB : nA * (nA - 1) / 2 + (nA + 1) / 2;
-}
-
-// Diagonal tiles (two blocks, or an odd last block alone) of the super-tile
-// (iG, G) on the diagonal group pair (gA, gA).
-__host__ __device__ inline int wt_ndiag(int iG, int G, int gA, int nIB, int nAC) {
-    int nI = 0;
-    for (int I = iG * WT_GI; I < std::min(nIB, (iG + 1) * WT_GI); ++I) nI += (8 * G + 7 >= 4 * I) ? 1 : 0;
-    const int nA = std::min(nAC, (gA + 1) * WT_GA) - gA * WT_GA;
-    return nI * ((nA + 1) / 2);
-}
-
-__host__ __device__ inline int wt_count(int64_t st, int nIB, int nAC, int nJG, int nSP, int& iG, int& G, int& gA, int& gB) {
-    const int sp = (int)(st % nSP);
-    G = (int)((st / nSP) % nJG);
-    iG = (int)(st / ((int64_t)nSP * nJG));
-    // (gA, gB), gA <= gB, row-major over nAG groups
-    const int nAG = (nAC + WT_GA - 1) / WT_GA;
-    int r = 0, rem = sp;
-    while (rem >= nAG - r) { rem -= nAG - r; ++r; }
-    gA = r;
-    gB = r + rem;
-    int nI = 0;
-    for (int I = iG * WT_GI; I < std::min(nIB, (iG + 1) * WT_GI); ++I) nI += (8 * G + 7 >= 4 * I) ? 1 : 0;
-    return nI * wt_npairs(gA, gB, nAC);
-}
-
-// One workgroup of 1024 threads: thread t takes a contiguous chunk of the NS
-// super-tiles, counts its tiles, a workgroup scan gives its offset, and it
-// writes its tiles in order.
-__global__ __launch_bounds__(1024) void wide_tile_list(ushort4* __restrict__ tiles, int nIB, int nAC, int nJG,
-                                                      int nSP, int64_t NS) {
+// One workgroup of 1024 threads builds the wide kernel's tile list
+// (wide_tiles.hpp): thread t takes a contiguous chunk of the NS super-tiles,
+// counts its tiles, a workgroup scan gives its offsets, and it writes its
+// tiles in order, the diagonal ones after all the others.
+__global__ __launch_bounds__(1024) void wide_tile_list(ushort4* __restrict__ tiles, WtGrid g) {
     __shared__ int64_t sc[1024], sd[1024];
     const int t = threadIdx.x;
-    const int64_t C = (NS + 1023) / 1024;
-    const int64_t s0 = std::min<int64_t>(NS, t * C), s1 = std::min<int64_t>(NS, s0 + C);
+    const int64_t C = (g.NS + 1023) / 1024;
+    const int64_t s0 = std::min<int64_t>(g.NS, t * C), s1 = std::min<int64_t>(g.NS, s0 + C);
     int64_t n = 0, nd = 0;   // this thread's tiles with A < B, and its diagonal tiles
-    int iG, G, gA, gB;
     for (int64_t st = s0; st < s1; ++st) {
-        const int c = wt_count(st, nIB, nAC, nJG, nSP, iG, G, gA, gB);
-        const int d = gA == gB ? wt_ndiag(iG, G, gA, nIB, nAC) : 0;
+        int c, d;
+        wt_count(g, st, c, d);
         n += c - d;
         nd += d;
     }
@@ -961,42 +915,22 @@ __global__ __launch_bounds__(1024) void wide_tile_list(ushort4* __restrict__ til
         sd[t] += y;
         __syncthreads();
     }
-    // exclusive offsets; the diagonal tiles follow all the others
-    int64_t k = sc[t] - n, kd = sc[1023] + sd[t] - nd;
-    for (int64_t st = s0; st < s1; ++st) {
-        wt_count(st, nIB, nAC, nJG, nSP, iG, G, gA, gB);
-        for (int I = iG * WT_GI; I < std::min(nIB, (iG + 1) * WT_GI); ++I) {
-            if (8 * G + 7 < 4 * I) continue;   // every pair has i > j
-            const int aEnd = std::min(nAC, (gA + 1) * WT_GA), bEnd = std::min(nAC, (gB + 1) * WT_GA);
-            for (int A = gA * WT_GA; A < aEnd; ++A) {
-                if (gA == gB && (A - gA * WT_GA) % 2 == 0) {
-                    // (z, w) = (A + 1, A): the diagonal blocks A and A + 1; (A, A): A alone
-                    const int A2 = A + 1 < aEnd ? A + 1 : A;
-                    tiles[kd++] = make_ushort4((unsigned short)I, (unsigned short)G, (unsigned short)A2,
-                                               (unsigned short)A);
-                }
-                for (int B = std::max(A + 1, gB * WT_GA); B < bEnd; ++B)
-                    tiles[k++] = make_ushort4((unsigned short)I, (unsigned short)G, (unsigned short)A,
-                                              (unsigned short)B);
-            }
-        }
-    }
+    int64_t k = sc[t] - n, kd = sc[1023] + sd[t] - nd;   // exclusive offsets
+    for (int64_t st = s0; st < s1; ++st)
+        wt_emit(g, st, [&](int x, int y, int z, int w, bool diag) {
+            tiles[diag ? kd++ : k++] =
+                make_ushort4((unsigned short)x, (unsigned short)y, (unsigned short)z, (unsigned short)w);
+        });
 }
 
 struct WideTiles {
-    int nJG, nSP;
-    int64_t NS, nT;
+    WtGrid g;
+    int64_t nT;
 };
 
 static WideTiles wide_tiles(int nIB, int nAC) {
     WideTiles w;
-    w.nJG = (nIB + 1) / 2;
-    const int nIG = (nIB + WT_GI - 1) / WT_GI, nAG = (nAC + WT_GA - 1) / WT_GA;
-    w.nSP = nAG * (nAG + 1) / 2;
-    w.NS = (int64_t)nIG * w.nJG * w.nSP;
-    w.nT = 0;
-    int iG, G, gA, gB;
-    for (int64_t st = 0; st < w.NS; ++st) w.nT += wt_count(st, nIB, nAC, w.nJG, w.nSP, iG, G, gA, gB);
+    w.g = wt_grid(nIB, nAC, w.nT);
     return w;
 }
 
@@ -1017,7 +951,7 @@ static RimpPlan rimp2_plan(int64_t naux, int64_t o, int64_t v, int prec) {
         p.b_partial = sizeof(double) * 2 * p.wt.nT;
         p.b_tiles = sizeof(ushort4) * p.wt.nT;
     } else {
-        p.wt = WideTiles{0, 0, 0, 0};
+        p.wt = WideTiles{WtGrid{0, 0, 0, 0, 0, 0}, 0};
         p.b_partial = sizeof(double) * 2 * p.l.nWG;
         p.b_tiles = 0;
     }
@@ -1070,7 +1004,7 @@ static int rimp2_energy_impl(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t 
     const RimpPlan p = rimp2_plan(naux, nocc, nvir, prec);
     const RiLayout& l = p.l;
     const bool bf = prec == CORR_PREC_BF16X3;
-    CORR_REQUIRE(ctx, !bf || (l.nIB < 65535 && l.nAC < 65535), "rimp2: nocc/nvir too large for the tile list");
+    CORR_REQUIRE(ctx, !bf || (l.nIB <= WT_IMASK && l.nAC < 65535), "rimp2: nocc/nvir too large for the tile list");
     const int64_t nWG = bf ? p.wt.nT : l.nWG, per = cdiv(nWG, 8);
     CORR_REQUIRE(ctx, per * 8 < ((int64_t)1 << 31), "rimp2: too many tiles");
     RimpWs w;
@@ -1085,8 +1019,7 @@ static int rimp2_energy_impl(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t 
     }
     if (!prepacked) CORR_TRY(ri_repack(ctx, l, naux, (int)nocc, (int)nvir, Bia, prec, w.panels));
     if (bf) {
-        hipLaunchKernelGGL(wide_tile_list, dim3(1), dim3(1024), 0, ctx->stream, w.tiles, l.nIB, l.nAC, p.wt.nJG,
-                           p.wt.nSP, p.wt.NS);
+        hipLaunchKernelGGL(wide_tile_list, dim3(1), dim3(1024), 0, ctx->stream, w.tiles, p.wt.g);
         CORR_LAUNCHED(ctx, "wide_tile_list");
     }
     {
