@@ -203,6 +203,95 @@ __global__ __launch_bounds__(256) void oz_split_rmajor_kernel(const double* __re
     }
 }
 
+// The same slices in one pass when K is short (2 Ks 16-deep chunks <= 64 U):
+// a workgroup of 1024 threads takes 16 columns n and the whole K range in
+// registers (thread t: column t % 16, chunks t / 16 + 64 u), so the operand is
+// read once; the column maxima go over the four lanes of a column in a wave,
+// then over the 16 waves through LDS.  Eight workgroups cover a 128-column
+// panel, the columns past N included (zero slices), as in the two-pass
+// kernel.  Each slice is taken in place in the value registers, with the
+// arithmetic of oz_slices.
+template <int U>
+__global__ __launch_bounds__(1024) void oz_split_rmajor1_kernel(const double* __restrict__ B, int64_t ldb,
+                                                               int64_t N, int64_t K, int64_t Ks,
+                                                               double* __restrict__ tau, int8_t* __restrict__ Bp) {
+    __shared__ double smx[16][16];
+    __shared__ int sbad[16][16];
+    const int t = threadIdx.x, c = t & 15, ck0 = t >> 4, wave = t >> 6, lane = t & 63;
+    const int64_t n = (int64_t)blockIdx.x * 16 + c;
+    const bool col_ok = n < N;
+    const int64_t nck = 2 * Ks;
+    double x[U][16];
+    double mx = 0.0;
+    bool bad = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t ck = ck0 + 64 * u;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int64_t k = ck * 16 + j;
+            x[u][j] = (col_ok && ck < nck && k < K) ? B[k * ldb + n] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const double a = fabs(x[u][j]);
+            bad |= !(a <= DBL_MAX);
+            mx = a > mx ? a : mx;
+        }
+    }
+#pragma unroll
+    for (int off = 16; off < 64; off <<= 1) {
+        const double o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+        bad |= __shfl_xor((int)bad, off, 64) != 0;
+    }
+    if (lane < 16) {
+        smx[wave][lane] = mx;
+        sbad[wave][lane] = bad;
+    }
+    __syncthreads();
+    double cm = 0.0;
+    bool cb = false;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const double o = smx[w][c];
+        cm = o > cm ? o : cm;
+        cb |= sbad[w][c] != 0;
+    }
+    const double scale = oz_scale(cm, cb);
+    if (col_ok && ck0 == 0) tau[n] = scale;
+    const double inv = col_ok ? oz_inv(scale) : 0.0;
+    const int64_t nt = n / OZ_T;
+    const int col = (int)(n % OZ_T);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t ck = ck0 + 64 * u;
+        if (ck >= nck) continue;
+        const int ch = (int)(ck & 1);
+        int8_t* out = Bp + (nt * Ks + (ck >> 1)) * OZ_PANEL + col * 32 + oz_slot(col, ch) * 16;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[u][j] = inv == 0.0 ? 0.0 : x[u][j] * inv;   // power-of-two scaling: exact
+#pragma unroll
+        for (int sl = 0; sl < OZ_S; ++sl) {
+            oz_i4 w;
+#pragma unroll
+            for (int v4 = 0; v4 < 4; ++v4) {
+                int word = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    double& r = x[u][4 * v4 + b];
+                    r *= 128.0;
+                    const double q = rint(r);
+                    r -= q;
+                    word |= (int)(uint8_t)(int8_t)(int)q << (8 * b);
+                }
+                w[v4] = word;
+            }
+            *reinterpret_cast<oz_i4*>(out + sl * OZ_SLICE) = w;
+        }
+    }
+}
+
 __device__ inline int64_t oz_xcd_logical(int64_t per) {
     const int64_t w = blockIdx.x;
     return (w % 8) * per + (w / 8);
@@ -431,6 +520,16 @@ static int oz_split(corr_ctx* ctx, const double* X, int64_t ld, int64_t R, int64
         hipLaunchKernelGGL(oz_split_kmajor_kernel, dim3((unsigned)(cdiv(R, OZ_T) * Ks)), dim3(256), 0, ctx->stream,
                            X, ld, R, K, Ks, scale, P);
         CORR_LAUNCHED(ctx, "oz_split_kmajor_kernel");
+    } else if (2 * Ks <= 64 * 3) {
+        // short K: one pass, the operand held in registers (Ks <= 96, K <= 3072)
+        const dim3 grid((unsigned)(cdiv(R, OZ_T) * (OZ_T / 16)));
+        if (2 * Ks <= 64)
+            hipLaunchKernelGGL(oz_split_rmajor1_kernel<1>, grid, dim3(1024), 0, ctx->stream, X, ld, R, K, Ks, scale, P);
+        else if (2 * Ks <= 128)
+            hipLaunchKernelGGL(oz_split_rmajor1_kernel<2>, grid, dim3(1024), 0, ctx->stream, X, ld, R, K, Ks, scale, P);
+        else
+            hipLaunchKernelGGL(oz_split_rmajor1_kernel<3>, grid, dim3(1024), 0, ctx->stream, X, ld, R, K, Ks, scale, P);
+        CORR_LAUNCHED(ctx, "oz_split_rmajor1_kernel");
     } else {
         hipLaunchKernelGGL(oz_split_rmajor_kernel, dim3((unsigned)cdiv(R, OZ_T)), dim3(256), 0, ctx->stream, X, ld,
                            R, K, Ks, scale, P);
