@@ -418,7 +418,9 @@ int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double*
 // small launches never queue behind GEMM workgroups.  Measured on the
 // (H2O)32 B formation (FP64, Cholesky): 26.7 ms against 29.1-29.5 ms with
 // both on all CUs; 16 or 24 reserved CUs gave 27.8 and 26.7 ms, 8 gave
-// 38.6 ms.  Falls back to one unmasked aux_stream when masks are unavailable.
+// 38.6 ms.  The mixed path (int8 AO->MO GEMM, one-pass slicing): 17.2-17.5 ms
+// with 32 reserved CUs, 17.3-17.5 with 24, 18.9-19.2 with 40 or 48.  Falls
+// back to one unmasked aux_stream when masks are unavailable.
 static int overlap_streams(corr_ctx* ctx) {
     if (ctx->aux_stream) return CORR_OK;
     int ncu = 0;
@@ -447,9 +449,9 @@ static int overlap_streams(corr_ctx* ctx) {
 // it (on the CUs the auxiliary stream leaves free, when the CUs are split),
 // both ordered after the caller's prior work by an event; the metric GEMM
 // runs on the caller's stream once both have finished.  When that GEMM runs
-// on int8 MFMA (the mixed path), each stream slices its own operand first
-// ((P|ia) after the AO->MO GEMMs, L^-1 after the chain), so the slicing of
-// (P|ia) overlaps the end of the chain.
+// on int8 MFMA (the mixed path), the AO->MO stream slices (P|ia) after its
+// GEMMs, and L^-1 is sliced after the join, on all CUs (the chain, on one
+// eighth of them, ends after the AO->MO side).
 // pan != nullptr (Cholesky metric, mixed path only): the metric product writes
 // the BF16x3 energy's panels instead of Bia (which may then be null).
 static int ri_bia_from_j_impl(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double* C,
@@ -500,7 +502,8 @@ static int ri_bia_from_j_impl(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o,
     CORR_HIP(ctx, hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0));
     ctx->stream = ctx->aux_stream;
     rc = ri_metric(ctx, naux, J, mode, M, nkept);
-    if (rc == CORR_OK && int8m) rc = ozaki_prepare(ctx, "oz_metric_L", M, naux, naux, naux, true, &opA);
+    if (rc == CORR_OK && int8m && !ctx->aux_masked)
+        rc = ozaki_prepare(ctx, "oz_metric_L", M, naux, naux, naux, true, &opA);
     if (rc == CORR_OK && hipEventRecord(ctx->ev_join, ctx->aux_stream) != hipSuccess)
         rc = set_error(ctx, CORR_EHIP, "ri_bia_from_j: event record failed");
     ctx->stream = main;
@@ -511,6 +514,7 @@ static int ri_bia_from_j_impl(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o,
     }
     if (ctx->aux_masked) CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_part, 0));
     CORR_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_join, 0));
+    if (int8m && ctx->aux_masked) CORR_TRY(ozaki_prepare(ctx, "oz_metric_L", M, naux, naux, naux, true, &opA));
     if (int8m) return ozaki_gemm_prepared(ctx, opA, opB, 1.0, 0.0, Bia, o * v, true, pan);
     return gemm(ctx, 0, 0, naux, o * v, naux, 1.0, M, naux, 0, X, o * v, 0, 0.0, Bia, o * v, 0, 1,
                 metric_gemm_prec(mode, prec), mode == CORR_METRIC_CHOLESKY ? GEMM_TRI_A_LOWER : 0);
