@@ -1001,10 +1001,10 @@ static int rimp2_energy_impl(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t 
     CORR_REQUIRE(ctx, !prepacked || (prec == CORR_PREC_BF16X3 && !ws), "rimp2: prepacked panels are BF16x3 scratch");
     CORR_REQUIRE(ctx, prec == CORR_PREC_FP64 || prec == CORR_PREC_FP32 || prec == CORR_PREC_BF16X3,
                  "rimp2: unknown precision");
+    const bool bf = prec == CORR_PREC_BF16X3;
+    CORR_REQUIRE(ctx, !bf || cdiv(nocc, PI) <= WT_IMASK, "rimp2: nocc too large for the BF16x3 tile list");
     const RimpPlan p = rimp2_plan(naux, nocc, nvir, prec);
     const RiLayout& l = p.l;
-    const bool bf = prec == CORR_PREC_BF16X3;
-    CORR_REQUIRE(ctx, !bf || (l.nIB <= WT_IMASK && l.nAC < 65535), "rimp2: nocc/nvir too large for the tile list");
     const int64_t nWG = bf ? p.wt.nT : l.nWG, per = cdiv(nWG, 8);
     CORR_REQUIRE(ctx, per * 8 < ((int64_t)1 << 31), "rimp2: too many tiles");
     RimpWs w;

@@ -202,7 +202,9 @@ int corr_ri_bia_from_j(corr_ctx* ctx, int64_t naux, int64_t nroao, int64_t nocc,
                        int64_t* h_nkept);
 
 /* ---- a9: RI-MP2 energy, fused pair-block GEMM + denominator + reduction ---
- * (can_mp2.cc:53-132).  eps_occ[i] = FMo[i,i], eps_vir[a] = FMo[o+a,o+a]. */
+ * (can_mp2.cc:53-132).  eps_occ[i] = FMo[i,i], eps_vir[a] = FMo[o+a,o+a].
+ * Sizes: nocc, nvir < 2^20, and with CORR_PREC_BF16X3 nocc <= 65532 (the
+ * tile list's 14-bit occupied block index); CORR_EINVAL otherwise. */
 int corr_rimp2_energy(corr_ctx* ctx, int64_t naux, int64_t nocc, int64_t nvir,
                       const double* d_Bia, const double* d_eps_occ,
                       const double* d_eps_vir, int prec, corr_mp2_energy* h_out);
