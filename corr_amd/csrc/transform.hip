@@ -419,8 +419,10 @@ int ri_form_bia(corr_ctx* ctx, int64_t naux, int64_t N, int64_t o, const double*
 // (H2O)32 B formation (FP64, Cholesky): 26.7 ms against 29.1-29.5 ms with
 // both on all CUs; 16 or 24 reserved CUs gave 27.8 and 26.7 ms, 8 gave
 // 38.6 ms.  The mixed path (int8 AO->MO GEMM, one-pass slicing): 17.2-17.5 ms
-// with 32 reserved CUs, 17.3-17.5 with 24, 18.9-19.2 with 40 or 48.  Falls
-// back to one unmasked aux_stream when masks are unavailable.
+// with 32 reserved CUs, 17.3-17.5 with 24, 18.9-19.2 with 40 or 48, and
+// 19.2-19.4 with 32 or 64 CUs taken as every eighth CU (c % 8 == 0, < 2)
+// instead of the first ones.  Falls back to one unmasked aux_stream when
+// masks are unavailable.
 static int overlap_streams(corr_ctx* ctx) {
     if (ctx->aux_stream) return CORR_OK;
     int ncu = 0;
