@@ -97,7 +97,8 @@ def test_h2o32_mixed_precision_error_real_chemistry():
     product emulated on int8 MFMA, BF16x3 energy) stays within 1 uEh of FP64
     (0.76 measured); its B is within 1e-6 of FP64 relative to max |B|
     (2.6e-7 measured) and moves the FP64 energy by at most 0.02 uEh (0.003
-    measured); the
+    measured); the one-call entry the bench times (ri_mp2_energy_from_j)
+    gives the same bits as ri_bia_from_j + rimp2_energy; the
     FP32 energy kernel on the FP64 B stays within 0.2 uEh.  B formed with
     FP32 GEMMs is off by up to a few uEh: the AO->MO sums cancel (3e-6 rms
     relative error in (Q|ia)) and the Cholesky metric (cond(J) = 1.1e6)
@@ -115,10 +116,15 @@ def test_h2o32_mixed_precision_error_real_chemistry():
     Bmx, _ = corr_amd.ri_bia_from_j(r["C"], r["Pmn"], r["J"], o, "cholesky", "bf16x3")
     rel = ((Bmx - B64).abs().max() / B64.abs().max()).item()
     eb = corr_amd.rimp2_energy(Bmx, eo, ev, "fp64")["EMP2"]
-    emx = corr_amd.rimp2_energy(Bmx, eo, ev, "bf16x3")["EMP2"]
+    emx_all = corr_amd.rimp2_energy(Bmx, eo, ev, "bf16x3")
+    emx = emx_all["EMP2"]
     del Bmx
+    # the bench's step: B formation and energy in one call, the metric product
+    # writing the energy kernel's BF16 panels; the same bits as the two calls
+    fused, kept = corr_amd.ri_mp2_energy_from_j(r["C"], r["Pmn"], r["J"], o, eo, ev, "cholesky", "bf16x3")
+    assert kept == 2432 and fused == emx_all
     say(f"mixed B formation: max |dB| / max |B| = {rel:.3e}, FP64 EMP2 moves {abs(eb - e64) * 1e6:.4f} uEh")
-    say(f"EMP2 fp64 {e64:.10f} mixed {emx:.10f} error {abs(emx - e64) * 1e6:.4f} uEh")
+    say(f"EMP2 fp64 {e64:.10f} mixed {emx:.10f} error {abs(emx - e64) * 1e6:.4f} uEh (one-call entry: same bits)")
     assert -0.22 * 32 < e64 < -0.19 * 32
     assert rel <= 1e-6 and abs(eb - e64) <= 2e-8
     assert abs(emx - e64) <= 1e-6
