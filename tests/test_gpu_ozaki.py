@@ -32,11 +32,13 @@ def corr():
 
 
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (37, 53, 29), (128, 128, 32), (130, 257, 70), (300, 64, 400),
-                                   (257, 384, 1000), (70, 200, 1500), (40, 150, 2500), (33, 129, 3100)])
+                                   (257, 384, 1000), (70, 200, 1500), (40, 150, 2500), (33, 129, 3100),
+                                   (48, 96, 1024), (48, 96, 1025), (40, 64, 2048), (40, 64, 2049),
+                                   (24, 40, 3072), (24, 40, 3073)])
 def test_gemm_int8x5_bit_exact(corr, M, N, K):
     """B (k-strided columns) goes through the one-pass slicing kernel held in
-    1, 2 or 3 chunk rounds for K <= 1024, 2048, 3072, and through the two-pass
-    kernel above (K = 3100)."""
+    1, 2 or 3 chunk rounds for K <= 1024, 2048, 3072 (each bound and one past
+    it tested), and through the two-pass kernel above."""
     _say(f"gemm int8x5 M={M} N={N} K={K}")
     rng = np.random.default_rng(M * 7 + N + K)
     A = rng.standard_normal((M, K)) * 10.0 ** rng.uniform(-6, 0, (M, K))
