@@ -144,6 +144,7 @@ def test_mixed_b_formation(corr, naux, N, o):
     (150, 61, 7, "cholesky", "bf16x3"),     # ragged o, v and naux
     (300, 90, 13, "cholesky", "bf16x3"),    # several row and column tiles
     (150, 61, 7, "cholesky", "fp64"),
+    (150, 61, 7, "cholesky", "fp32"),
     (150, 61, 7, "eigen", "bf16x3"),
 ])
 def test_ri_mp2_energy_from_j(corr, naux, N, o, mode, prec):
